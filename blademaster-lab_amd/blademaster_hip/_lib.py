@@ -1,0 +1,139 @@
+"""ctypes binding of libbm_hip.so (declared in include/bm_hip.h).
+
+The library must exist next to this file (built by `__graft_entry__.build()` or
+`make -C blademaster-lab_amd/csrc`); there is no fallback implementation, so a missing or
+stale library raises immediately.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libbm_hip.so")
+
+c_f32p = ctypes.POINTER(ctypes.c_float)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("node_count", ctypes.c_int64),
+        ("fields", ctypes.c_int64),
+        ("dims", ctypes.c_int64 * 3),
+        ("pair_capacity", ctypes.c_int64),
+        ("list_buckets", ctypes.c_int64),
+        ("list_lanes", ctypes.c_int64),
+        ("tool_model", ctypes.c_int64),
+        ("free_blade", ctypes.c_int64),
+        ("lock_rotation", ctypes.c_int64),
+        ("predict", ctypes.c_int64),
+        ("plastic", ctypes.c_int64),
+    ] + [
+        (name, ctypes.c_double)
+        for name in (
+            "dt spacing mass volume radius blade_offset mu lam yield_stress flip "
+            "sn gk mus mud gc board_sn threshold"
+        ).split()
+    ] + [
+        ("half", ctypes.c_double * 3),
+        ("bevel", ctypes.c_double),
+        ("blade_mass", ctypes.c_double),
+        ("inertia_inv", ctypes.c_double * 3),
+        ("board_half", ctypes.c_double * 2),
+        ("drive_k", ctypes.c_double),
+        ("drive_c", ctypes.c_double),
+        ("drive_limit", ctypes.c_double),
+        ("rot_k", ctypes.c_double),
+        ("rot_c", ctypes.c_double),
+        ("pair_skin", ctypes.c_double),
+    ]
+
+
+STATE_FIELDS = [
+    ("x", c_f32p), ("v", c_f32p), ("F", c_f32p), ("Fp", c_f32p), ("stress", c_f32p), ("damage", c_f32p),
+    ("ref", c_f32p), ("local", c_f32p), ("impulses", c_f32p), ("histories", c_f32p),
+    ("codes", c_i64p), ("transfer_codes", c_i64p), ("base_node", c_i32p), ("field_idx", c_u8p),
+    ("field_code", c_i64p), ("node_nf", c_i32p),
+    ("grid_mass", c_f32p), ("momentum", c_f32p), ("grid_impulse", c_f32p), ("old_v", c_f32p), ("new_v", c_f32p),
+    ("remapped", c_f32p), ("grid_vel", c_f32p),
+    ("tool_pos", c_f32p), ("tool_vel", c_f32p), ("tool_rot", c_f32p), ("tool_omega", c_f32p), ("mobility", c_f32p),
+    ("reaction", c_f32p), ("torque", c_f32p), ("board", c_f32p), ("work", c_f32p), ("speed", c_f32p),
+    ("table", c_f32p), ("interval_impulse", c_f32p), ("interval_angular", c_f32p),
+    ("errors", c_i32p), ("topology_dirty", c_i32p),
+    ("pair_keys", c_i64p), ("pair_history", c_f32p), ("pair_used", c_i32p), ("pair_count", c_i32p),
+    ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
+    ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("bucket_count", c_i32p),
+    ("bucket_start", c_i32p), ("scan_partial", c_i32p), ("particle_bucket", c_i32p), ("particle_rank", c_i32p),
+    ("sorted_x", c_f32p),
+]
+
+
+class State(ctypes.Structure):
+    _fields_ = STATE_FIELDS
+
+
+P = ctypes.POINTER(Params)
+S = ctypes.POINTER(State)
+VP = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+F32 = ctypes.c_float
+
+# name -> argument types (all return int)
+SIGNATURES = {
+    "bm_version": [],
+    "bm_refresh_stress": [P, S, VP],
+    "bm_clear_step": [P, S, VP],
+    "bm_rigid_contacts": [P, S, I32, VP],
+    "bm_pair_contacts": [P, S, I32, VP],
+    "bm_register": [P, S, VP],
+    "bm_p2g": [P, S, VP],
+    "bm_couple_and_velocity": [P, S, VP],
+    "bm_g2p_remap": [P, S, VP],
+    "bm_couple_remapped": [P, S, VP],
+    "bm_update_F": [P, S, VP],
+    "bm_material_step": [P, S, VP],
+    "bm_tool_step": [P, S, VP, VP, I32, VP],
+    "bm_finish_external": [P, S, VP],
+    "bm_accumulate_reaction": [P, S, VP],
+    "bm_assign": [P, S, I64, I32, VP],
+    "bm_substep": [P, S, I32, I32, I64, I32, I32, VP, VP, I32, VP],
+    "bm_load_external_state": [P, S, VP, VP],
+    "bm_record_pose": [P, S, I32, VP, VP],
+    "bm_store_interval": [P, S, VP, VP],
+    "bm_validate_interval": [P, S, VP],
+    "bm_count_embedded": [P, S, VP, VP],
+    "bm_metrics": [P, S, VP, VP],
+    "bm_plastic_project": [P, S, VP],
+    "bm_probe_blade_query": [VP, ctypes.POINTER(ctypes.c_float), F32, VP, I32, VP],
+    "bm_probe_friction": [VP, F32, F32, F32, F32, F32, VP, I32, VP],
+    "bm_probe_svd3": [VP, VP, VP, VP, I32, VP],
+}
+
+_lib = None
+
+
+def load():
+    """Load the library once and declare every entry point; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY_PATH):
+        raise RuntimeError(f"HIP library not built: {LIBRARY_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(code, name):
+    if code < 0:
+        raise ValueError(f"{name}: invalid argument ({code})")
+    if code > 0:
+        raise RuntimeError(f"{name}: HIP launch failed with error {code}")

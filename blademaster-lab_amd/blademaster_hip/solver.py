@@ -1,0 +1,494 @@
+"""TLMPM material solver with a rigid blade; every per-substep stage runs in libbm_hip.
+
+Keeps the `Solver` surface of `blademaster_warp/solver.py:157-1051` (constructor, buffers
+read by callers, `step`, `advance`, `prepare`, `state`, stage methods) so experiment
+drivers call it unchanged.  Device buffers are PyTorch-ROCm tensors wrapped in `DArray`;
+each stage is one call into the C-ABI on torch's current stream, and `prepare` records a
+batch of substeps into a HIP graph (torch.cuda.CUDAGraph) keyed like the reference's graph
+cache (solver.py:995).  The reference's conditional graph nodes are replaced by kernels
+that read their device flag (topology_dirty, pair_dirty) and return early.
+"""
+
+import ctypes
+import time
+import numpy as np
+import torch
+
+from . import _lib
+from .arrays import DArray
+from .config import Config, derived, max_cut_index
+from .geometry import points, stencil, compact_stencil
+
+# Listed pair partners per particle before the bucket fallback (solver.py:152-154).
+PAIR_LIST_LANES = 16
+CUT_ON, CUT_ADVANCE, CUT_HOLD = 1, 2, 4
+
+
+def _pow2_at_least(v):
+    return 1 << (max(int(v), 16) - 1).bit_length()
+
+
+class _Device:
+    def __init__(self, device):
+        self.torch = torch.device(device)
+        self.name = str(self.torch)
+        self.is_cuda = self.torch.type == "cuda"
+
+
+class Solver:
+    def __init__(self, config=Config(), device="cuda:0"):
+        self.config = cfg = config
+        self.device = _Device(device)
+        if not self.device.is_cuda or not torch.cuda.is_available():
+            raise ValueError("CUDA device required")
+        if cfg.tool_count != 1:
+            raise ValueError("tool_count > 1 (several free knives) is not provided by this build")
+        self._lib = _lib.load()
+        dev = self.device.torch
+        self.reference, self.origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+        self.cells = np.asarray(cells)
+        self.n = n = len(self.reference)
+        self.node_numpy, self.weight_numpy, self._gradient_numpy = stencil(
+            self.reference, self.origin, cfg.grid_spacing, cells
+        )
+        self.dims = self.cells + 1
+        self.node_count = int(np.prod(self.dims))
+        if n and (self.node_numpy.min() < 0 or self.node_numpy.max() >= self.node_count):
+            raise ValueError("Reference stencil leaves the grid")
+        self.max_cut_index = max_cut_index(self.node_count)
+        # Sizes kept from the reference for callers that read them (solver.py:177-180).
+        self.capacity = _pow2_at_least(16 * n)
+        self.pair_capacity = _pow2_at_least(64 * n)
+        self.fields = int(cfg.fields_per_node)
+        d = derived(cfg)
+        self.volume, self.mass, self.radius = d["volume"], d["mass"], d["radius"]
+        self.blade_offset = d["blade_offset"]
+        self.mu, self.lam = d["mu"], d["lam"]
+        self.threshold = d["threshold"]
+        self.normal_scale, self.tangent_stiffness = d["normal_scale"], d["tangent_stiffness"]
+        self.board_scale = d["board_scale"]
+        self.inertia_inv = d["inertia_inv"]
+        self.maxwell = d["maxwell"]
+        if self.maxwell:
+            raise ValueError("Maxwell viscoelasticity is not provided by this build")
+        if cfg.dynamic_viscosity:
+            raise ValueError("dynamic_viscosity > 0 is not provided by this build")
+        self.compact = cfg.grid_storage == "compact"
+
+        f32, i64, i32 = torch.float32, torch.int64, torch.int32
+
+        def t(shape, dtype=f32, value=0):
+            return torch.full(shape, value, dtype=dtype, device=dev)
+
+        def arr(tensor, hook=None):
+            return DArray(tensor, hook)
+
+        base_node, local, _ = compact_stencil(self.reference, self.origin, cfg.grid_spacing, cells)
+        eye = torch.eye(3, dtype=f32, device=dev).expand(n, 3, 3).contiguous()
+        self._t = {}
+        T = self._t
+        T["x"] = torch.tensor(self.reference, dtype=f32, device=dev).reshape(n, 3)
+        T["ref"] = T["x"].clone()
+        T["v"] = t((n, 3))
+        T["F"] = eye.clone()
+        T["Fp"] = eye.clone()
+        T["stress"] = t((n, 3, 3))
+        T["damage"] = t((n,))
+        T["codes"] = t((n,), i64)
+        T["single_codes"] = t((n,), i64)
+        T["local"] = torch.tensor(local, device=dev).reshape(n, 3)
+        T["base_node"] = torch.tensor(base_node, device=dev)
+        T["field_idx"] = t((n, 8), torch.uint8)
+        T["impulses"] = t((n, 3))
+        T["histories"] = t((2, n, 3))
+        F_, N_ = self.fields, self.node_count
+        T["field_code"] = t((F_, N_), i64, -1)
+        T["node_nf"] = t((N_,), i32)
+        T["grid_mass"] = t((F_, N_))
+        for name in ("momentum", "grid_impulse", "old_v", "new_v", "remapped", "grid_vel"):
+            T[name] = t((F_, N_, 3))
+        hz = cfg.blade_half_size[2]
+        T["tool_pos"] = torch.tensor([[0.0, 0.0, cfg.size[2] + 0.012 + hz]], dtype=f32, device=dev)
+        T["tool_vel"] = t((1, 3))
+        T["tool_rot"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]], dtype=f32, device=dev)
+        T["tool_omega"] = t((1, 3))
+        T["tool_mobility"] = t((6, 6))
+        T["reaction"] = t((1, 3))
+        T["torque"] = t((1, 3))
+        T["board"] = t((1, 3))
+        T["work"] = t((1,))
+        T["speed"] = t((1,))
+        T["table"] = t((1, 3))
+        T["interval_impulse"] = t((1, 3))
+        T["interval_angular"] = t((1, 3))
+        T["errors"] = t((3,), i32)
+        T["topology_dirty"] = t((1,), i32, 1)
+        # pair contacts
+        P_ = self.pair_capacity
+        self.list_buckets = _pow2_at_least(2 * n)
+        T["pair_keys"] = t((2, P_), i64, -1)
+        T["pair_history"] = t((2, P_, 3))
+        T["pair_used"] = t((2, P_), i32)
+        T["pair_count"] = t((2,), i32)
+        T["pair_dirty"] = t((1,), i32, 1)
+        T["pair_builds"] = t((1,), i32)
+        T["list_reach"] = t((1,))
+        T["built_x"] = t((n, 3))
+        T["built_codes"] = t((n,), i64)
+        T["partners"] = t((max(n, 1) * PAIR_LIST_LANES,), i32)
+        T["partner_count"] = t((n,), i32)
+        T["bucket_count"] = t((self.list_buckets + 1,), i32)
+        T["bucket_start"] = t((self.list_buckets + 1,), i32)
+        T["scan_partial"] = t(((self.list_buckets + 4095) // 4096 + 1,), i32)
+        T["particle_bucket"] = t((n,), i32)
+        T["particle_rank"] = t((n,), i32)
+        T["sorted_x"] = t((n, 4))
+        T["single_target"] = t((1, 3))
+        T["single_rotation"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]], dtype=f32, device=dev)
+
+        mark_topology = self._mark_topology
+        self.x, self.v = arr(T["x"]), arr(T["v"])
+        self.ref = arr(T["ref"])
+        self.F, self.Fp, self.stress = arr(T["F"]), arr(T["Fp"]), arr(T["stress"])
+        self.codes = arr(T["codes"], mark_topology)
+        self.single_field_codes = arr(T["single_codes"], mark_topology)
+        self.damage = arr(T["damage"])
+        self.tool_damage = [self.damage]
+        self.impulses, self.histories = arr(T["impulses"]), arr(T["histories"])
+        self.tool_pos, self.tool_vel = arr(T["tool_pos"]), arr(T["tool_vel"])
+        self.tool_rot, self.tool_omega = arr(T["tool_rot"]), arr(T["tool_omega"])
+        self.tool_mobility = arr(T["tool_mobility"])
+        self.reaction, self.torque, self.board = arr(T["reaction"]), arr(T["torque"]), arr(T["board"])
+        self.work, self.speed = arr(T["work"]), arr(T["speed"])
+        self.tool_table_impulse = arr(T["table"])
+        self.interval_impulse = arr(T["interval_impulse"])
+        self.interval_angular_impulse = arr(T["interval_angular"])
+        self.errors = arr(T["errors"])
+        self.topology_dirty = arr(T["topology_dirty"])
+        self.pair_count = [arr(T["pair_count"][0:1]), arr(T["pair_count"][1:2])]
+        self.pair_list_builds = arr(T["pair_builds"])
+        self.pair_list_dirty = arr(T["pair_dirty"])
+        self.grid_mass = arr(T["grid_mass"])
+        self.new_v = arr(T["new_v"])
+        self.old_v = arr(T["old_v"])
+        self.single_target = arr(T["single_target"])
+        self.single_rotation = arr(T["single_rotation"])
+        self.pair_skin = cfg.pair_skin or self.radius
+        self.pair_reach = 2 * self.radius + self.pair_skin
+        self._single_field = False
+        self.step_count = 0
+        self.cut_index = 0
+        self._batches = {}
+        self.graph_build_s = 0.0
+        self.tool_views = [
+            dict(pos=self.tool_pos, vel=self.tool_vel, rot=self.tool_rot, omega=self.tool_omega,
+                 reaction=self.reaction, torque=self.torque, table=self.tool_table_impulse)
+        ]
+        self._params = self._make_params()
+        self._state = self._make_state()
+        self.blade_constraint = None
+        self.refresh_stress()
+
+    # ------------------------------------------------------------------ C-ABI plumbing
+    def _make_params(self):
+        cfg = self.config
+        p = _lib.Params()
+        p.n = self.n
+        p.node_count = self.node_count
+        p.fields = self.fields
+        for i in range(3):
+            p.dims[i] = int(self.dims[i])
+            p.half[i] = cfg.blade_half_size[i]
+            p.inertia_inv[i] = float(self.inertia_inv[i])
+        p.pair_capacity = self.pair_capacity
+        p.list_buckets = self.list_buckets
+        p.list_lanes = PAIR_LIST_LANES
+        p.tool_model = self._tool_model()
+        p.free_blade = int(cfg.free_blade)
+        p.lock_rotation = int(not (cfg.free_blade or cfg.pose_servo))
+        p.predict = int(cfg.tool_substep == "predict")
+        p.plastic = int(cfg.yield_stress > 0)
+        p.dt, p.spacing, p.mass, p.volume = cfg.dt, cfg.grid_spacing, self.mass, self.volume
+        p.radius, p.blade_offset, p.mu, p.lam = self.radius, self.blade_offset, self.mu, self.lam
+        p.yield_stress, p.flip = cfg.yield_stress, cfg.flip_fraction
+        p.sn, p.gk = self.normal_scale, self.tangent_stiffness
+        p.mus, p.mud, p.gc = cfg.friction_static, cfg.friction_dynamic, cfg.tangent_damping
+        p.board_sn, p.threshold = self.board_scale, self.threshold
+        p.bevel, p.blade_mass = cfg.blade_bevel_height, cfg.blade_mass
+        p.board_half[0], p.board_half[1] = cfg.board_half_extent
+        p.drive_k, p.drive_c, p.drive_limit = cfg.drive_stiffness, cfg.drive_damping, cfg.drive_force_limit
+        p.rot_k, p.rot_c = cfg.rotational_stiffness, cfg.rotational_damping
+        p.pair_skin = self.pair_skin
+        return p
+
+    def _make_state(self):
+        s = _lib.State()
+        T = self._t
+        for field, ctype in _lib.STATE_FIELDS:
+            key = field
+            if field == "mobility":
+                key = "tool_mobility"
+            elif field == "transfer_codes":
+                key = "single_codes" if self._single_field_flag() else "codes"
+            tensor = T[key]
+            setattr(s, field, ctypes.cast(ctypes.c_void_p(tensor.data_ptr()), ctype))
+        return s
+
+    def _single_field_flag(self):
+        return getattr(self, "_single_field", False)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device.torch).cuda_stream)
+
+    def _call(self, name, *args):
+        code = getattr(self._lib, name)(ctypes.byref(self._params), ctypes.byref(self._state), *args, self._stream())
+        _lib.check(code, name)
+
+    def _mark_topology(self):
+        self._t["topology_dirty"].fill_(1)
+
+    def launch(self, name, *args):
+        """Launch one C-ABI stage by name with the solver's params/state (cf. solver.py:331)."""
+        self._call(name, *args)
+
+    # ------------------------------------------------------------------ stages
+    def refresh_stress(self):
+        """Recompute the stored stress after host writes to F or Fp (solver.py:318-329)."""
+        self._call("bm_refresh_stress")
+
+    def material_step(self):
+        """Registration when dirty, P2G, exchange, G2P + MUSL, F update (solver.py:334-376)."""
+        self._call("bm_material_step")
+
+    @property
+    def single_field(self):
+        return self._single_field
+
+    def set_single_field(self, enabled):
+        """Paper Fig. 4 baseline: grid transfers ignore the cut codes (solver.py:382-389)."""
+        if bool(enabled) != self._single_field:
+            self._single_field = bool(enabled)
+            self._state = self._make_state()
+            self._mark_topology()
+
+    def _register_topology(self):
+        self._mark_topology()
+        self._call("bm_register")
+
+    def _tool_model(self):
+        """-1 articulated (external), 0 guided translation, 1 free rigid blade (solver.py:571-576)."""
+        cfg = self.config
+        if cfg.external_tool:
+            return -1
+        return int(cfg.free_blade or cfg.pose_servo)
+
+    def _cut_bits(self, cutting):
+        if not cutting:
+            return 0
+        cfg = self.config
+        return CUT_ON | (CUT_ADVANCE if cfg.cut_work == "advance" else 0) | (CUT_HOLD if cfg.cut_hold else 0)
+
+    def rigid_contacts(self, cutting=True):
+        """Gravity plus board and blade contact impulses on every particle (solver.py:588-630)."""
+        self._call("bm_rigid_contacts", self._cut_bits(cutting))
+
+    def particle_contacts(self):
+        """Contacts between particles on incompatible sides of a cut (solver.py:670-751)."""
+        self._call("bm_pair_contacts", self.step_count % 2)
+
+    def assign_sides(self, cut_index, cutting=True):
+        """Side digit 3^cut_index for particles crossed by the blade edge (solver.py:866-890)."""
+        self._call("bm_assign", 3**cut_index, int(cutting))
+
+    def step(self, target_edge=(0.0, 0.0, 0.042), cutting=True, cut_index=0, orientation=None):
+        if self.config.external_tool:
+            raise ValueError("Use ExternalToolPort.advance for an externally owned tool")
+        cut_index = self._set_cut(cut_index)
+        self._step_kernels(target_edge, cutting, cut_index, orientation=orientation)
+
+    def _set_cut(self, cut_index):
+        if isinstance(cut_index, (bool, np.bool_)) or not isinstance(cut_index, (int, np.integer)):
+            raise ValueError("Cut index must be an integer")
+        cut_index = int(cut_index)
+        last = self.max_cut_index - (self.config.tool_count - 1)
+        if cut_index < self.cut_index or cut_index > last:
+            raise ValueError(f"Cut indices must be monotonic and within [0, {last}]")
+        if cut_index != self.cut_index:
+            for damage in self.tool_damage:
+                damage.zero_()
+            self.cut_index = cut_index
+        return cut_index
+
+    def _drive_tool(self, target_edge, orientation, buffer, rotations, index):
+        """Servo, free or guided knife step with the shared board contact (solver.py:824-864)."""
+        if buffer is None:
+            self.single_target.assign([target_edge])
+            self.single_rotation.assign([orientation if orientation is not None else [0.0, 0.0, 0.0, 1.0]])
+            buffer, rotations, index = self._t["single_target"], self._t["single_rotation"], 0
+        self._call("bm_tool_step", ctypes.c_void_p(buffer.data_ptr()), ctypes.c_void_p(rotations.data_ptr()), index)
+
+    def _step_kernels(self, target_edge, cutting, cut_index, buffer=None, index=0, rotations=None, orientation=None):
+        """One substep: contacts, material update, tool update, side labels (solver.py:920-940)."""
+        cfg = self.config
+        if buffer is None and not cfg.external_tool:
+            self.single_target.assign([target_edge])
+            self.single_rotation.assign([orientation if orientation is not None else [0.0, 0.0, 0.0, 1.0]])
+            buffer, rotations, index = self._t["single_target"], self._t["single_rotation"], 0
+        tptr = ctypes.c_void_p(buffer.data_ptr()) if buffer is not None else ctypes.c_void_p(
+            self._t["single_target"].data_ptr())
+        rptr = ctypes.c_void_p(rotations.data_ptr()) if rotations is not None else ctypes.c_void_p(
+            self._t["single_rotation"].data_ptr())
+        self._call(
+            "bm_substep",
+            self._cut_bits(cutting),
+            int(bool(cutting)),
+            3**cut_index,
+            self.step_count % 2,
+            int(cfg.accumulate_tool_impulses),
+            tptr,
+            rptr,
+            int(index),
+        )
+        self.step_count += 1
+
+    def advance(self, targets, cutting=True, cut_index=0, graph=True, orientations=None):
+        """Submit exact per-substep edge targets (solver.py:942-973)."""
+        if self.config.external_tool:
+            raise ValueError("Use ExternalToolPort.advance for an externally owned tool")
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        if targets.ndim != 2 or targets.shape[1] != 3 or not np.isfinite(targets).all() or not len(targets):
+            raise ValueError("targets must be finite (substeps, 3) edge positions")
+        orientations = (
+            np.tile([0.0, 0.0, 0.0, 1.0], (len(targets), 1)) if orientations is None else np.asarray(orientations)
+        )
+        if orientations.shape != (len(targets), 4) or not np.isfinite(orientations).all():
+            raise ValueError("orientations must be finite (substeps, 4) xyzw quaternions")
+        lengths = np.linalg.norm(orientations, axis=1)
+        if np.any(lengths < 1.0e-6):
+            raise ValueError("Zero quaternion")
+        orientations = np.ascontiguousarray(orientations / lengths[:, None], dtype=np.float32)
+        cut_index = self._set_cut(cut_index)
+        if not graph:
+            for target, orientation in zip(targets, orientations):
+                self._step_kernels(target, cutting, cut_index, orientation=orientation)
+            return
+        buffer, rotations, captured = self.prepare(targets, cutting, cut_index, orientations)
+        buffer.copy_(torch.from_numpy(targets).to(buffer.device))
+        rotations.copy_(torch.from_numpy(orientations).to(rotations.device))
+        captured.replay()
+        self.step_count += len(targets)
+
+    def prepare(self, targets, cutting=True, cut_index=0, orientations=None, before=(), after=(), tag=None,
+                after_step=None):
+        """Record a batch of substeps into a HIP graph without advancing state (solver.py:975-1019).
+
+        `before` and `after` are callables run around the substeps during capture (each
+        issues C-ABI launches on the current stream); `tag` names that wrapping in the cache
+        key.
+        """
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        if orientations is None:
+            orientations = np.tile([0.0, 0.0, 0.0, 1.0], (len(targets), 1))
+        key = (len(targets), bool(cutting), cut_index, self.step_count % 2, self._single_field, tag)
+        if key not in self._batches:
+            start = time.perf_counter()
+            dev = self.device.torch
+            buffer = torch.tensor(targets, dtype=torch.float32, device=dev)
+            rotations = torch.tensor(np.asarray(orientations, dtype=np.float32), device=dev)
+            count = self.step_count
+            graph = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize(dev)
+            try:
+                with torch.cuda.graph(graph):
+                    for fn in before:
+                        fn()
+                    for i in range(len(targets)):
+                        self._step_kernels(targets[i], cutting, cut_index, buffer, i, rotations)
+                        if after_step is not None:
+                            after_step(i)
+                    for fn in after:
+                        fn()
+            finally:
+                self.step_count = count
+            self._batches[key] = buffer, rotations, graph
+            self.graph_build_s += time.perf_counter() - start
+        return self._batches[key]
+
+    # ------------------------------------------------------------------ readback
+    def state(self):
+        """Host copies of the particle and blade state; raises on device errors (solver.py:1021-1051)."""
+        errors = self.errors.numpy()
+        if errors.any():
+            raise RuntimeError(
+                f"GPU errors [storage overflow, inverted material, coincident proxies]: {errors.tolist()}"
+            )
+        state = {
+            "position": self.x.numpy(),
+            "velocity": self.v.numpy(),
+            "deformation": self.F.numpy(),
+            "plastic_deformation": self.Fp.numpy(),
+            "sides": self.codes.numpy(),
+            "damage": self.damage.numpy(),
+            "blade_position": self.tool_pos.numpy(),
+            "blade_velocity": self.tool_vel.numpy(),
+            "blade_orientation": self.tool_rot.numpy(),
+            "blade_omega": self.tool_omega.numpy(),
+        }
+        state["time"] = self.step_count * self.config.dt
+        if any(not np.isfinite(v).all() for v in state.values()):
+            raise RuntimeError("Nonfinite state; run is invalid")
+        if np.linalg.det(state["deformation"]).min() <= 0:
+            raise RuntimeError("Inverted material; run is invalid")
+        return state
+
+    def embedded_particles(self):
+        """Particles beside the blade body (solver.py:892-900)."""
+        count = torch.zeros(1, dtype=torch.int32, device=self.device.torch)
+        self._call("bm_count_embedded", ctypes.c_void_p(count.data_ptr()))
+        return int(count.item())
+
+    # ------------------------------------------------------------------ field-table views
+    @property
+    def weights(self):
+        return DArray(torch.tensor(self.weight_numpy, dtype=torch.float32, device=self.device.torch))
+
+    @property
+    def nodes(self):
+        return DArray(torch.tensor(self.node_numpy, dtype=torch.int32, device=self.device.torch))
+
+    @property
+    def gradients(self):
+        return DArray(torch.tensor(self._gradient_numpy, dtype=torch.float32, device=self.device.torch))
+
+    def field_table(self):
+        """(codes [F, N] int64 with -1 for empty slots, used slots per node [N])."""
+        return self._t["field_code"].cpu().numpy(), self._t["node_nf"].cpu().numpy()
+
+    @property
+    def active_count(self):
+        _, nf = self.field_table()
+        return DArray(torch.tensor([int(nf.sum())], dtype=torch.int32))
+
+    @property
+    def grid_keys(self):
+        """Registered field keys code * node_count + node, ordered by slot then node."""
+        table, _ = self.field_table()
+        f, node = np.nonzero(table >= 0)
+        keys = table[f, node] * self.node_count + node
+        return DArray(torch.tensor(keys, dtype=torch.int64))
+
+    @property
+    def keys(self):
+        return self.grid_keys
+
+    @property
+    def slots(self):
+        """Global field slot (f * node_count + node) of every (particle, corner)."""
+        idx = self._t["field_idx"].cpu().numpy().astype(np.int64)
+        return DArray(torch.tensor(idx * self.node_count + self.node_numpy, dtype=torch.int64))
+
+    def corner_keys(self):
+        """Key code * node_count + node of the field each (particle, corner) transfers into."""
+        table, _ = self.field_table()
+        idx = self._t["field_idx"].cpu().numpy().astype(np.int64)
+        return table[idx, self.node_numpy] * self.node_count + self.node_numpy

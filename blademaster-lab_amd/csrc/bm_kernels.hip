@@ -1,0 +1,1488 @@
+// BladeMaster TLMPM cutting substep for MI355X (gfx950): HIP kernels and the C-ABI.
+//
+// Design (differences from the reference Warp kernels, same physics):
+//  * Grid fields live in a node-major table of F slots per node (field_code[f][node]); the
+//    field of particle p at stencil corner a is the slot holding its transfer code at that
+//    node.  It replaces the reference's int64 open-addressing key table, compaction and
+//    per-node linked lists (material.py:37-75, 621-653): the compatible-field exchange
+//    walks at most F slots of one node instead of a list.
+//  * The stencil is rebuilt in registers from the base node and three local fractions (16 B
+//    per particle) instead of loading 8 node ids, 8 weights and 24 gradient terms (128 B).
+//  * Scalar reductions onto one address (speed, reaction, torque, board, work) are reduced
+//    across the wavefront first, so one lane per wave issues the global atomic.
+//  * Pair contacts use a sort-based cell list (bucket counting sort + scan) rebuilt on the
+//    device when the list is stale, instead of per-bucket linked lists.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bm_device.h"
+#include "bm_hip.h"
+
+using namespace bm;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int blocks_for(int64_t n, int block = kBlock) {
+  int64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  return int(b);
+}
+
+// ------------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ bool lane0() { return (threadIdx.x & 63) == 0; }
+
+// atomicMax on non-negative floats through their integer bit patterns.
+__device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {
+  atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
+}
+// Wave-reduced add of a vec3 (all lanes of the wave must call it).
+__device__ __forceinline__ void wave_add3(float* addr, f3 v) {
+  float a = wave_sum(v.x), b = wave_sum(v.y), c = wave_sum(v.z);
+  if (lane0() && (a != 0.0f || b != 0.0f || c != 0.0f)) {
+    atomicAdd(addr, a);
+    atomicAdd(addr + 1, b);
+    atomicAdd(addr + 2, c);
+  }
+}
+
+// ------------------------------------------------------------------------ contact laws
+struct Contact {
+  float dt, sn, mus, mud, gk, gc;
+};
+
+// friction (contact.py:21-43): trial tangential impulse from the history spring and
+// damper, Coulomb cap; returns impulse and the new history.
+__device__ __forceinline__ void friction(const Contact& c, f3 n, f3 relative, float jn, f3 history, float me,
+                                         f3& jt, f3& ht) {
+  f3 vt = relative - dot(relative, n) * n;
+  f3 htan = history - dot(history, n) * n;
+  float k = c.gk * me / (c.dt * c.dt);
+  float damping = 2.0f * c.gc * sqrtf(me * k);
+  f3 trial = ((-k) * htan - damping * vt) * c.dt;
+  float magnitude = len(trial);
+  if (magnitude <= c.mus * jn) {
+    jt = trial;
+    ht = htan + c.dt * vt;
+    return;
+  }
+  f3 applied = trial * (c.mud * jn / magnitude);
+  jt = applied;
+  ht = -((applied / c.dt + damping * vt) / k);
+}
+
+// penalty_impulse (contact.py:123-147)
+__device__ __forceinline__ void penalty(const Contact& c, float gap, f3 relative, f3 n, float me, f3 history,
+                                        float sn, f3& j, f3& h) {
+  float predicted = gap + c.dt * fminf(dot(relative, n), 0.0f);
+  if (predicted >= 0.0f) {
+    j = f3{0.f, 0.f, 0.f};
+    h = f3{0.f, 0.f, 0.f};
+    return;
+  }
+  float jn = sn * me / c.dt * (-predicted);
+  f3 jt, ht;
+  friction(c, n, relative, jn, history, me, jt, ht);
+  j = jn * n + jt;
+  h = ht;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// blade_query (contact.py:46-120): closest point, outward normal and signed distance of a
+// point in blade coordinates to the (optionally beveled) box.
+__device__ inline void blade_query(f3 local, f3 hf, float bevel, f3& closest, f3& normal, float& distance) {
+  closest = f3{clampf(local.x, -hf.x, hf.x), clampf(local.y, -hf.y, hf.y), clampf(local.z, -hf.z, hf.z)};
+  f3 delta = local - closest;
+  distance = len(delta);
+  normal = f3{0.f, 0.f, 0.f};
+  if (bevel == 0.0f) {
+    if (distance > 1.0e-12f) {
+      normal = delta / distance;
+    } else {
+      f3 margin = f3{hf.x - fabsf(local.x), hf.y - fabsf(local.y), hf.z - fabsf(local.z)};
+      int axis = 0;
+      float m = margin.x;
+      if (margin.y < m) {
+        axis = 1;
+        m = margin.y;
+      }
+      if (margin.z < m) {
+        axis = 2;
+        m = margin.z;
+      }
+      float sgn = comp(local, axis) < 0.0f ? -1.0f : 1.0f;
+      if (axis == 0) {
+        normal.x = sgn;
+        closest.x = sgn * hf.x;
+      } else if (axis == 1) {
+        normal.y = sgn;
+        closest.y = sgn * hf.y;
+      } else {
+        normal.z = sgn;
+        closest.z = sgn * hf.z;
+      }
+      distance = -m;
+    }
+    return;
+  }
+  float qx = fabsf(local.x), qz = local.z;
+  float sign = local.x < 0.0f ? -1.0f : 1.0f;
+  float edge_len_sq = hf.x * hf.x + bevel * bevel;
+  float t = clampf((qx * hf.x + (qz + hf.z) * bevel) / edge_len_sq, 0.0f, 1.0f);
+  float nx = t * hf.x, nz = -hf.z + t * bevel;
+  float best_sq = (qx - nx) * (qx - nx) + (qz - nz) * (qz - nz);
+  float il = sqrtf(edge_len_sq);
+  float ox = bevel / il, oz = -hf.x / il;
+  float cx = hf.x, cz = clampf(qz, -hf.z + bevel, hf.z);
+  float dsq = (qx - cx) * (qx - cx) + (qz - cz) * (qz - cz);
+  if (dsq < best_sq) {
+    best_sq = dsq;
+    nx = cx;
+    nz = cz;
+    ox = 1.0f;
+    oz = 0.0f;
+  }
+  cx = clampf(qx, 0.0f, hf.x);
+  cz = hf.z;
+  dsq = (qx - cx) * (qx - cx) + (qz - cz) * (qz - cz);
+  if (dsq < best_sq) {
+    best_sq = dsq;
+    nx = cx;
+    nz = cz;
+    ox = 0.0f;
+    oz = 1.0f;
+  }
+  bool inside = qx <= hf.x && qz <= hf.z && hf.x * (qz + hf.z) >= bevel * qx;
+  float best = sqrtf(best_sq);
+  nx = sign * nx;
+  ox = sign * ox;
+  closest = f3{nx, clampf(local.y, -hf.y, hf.y), nz};
+  if (inside && fabsf(local.y) <= hf.y) {
+    normal = f3{ox, 0.0f, oz};
+    distance = -best;
+    if (hf.y - fabsf(local.y) < best) {
+      float sy = local.y < 0.0f ? -1.0f : 1.0f;
+      closest = f3{local.x, sy * hf.y, local.z};
+      normal = f3{0.0f, sy, 0.0f};
+      distance = -(hf.y - fabsf(local.y));
+    }
+  } else {
+    if (inside) {
+      closest.x = local.x;
+      closest.z = local.z;
+    }
+    delta = local - closest;
+    distance = len(delta);
+    normal = f3{ox, 0.0f, oz};
+    if (distance > 1.0e-12f) normal = delta / distance;
+  }
+}
+
+struct RigidArgs {
+  bm_params p;
+  bm_state s;
+  int cut_bits;
+};
+
+// blade_effective_mass (contact.py:150-178)
+__device__ __forceinline__ float effective_mass(const bm_params& P, const float* mobility, f3 lever, f3 n, quat q) {
+  float inv = 1.0f / float(P.blade_mass);
+  if (P.tool_model < 0) {
+    f3 rn = cross(lever, n);
+    float u[6] = {n.x, n.y, n.z, rn.x, rn.y, rn.z};
+    inv = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) inv += u[r] * mobility[6 * r + k] * u[k];
+  } else if (P.tool_model > 0) {
+    f3 rn = rotate_inv(q, cross(lever, n));
+    f3 ii = f3{float(P.inertia_inv[0]), float(P.inertia_inv[1]), float(P.inertia_inv[2])};
+    inv += dot(rn, f3{ii.x * rn.x, ii.y * rn.y, ii.z * rn.z});
+  }
+  return 1.0f / (1.0f / float(P.mass) + inv);
+}
+
+// rigid_and_speed (contact.py:332-497) with check_pair_list (contact.py:716-735) fused in.
+__global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_state S, int cut_bits) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool live = p < P.n;
+  const float dt = float(P.dt);
+  Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  quat q = ldq(S.tool_rot);
+  f3 origin = ld3(S.tool_pos, 0), tvel = ld3(S.tool_vel, 0), tom = ld3(S.tool_omega, 0);
+  f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
+  float work = 0.0f, spd = 0.0f;
+  if (live) {
+    f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
+    spd = len(vp);
+    // pair list staleness (check_pair_list)
+    if (S.built_x != nullptr) {
+      if (len(xp - ld3(S.built_x, p)) + dt * spd > 0.5f * float(P.pair_skin) || S.codes[p] != S.built_codes[p])
+        S.pair_dirty[0] = 1;
+    }
+    f3 total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
+    // board: plane z = 0 inside its rectangle
+    float gap = xp.z - float(P.radius);
+    if (P.board_half[0] > 0.0 && fabsf(xp.x) > float(P.board_half[0])) gap = 1.0e30f;
+    if (P.board_half[1] > 0.0 && fabsf(xp.y) > float(P.board_half[1])) gap = 1.0e30f;
+    f3 jb, hb;
+    penalty(c, gap, vp, f3{0.f, 0.f, 1.f}, float(P.mass), ld3(S.histories, p), float(P.board_sn), jb, hb);
+    st3(S.histories, p, hb);
+    if (dot(jb, jb) > 0.0f) {
+      total = total + jb;
+      board_j = jb;
+    }
+    // blade (blade_at, contact.py:235-329)
+    float blade_radius = len(hf);
+    float max_closing = spd + len(tvel) + len(tom) * blade_radius;
+    float offset = float(P.blade_offset);
+    int64_t hrow = P.n + p;
+    if (len(xp - origin) - blade_radius - offset > dt * max_closing) {
+      st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
+    } else {
+      f3 local = rotate_inv(q, xp - origin);
+      f3 closest, normal;
+      float distance;
+      blade_query(local, hf, float(P.bevel), closest, normal, distance);
+      float front = -hf.z - local.z;
+      float dmg = S.damage[p];
+      if ((cut_bits & BM_CUT_HOLD) && dmg < 1.0f && fabsf(local.x) <= hf.x + offset && fabsf(local.y) <= hf.y &&
+          front >= 0.0f) {
+        closest = f3{local.x, local.y, -hf.z};
+        normal = f3{0.f, 0.f, -1.f};
+        distance = front;
+      }
+      f3 lever = rotate(q, closest);
+      f3 n = rotate(q, normal);
+      f3 relative = vp - tvel - cross(tom, lever);
+      if (distance - offset + dt * fminf(dot(relative, n), 0.0f) >= 0.0f) {
+        st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
+      } else {
+        float me = effective_mass(P, S.mobility, lever, n, q);
+        f3 j, h;
+        penalty(c, distance - offset, relative, n, me, ld3(S.histories, hrow), float(P.sn), j, h);
+        st3(S.histories, hrow, h);
+        if (dot(j, j) > 0.0f) {
+          if (cut_bits != 0) {
+            // resist_cut (contact.py:181-232)
+            f3 edge = origin + rotate(q, f3{0.f, 0.f, -hf.z});
+            f3 down = rotate(q, f3{0.f, 0.f, -1.f});
+            f3 edge_velocity = tvel + cross(tom, edge - origin);
+            float fr = dot(xp - edge, down);
+            float advance = dot(edge_velocity - vp, down);
+            if (cut_bits & BM_CUT_ADVANCE) advance = dot(edge_velocity, down);
+            f3 le = rotate_inv(q, xp - edge);
+            bool ahead = fr >= 0.0f && fr <= 2.0f * offset && fabsf(le.x) <= 6.0f * float(P.spacing) &&
+                         fabsf(le.y) <= hf.y;
+            if (ahead) {
+              float push = fmaxf(dot(j, down), 0.0f);
+              if (dmg >= 1.0f) {
+                j = j - push * down;
+              } else if (advance > 0.0f) {
+                float delta = push * advance;
+                work = delta;
+                S.damage[p] = fminf(1.0f, dmg + delta / float(P.threshold));
+              }
+            }
+          }
+          total = total + j;
+          react = -j;
+          torq = -cross(lever, j);
+        }
+      }
+    }
+    st3(S.impulses, p, total);
+  }
+  // Wave-level reductions, one atomic per wave.
+  float m = wave_max(spd);
+  if (lane0()) atomic_max_nonneg(S.speed, m);
+  wave_add3(S.board, board_j);
+  wave_add3(S.reaction, react);
+  wave_add3(S.torque, torq);
+  float w = wave_sum(work);
+  if (lane0() && w != 0.0f) atomicAdd(S.work, w);
+}
+
+__global__ void k_clear_step(bm_state S) {
+  for (int i = 0; i < 3; ++i) {
+    S.reaction[i] = 0.0f;
+    S.torque[i] = 0.0f;
+    S.board[i] = 0.0f;
+  }
+  S.speed[0] = 0.0f;
+}
+
+// ------------------------------------------------------------------------ pair contacts
+__device__ __forceinline__ uint64_t mix_key(int64_t key) { return uint64_t(key) * 2654435761ull; }
+
+// Insert into / find in an open-addressing int64 table with linear probing
+// (material.py:37-47, contact.py:569-580).
+__device__ __forceinline__ int64_t table_insert(int64_t* keys, int64_t cap, int64_t key) {
+  int64_t slot = int64_t(mix_key(key) & uint64_t(cap - 1));
+  for (int64_t probe = 0; probe < cap; ++probe) {
+    unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(keys + slot),
+                                        (unsigned long long)(-1LL), (unsigned long long)key);
+    if (int64_t(prev) == -1 || int64_t(prev) == key) return slot;
+    slot = (slot + 1) & (cap - 1);
+  }
+  return -1;
+}
+__device__ __forceinline__ int64_t table_find(const int64_t* keys, int64_t cap, int64_t key) {
+  int64_t slot = int64_t(mix_key(key) & uint64_t(cap - 1));
+  for (int64_t probe = 0; probe < cap; ++probe) {
+    int64_t k = keys[slot];
+    if (k == -1) return -1;
+    if (k == key) return slot;
+    slot = (slot + 1) & (cap - 1);
+  }
+  return -1;
+}
+
+struct Cell {
+  int x, y, z;
+};
+__device__ __forceinline__ Cell cell_of(f3 x, float size) {
+  return Cell{int(floorf(x.x / size)), int(floorf(x.y / size)), int(floorf(x.z / size))};
+}
+// Spatial hash of Teschner et al. (2003), "Optimized Spatial Hashing for Collision
+// Detection of Deformable Objects".
+__device__ __forceinline__ int bucket_of(Cell c, int mask) {
+  return int((uint32_t(c.x) * 73856093u) ^ (uint32_t(c.y) * 19349663u) ^ (uint32_t(c.z) * 83492791u)) & mask;
+}
+
+__global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
+  int count = S.pair_count[parity_new];
+  int64_t* keys = S.pair_keys + int64_t(parity_new) * P;
+  const int32_t* used = S.pair_used + int64_t(parity_new) * P;
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < count; a += gridDim.x * blockDim.x) keys[used[a]] = -1;
+}
+__global__ void k_zero_pair_count(bm_state S, int parity_new) { S.pair_count[parity_new] = 0; }
+
+// set_pair_reach (contact.py:749-761) + zero bucket counts; only when the list is stale.
+__global__ void k_list_reset(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  int64_t B = P.list_buckets;
+  for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b <= B; b += int64_t(gridDim.x) * blockDim.x)
+    S.bucket_count[b] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt), sp = S.speed[0];
+    float value = 2.0f * r + skin;
+    if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
+    S.list_reach[0] = value;
+  }
+}
+
+__global__ void k_list_count(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float reach = S.list_reach[0];
+  int b = bucket_of(cell_of(ld3(S.x, p), reach), int(P.list_buckets - 1));
+  S.particle_bucket[p] = b;
+  S.particle_rank[p] = atomicAdd(S.bucket_count + b, 1);
+}
+
+// Exclusive scan of bucket_count[0..B) into bucket_start, three passes of 4096 entries per
+// block (1024 threads x 4).
+__global__ void __launch_bounds__(1024) k_scan_partial(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  __shared__ int wsum[16];
+  int64_t base = int64_t(blockIdx.x) * 4096;
+  int t = threadIdx.x;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t i = base + 4 * t + k;
+    if (i < P.list_buckets) sum += S.bucket_count[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if ((t & 63) == 0) wsum[t >> 6] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int total = 0;
+    for (int w = 0; w < 16; ++w) total += wsum[w];
+    S.scan_partial[blockIdx.x] = total;
+  }
+}
+__global__ void __launch_bounds__(1024) k_scan_top(const bm_params P, bm_state S, int nblocks) {
+  if (S.pair_dirty[0] == 0) return;
+  __shared__ int vals[4096];
+  int t = threadIdx.x;
+  for (int i = t; i < 4096; i += 1024) vals[i] = i < nblocks ? S.scan_partial[i] : 0;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int i = 0; i < nblocks; ++i) {
+      int v = vals[i];
+      vals[i] = run;
+      run += v;
+    }
+    S.bucket_start[P.list_buckets] = run;
+  }
+  __syncthreads();
+  for (int i = t; i < nblocks; i += 1024) S.scan_partial[i] = vals[i];
+}
+__global__ void __launch_bounds__(1024) k_scan_final(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  __shared__ int wsum[16];
+  int64_t base = int64_t(blockIdx.x) * 4096;
+  int t = threadIdx.x;
+  int v[4];
+  int local = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t i = base + 4 * t + k;
+    v[k] = i < P.list_buckets ? S.bucket_count[i] : 0;
+    local += v[k];
+  }
+  // inclusive scan of `local` across the block
+  int lane = t & 63, wave = t >> 6;
+  int incl = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    int o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int wave_base = 0;
+  for (int w = 0; w < wave; ++w) wave_base += wsum[w];
+  int run = S.scan_partial[blockIdx.x] + wave_base + incl - local;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t i = base + 4 * t + k;
+    if (i < P.list_buckets) S.bucket_start[i] = run;
+    run += v[k];
+  }
+}
+
+// Scatter particles into bucket order, then build the partner lists (build_pair_list,
+// contact.py:778-822): partners q > p within reach, in their own cell, not compatible
+// reference neighbours.
+__global__ void k_list_scatter(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  int dst = S.bucket_start[S.particle_bucket[p]] + S.particle_rank[p];
+  f3 xp = ld3(S.x, p);
+  float4 e = make_float4(xp.x, xp.y, xp.z, __int_as_float(int(p)));
+  reinterpret_cast<float4*>(S.sorted_x)[dst] = e;
+}
+
+__global__ void k_list_build(const bm_params P, bm_state S) {
+  if (S.pair_dirty[0] == 0) return;
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p == 0) S.pair_builds[0] += 1;
+  if (p >= P.n) return;
+  float reach = S.list_reach[0];
+  int mask = int(P.list_buckets - 1);
+  int lanes = int(P.list_lanes);
+  f3 xp = ld3(S.x, p);
+  st3(S.built_x, p, xp);
+  int64_t cp = S.codes[p];
+  S.built_codes[p] = cp;
+  f3 rp = ld3(S.ref, p);
+  float two_h = 2.0f * float(P.spacing);
+  Cell home = cell_of(xp, reach);
+  int count = 0;
+  const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
+  for (int i = 0; i < 27; ++i) {
+    Cell c = Cell{home.x + i % 3 - 1, home.y + (i / 3) % 3 - 1, home.z + i / 9 - 1};
+    int b = bucket_of(c, mask);
+    int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
+    for (int k = lo; k < hi; ++k) {
+      float4 e = sx[k];
+      int q = __float_as_int(e.w);
+      if (q <= p || count < 0) continue;
+      f3 xq = f3{e.x, e.y, e.z};
+      Cell cq = cell_of(xq, reach);
+      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+      if (!(len(xq - xp) < reach)) continue;
+      if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
+      if (count == lanes) {
+        count = -1;
+      } else {
+        S.partners[p * lanes + count] = q;
+        count += 1;
+      }
+    }
+  }
+  S.partner_count[p] = count;
+}
+__global__ void k_list_done(bm_state S) { S.pair_dirty[0] = 0; }
+
+struct PairCtx {
+  const int64_t* old_keys;
+  const float* old_hist;
+  int64_t* new_keys;
+  float* new_hist;
+  int32_t* new_used;
+  int32_t* new_count;
+};
+
+// pair_at (contact.py:584-640)
+__device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T, int64_t p, int64_t q) {
+  float dt = float(P.dt), r = float(P.radius);
+  f3 xp = ld3(S.x, p), xq = ld3(S.x, q);
+  f3 vp = ld3(S.v, p), vq = ld3(S.v, q);
+  f3 delta = xp - xq;
+  float dist = len(delta);
+  if (dist >= 2.0f * r + dt * len(vp - vq)) return;
+  if (compatible(S.codes[p], S.codes[q]) && len(ld3(S.ref, p) - ld3(S.ref, q)) < 2.0f * float(P.spacing)) return;
+  if (dist < 1.0e-12f) {
+    atomicAdd(S.errors + 2, 1);
+    return;
+  }
+  f3 normal = delta / dist;
+  f3 relative = vp - vq;
+  float predicted = 2.0f * r - dist - dt * fminf(dot(relative, normal), 0.0f);
+  if (predicted <= 0.0f) return;
+  int64_t key = p * P.n + q;
+  int64_t old = table_find(T.old_keys, P.pair_capacity, key);
+  f3 history = f3{0.f, 0.f, 0.f};
+  if (old >= 0) history = ld3(T.old_hist, old);
+  int64_t slot = table_insert(T.new_keys, P.pair_capacity, key);
+  if (slot < 0) {
+    atomicAdd(S.errors + 0, 1);
+    return;
+  }
+  float half_mass = float(P.mass) / 2.0f;
+  float jn = float(P.sn) * half_mass / dt * predicted;
+  Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
+  f3 jt, ht;
+  friction(c, normal, relative, jn, history, half_mass, jt, ht);
+  st3(T.new_hist, slot, ht);
+  T.new_used[atomicAdd(T.new_count, 1)] = int32_t(slot);
+  f3 impulse = jn * normal + jt;
+  atomic_add3(S.impulses, p, impulse);
+  atomic_add3(S.impulses, q, -impulse);
+}
+
+// pairs_listed (contact.py:825-925)
+__global__ void k_pairs_listed(const bm_params P, bm_state S, int parity_old) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  int count = S.partner_count[p];
+  if (count == 0) return;
+  int parity_new = 1 - parity_old;
+  int64_t cap = P.pair_capacity;
+  PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
+            S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, S.pair_count + parity_new};
+  int lanes = int(P.list_lanes);
+  if (count > 0) {
+    for (int k = 0; k < count; ++k) pair_at(P, S, T, p, S.partners[p * lanes + k]);
+    return;
+  }
+  // Overflowing particle: search the buckets of this build at the build positions.
+  float reach = S.list_reach[0];
+  int mask = int(P.list_buckets - 1);
+  const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
+  Cell home = cell_of(ld3(S.built_x, p), reach);
+  for (int i = 0; i < 27; ++i) {
+    Cell c = Cell{home.x + i % 3 - 1, home.y + (i / 3) % 3 - 1, home.z + i / 9 - 1};
+    int b = bucket_of(c, mask);
+    int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
+    for (int k = lo; k < hi; ++k) {
+      float4 e = sx[k];
+      int q = __float_as_int(e.w);
+      if (q <= p) continue;
+      Cell cq = cell_of(f3{e.x, e.y, e.z}, reach);
+      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+      pair_at(P, S, T, p, q);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------ topology
+// Rebuild of the field table when topology_dirty is set.
+__global__ void k_topo_clear(const bm_params P, bm_state S) {
+  if (S.topology_dirty[0] == 0) return;
+  int64_t total = P.fields * P.node_count;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+    S.field_code[i] = -1;
+}
+
+__device__ __forceinline__ int corner_node(int32_t base, int a, int dyz, int dz) { return base + corner_offset(a, dyz, dz); }
+
+// Insert the transfer code of every particle at its 8 stencil nodes (register,
+// material.py:50-65); at most F codes per node, else errors[0].
+__global__ void k_topo_insert(const bm_params P, bm_state S) {
+  if (S.topology_dirty[0] == 0) return;
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  int64_t code = S.transfer_codes[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  int64_t N = P.node_count;
+  int F = int(P.fields);
+  for (int a = 0; a < 8; ++a) {
+    int64_t node = corner_node(base, a, dyz, dz);
+    bool done = false;
+    for (int f = 0; f < F && !done; ++f) {
+      int64_t* slot = S.field_code + int64_t(f) * N + node;
+      int64_t cur = *slot;
+      if (cur == code) {
+        done = true;
+        break;
+      }
+      if (cur == -1) {
+        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)(-1LL),
+                                            (unsigned long long)code);
+        if (int64_t(prev) == -1 || int64_t(prev) == code) done = true;
+      }
+    }
+    if (!done) atomicAdd(S.errors + 0, 1);
+  }
+}
+
+// Canonical slot order per node (ascending code) and the used-slot count.
+__global__ void k_topo_sort(const bm_params P, bm_state S) {
+  if (S.topology_dirty[0] == 0) return;
+  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= P.node_count) return;
+  int64_t N = P.node_count;
+  int F = int(P.fields);
+  int count = 0;
+  for (int f = 0; f < F; ++f) {
+    int64_t c = S.field_code[int64_t(f) * N + node];
+    if (c < 0) break;
+    // insertion into the sorted prefix [0, count)
+    int k = count;
+    while (k > 0 && S.field_code[int64_t(k - 1) * N + node] > c) {
+      S.field_code[int64_t(k) * N + node] = S.field_code[int64_t(k - 1) * N + node];
+      --k;
+    }
+    S.field_code[int64_t(k) * N + node] = c;
+    ++count;
+  }
+  S.node_nf[node] = count;
+}
+
+// Field slot of each stencil corner.
+__global__ void k_topo_index(const bm_params P, bm_state S) {
+  if (S.topology_dirty[0] == 0) return;
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  int64_t code = S.transfer_codes[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  int64_t N = P.node_count;
+  uint32_t packed[2] = {0u, 0u};
+  for (int a = 0; a < 8; ++a) {
+    int64_t node = corner_node(base, a, dyz, dz);
+    int nf = S.node_nf[node];
+    int found = 0;
+    for (int f = 0; f < nf; ++f)
+      if (S.field_code[int64_t(f) * N + node] == code) {
+        found = f;
+        break;
+      }
+    packed[a >> 2] |= uint32_t(found & 0xff) << (8 * (a & 3));
+  }
+  reinterpret_cast<uint2*>(S.field_idx)[p] = make_uint2(packed[0], packed[1]);
+}
+__global__ void k_topo_done(bm_state S) { S.topology_dirty[0] = 0; }
+
+__device__ __forceinline__ int field_of(uint2 idx, int a) {
+  uint32_t w = a < 4 ? idx.x : idx.y;
+  return int((w >> (8 * (a & 3))) & 0xffu);
+}
+
+// ------------------------------------------------------------------------ material
+// Zero the used field slots of every node (clear_compact_grid, material.py:656-669).
+__global__ void k_grid_clear(const bm_params P, bm_state S) {
+  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= P.node_count) return;
+  int nf = S.node_nf[node];
+  int64_t N = P.node_count;
+  for (int f = 0; f < nf; ++f) {
+    int64_t s = int64_t(f) * N + node;
+    S.grid_mass[s] = 0.0f;
+    st3(S.momentum, s, f3{0.f, 0.f, 0.f});
+    st3(S.grid_impulse, s, f3{0.f, 0.f, 0.f});
+    st3(S.remapped, s, f3{0.f, 0.f, 0.f});
+  }
+}
+
+// p2g (material.py:127-152): one thread per particle scatters its 8 corners.
+__global__ void __launch_bounds__(kBlock) k_p2g(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float mass = float(P.mass);
+  float sv = float(P.dt) * float(P.volume);
+  f3 vp = ld3(S.v, p), imp = ld3(S.impulses, p);
+  m3 Pk = ld9(S.stress, p);
+  Stencil st;
+  stencil_full(ld3(S.local, p), 1.0f / float(P.spacing), st);
+  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  int64_t N = P.node_count;
+  m3 sP;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sP.a[k] = sv * Pk.a[k];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    int64_t s = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
+    float w = st.w[a];
+    float mw = mass * w;
+    atomicAdd(S.grid_mass + s, mw);
+    atomic_add3(S.momentum, s, mw * vp);
+    atomic_add3(S.grid_impulse, s, w * imp - mvec(sP, st.g[a]));
+  }
+}
+
+// Compatible-field exchange of one node (Eq. 8, material.py:252-285): for field a,
+// x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b.
+__global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S) {
+  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= P.node_count) return;
+  int nf = S.node_nf[node];
+  int64_t N = P.node_count;
+  if (nf == 1) {
+    float m = S.grid_mass[node];
+    if (m > 0.0f) {
+      f3 pm = ld3(S.momentum, node), j = ld3(S.grid_impulse, node);
+      st3(S.old_v, node, pm / m);
+      st3(S.new_v, node, (pm + j) / m);
+    }
+    return;
+  }
+  float total = 0.0f;
+  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  for (int a = 0; a < nf; ++a) {
+    int64_t sa = int64_t(a) * N + node;
+    float ma = S.grid_mass[sa];
+    if (ma <= 0.0f) continue;
+    int64_t ca = S.field_code[sa];
+    f3 pa = ld3(S.momentum, sa), ja = ld3(S.grid_impulse, sa);
+    f3 dm = f3{0.f, 0.f, 0.f}, di = dm;
+    for (int b = 0; b < nf; ++b) {
+      if (b == a) continue;
+      int64_t sb = int64_t(b) * N + node;
+      if (!compatible(ca, S.field_code[sb])) continue;
+      float mb = S.grid_mass[sb];
+      dm = dm + (ma * ld3(S.momentum, sb) - mb * pa);
+      di = di + (ma * ld3(S.grid_impulse, sb) - mb * ja);
+    }
+    f3 pm = pa + dm / total, jm = ja + di / total;
+    st3(S.old_v, sa, pm / ma);
+    st3(S.new_v, sa, (pm + jm) / ma);
+  }
+}
+
+// g2p_remap (material.py:384-401)
+__global__ void __launch_bounds__(kBlock) k_g2p(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float w[8];
+  stencil_weights(ld3(S.local, p), w);
+  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  int64_t N = P.node_count;
+  int64_t slot[8];
+  f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    slot[a] = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
+    f3 nv = ld3(S.new_v, slot[a]);
+    pic = pic + w[a] * nv;
+    dv = dv + w[a] * (nv - ld3(S.old_v, slot[a]));
+  }
+  float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
+  f3 vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
+  st3(S.v, p, vp);
+  st3(S.x, p, ld3(S.x, p) + dt * pic);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) atomic_add3(S.remapped, slot[a], (mass * w[a]) * vp);
+}
+
+// couple (material.py:155-179) of the re-scattered momentum, then field velocity.
+__global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S) {
+  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= P.node_count) return;
+  int nf = S.node_nf[node];
+  int64_t N = P.node_count;
+  if (nf == 1) {
+    float m = S.grid_mass[node];
+    f3 vel = f3{0.f, 0.f, 0.f};
+    if (m > 0.0f) vel = ld3(S.remapped, node) / m;
+    st3(S.grid_vel, node, vel);
+    return;
+  }
+  float total = 0.0f;
+  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  for (int a = 0; a < nf; ++a) {
+    int64_t sa = int64_t(a) * N + node;
+    float ma = S.grid_mass[sa];
+    if (ma <= 0.0f) {
+      st3(S.grid_vel, sa, f3{0.f, 0.f, 0.f});
+      continue;
+    }
+    int64_t ca = S.field_code[sa];
+    f3 xa = ld3(S.remapped, sa);
+    f3 ex = f3{0.f, 0.f, 0.f};
+    for (int b = 0; b < nf; ++b) {
+      if (b == a) continue;
+      int64_t sb = int64_t(b) * N + node;
+      if (!compatible(ca, S.field_code[sb])) continue;
+      ex = ex + (ma * ld3(S.remapped, sb) - S.grid_mass[sb] * xa);
+    }
+    st3(S.grid_vel, sa, (xa + ex / total) / ma);
+  }
+}
+
+// Hencky von Mises return map (material.py:404-431).  Returns false when inverted.
+__device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t p, const m3& F, m3& Fp, m3& U,
+                                  f3& sg, m3& V) {
+  svd3(mmul(F, minv(Fp)), U, sg, V);
+  if (fminf(fminf(sg.x, sg.y), sg.z) <= 0.0f) {
+    atomicAdd(S.errors + 1, 1);
+    return false;
+  }
+  f3 e = f3{logf(sg.x), logf(sg.y), logf(sg.z)};
+  float mean = (e.x + e.y + e.z) / 3.0f;
+  f3 dev = e - f3{mean, mean, mean};
+  float norm = len(dev);
+  float limit = float(P.yield_stress) / (2.0f * float(P.mu)) * sqrtf(2.0f / 3.0f);
+  if (norm > limit) {
+    f3 ep = e - (1.0f - limit / norm) * dev;
+    sg = f3{expf(ep.x), expf(ep.y), expf(ep.z)};
+    m3 US;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      US.a[3 * i] = U.a[3 * i] * sg.x;
+      US.a[3 * i + 1] = U.a[3 * i + 1] * sg.y;
+      US.a[3 * i + 2] = U.a[3 * i + 2] * sg.z;
+    }
+    m3 next_f = mmul_t(US, V);
+    Fp = mmul(minv(next_f), F);
+    st9(S.Fp, p, Fp);
+  }
+  return true;
+}
+
+// stress_at (material.py:94-112)
+__device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const m3& F, const m3& Fp) {
+  m3 pinv = minv(Fp);
+  m3 f = mmul(F, pinv);
+  float J = mdet(f);
+  if (!(J > 0.0f) || !isfinite(J)) {
+    atomicAdd(S.errors + 1, 1);
+    return mzero();
+  }
+  m3 U, V;
+  f3 sg;
+  svd3(f, U, sg, V);
+  return corotated(U, sg, V, pinv, float(P.mu), float(P.lam));
+}
+
+__device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p, const m3& F) {
+  m3 Fp = ld9(S.Fp, p);
+  if (P.plastic) {
+    m3 U, V;
+    f3 sg;
+    if (!plastic_at(P, S, p, F, Fp, U, sg, V)) return mzero();
+    return corotated(U, sg, V, minv(Fp), float(P.mu), float(P.lam));
+  }
+  return elastic_stress(P, S, F, Fp);
+}
+
+// update_F_plastic (material.py:445-475)
+__global__ void __launch_bounds__(kBlock) k_update_F(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  Stencil st;
+  stencil_full(ld3(S.local, p), 1.0f / float(P.spacing), st);
+  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  int64_t N = P.node_count;
+  m3 rate = mzero();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    int64_t s = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
+    f3 vel = ld3(S.grid_vel, s);
+    f3 g = st.g[a];
+    rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
+    rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
+    rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
+  }
+  m3 F = ld9(S.F, p);
+  float dt = float(P.dt);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F.a[k] += dt * rate.a[k];
+  st9(S.F, p, F);
+  st9(S.stress, p, stress_of(P, S, p, F));
+}
+
+__global__ void k_refresh_stress(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  st9(S.stress, p, elastic_stress(P, S, ld9(S.F, p), ld9(S.Fp, p)));
+}
+
+__global__ void k_plastic_project(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  m3 Fp = ld9(S.Fp, p), U, V;
+  f3 sg;
+  plastic_at(P, S, p, ld9(S.F, p), Fp, U, sg, V);
+}
+
+// ------------------------------------------------------------------------ tool
+__device__ __forceinline__ f3 inv_inertia(quat q, f3 ii, f3 t) {
+  f3 l = rotate_inv(q, t);
+  return rotate(q, f3{ii.x * l.x, ii.y * l.y, ii.z * l.z});
+}
+__device__ __forceinline__ f3 blade_vertex(int i, f3 hf, float bevel) {
+  int j = i % 5;
+  float vx = 0.0f, vz = -hf.z;
+  if (j == 1) {
+    vx = hf.x;
+    vz += bevel;
+  } else if (j == 2) {
+    vx = hf.x;
+    vz = hf.z;
+  } else if (j == 3) {
+    vx = -hf.x;
+    vz = hf.z;
+  } else if (j == 4) {
+    vx = -hf.x;
+    vz += bevel;
+  }
+  return f3{vx, i >= 5 ? hf.y : -hf.y, vz};
+}
+
+// tool.step (tool.py:146-259): drive, gyroscopic term, 6 sweeps x 10 vertices of the
+// knife/board velocity constraint with Coulomb friction, integration.
+__global__ void k_tool_step(const bm_params P, bm_state S, const float* targets, const float* rotations, int index) {
+  float dt = float(P.dt), mass = float(P.blade_mass), bevel = float(P.bevel);
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  f3 x = ld3(S.tool_pos, 0);
+  quat q = qnormalize(ldq(S.tool_rot));
+  f3 v = ld3(S.tool_vel, 0), w = ld3(S.tool_omega, 0);
+  f3 force = f3{0.f, 0.f, -9.81f * mass};
+  f3 servo = f3{0.f, 0.f, 0.f};
+  if (P.free_blade == 0) {
+    quat tq = ldq(rotations + 4 * index);
+    f3 target_center = ld3(targets, index) + rotate(tq, f3{0.f, 0.f, hf.z});
+    force = float(P.drive_k) * (target_center - x) - float(P.drive_c) * v;
+    float norm = len(force);
+    if (norm > float(P.drive_limit)) force = force * (float(P.drive_limit) / norm);
+    quat dq = qnormalize(qmul(tq, quat{-q.x, -q.y, -q.z, q.w}));
+    if (dq.w < 0.0f) dq = quat{-dq.x, -dq.y, -dq.z, -dq.w};
+    f3 vec = qvec(dq);
+    float s = len(vec);
+    f3 error = 2.0f * vec;
+    if (s > 1.0e-8f) error = vec * (2.0f * atan2f(s, dq.w) / s);
+    servo = float(P.rot_k) * error - float(P.rot_c) * w;
+  }
+  v = v + (force * dt + ld3(S.reaction, 0)) / mass;
+  f3 ii = f3{float(P.inertia_inv[0]), float(P.inertia_inv[1]), float(P.inertia_inv[2])};
+  if (P.lock_rotation) {
+    ii = f3{0.f, 0.f, 0.f};
+    w = f3{0.f, 0.f, 0.f};
+  } else {
+    f3 lw = rotate_inv(q, w);
+    f3 inertia = f3{1.0f / ii.x, 1.0f / ii.y, 1.0f / ii.z};
+    f3 gyro = rotate(q, cross(lw, f3{inertia.x * lw.x, inertia.y * lw.y, inertia.z * lw.z}));
+    w = w + inv_inertia(q, ii, ld3(S.torque, 0) + dt * (servo - gyro));
+  }
+  f3 normal = f3{0.f, 0.f, 1.f};
+  f3 total = f3{0.f, 0.f, 0.f};
+  for (int sweep = 0; sweep < 6; ++sweep)
+    for (int i = 0; i < 10; ++i) {
+      f3 r = rotate(q, blade_vertex(i, hf, bevel));
+      f3 corner = x + r;
+      if ((P.board_half[0] > 0.0 && fabsf(corner.x) > float(P.board_half[0])) ||
+          (P.board_half[1] > 0.0 && fabsf(corner.y) > float(P.board_half[1])))
+        continue;
+      float gap = corner.z;
+      float vn = dot(v + cross(w, r), normal);
+      float desired = -gap / dt;
+      if (gap < 0.0f) desired *= 0.2f;
+      if (vn < desired) {
+        f3 rn = cross(r, normal);
+        float effective = 1.0f / mass + dot(rn, inv_inertia(q, ii, rn));
+        float jn = (desired - vn) / effective;
+        f3 impulse = jn * normal;
+        v = v + impulse / mass;
+        w = w + inv_inertia(q, ii, cross(r, impulse));
+        f3 vt = v + cross(w, r);
+        vt = vt - dot(vt, normal) * normal;
+        float sp = len(vt);
+        if (sp > 1.0e-10f) {
+          f3 direction = -vt / sp;
+          f3 rt = cross(r, direction);
+          float wt = 1.0f / mass + dot(rt, inv_inertia(q, ii, rt));
+          float jt = fminf(sp / wt, float(P.mud) * jn);
+          f3 tangent = jt * direction;
+          impulse = impulse + tangent;
+          v = v + tangent / mass;
+          w = w + inv_inertia(q, ii, cross(r, tangent));
+        }
+        total = total + impulse;
+      }
+    }
+  x = x + dt * v;
+  float sp = len(w);
+  if (sp > 1.0e-10f) q = qnormalize(qmul(axis_angle(w / sp, dt * sp), q));
+  st3(S.tool_pos, 0, x);
+  st3(S.tool_vel, 0, v);
+  S.tool_rot[0] = q.x;
+  S.tool_rot[1] = q.y;
+  S.tool_rot[2] = q.z;
+  S.tool_rot[3] = q.w;
+  st3(S.tool_omega, 0, w);
+  st3(S.table, 0, total);
+}
+
+// finish_external_step (tool.py:70-115)
+__global__ void k_finish_external(const bm_params P, bm_state S) {
+  f3 j = ld3(S.reaction, 0), tau = ld3(S.torque, 0);
+  f3 pos = ld3(S.tool_pos, 0);
+  st3(S.interval_impulse, 0, ld3(S.interval_impulse, 0) + j);
+  st3(S.interval_angular, 0, ld3(S.interval_angular, 0) + (tau + cross(pos, j)));
+  if (P.predict) {
+    float u[6] = {j.x, j.y, j.z, tau.x, tau.y, tau.z};
+    float dv[3], dw[3];
+    for (int r = 0; r < 3; ++r) {
+      float a = 0.0f, b = 0.0f;
+      for (int k = 0; k < 6; ++k) {
+        a += S.mobility[6 * r + k] * u[k];
+        b += S.mobility[6 * (r + 3) + k] * u[k];
+      }
+      dv[r] = a;
+      dw[r] = b;
+    }
+    f3 vel = ld3(S.tool_vel, 0) + f3{dv[0], dv[1], dv[2]};
+    st3(S.tool_vel, 0, vel);
+    f3 w = ld3(S.tool_omega, 0) + f3{dw[0], dw[1], dw[2]};
+    st3(S.tool_omega, 0, w);
+    st3(S.tool_pos, 0, pos + float(P.dt) * vel);
+    float rate = len(w);
+    if (rate > 1.0e-12f) {
+      quat nq = qnormalize(qmul(axis_angle(w / rate, float(P.dt) * rate), ldq(S.tool_rot)));
+      S.tool_rot[0] = nq.x;
+      S.tool_rot[1] = nq.y;
+      S.tool_rot[2] = nq.z;
+      S.tool_rot[3] = nq.w;
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    S.reaction[i] = 0.0f;
+    S.torque[i] = 0.0f;
+    S.board[i] = 0.0f;
+  }
+  S.speed[0] = 0.0f;
+}
+
+__global__ void k_accumulate_reaction(bm_state S) {
+  f3 j = ld3(S.reaction, 0);
+  st3(S.interval_impulse, 0, ld3(S.interval_impulse, 0) + j);
+  st3(S.interval_angular, 0, ld3(S.interval_angular, 0) + (ld3(S.torque, 0) + cross(ld3(S.tool_pos, 0), j)));
+}
+
+// assign (contact.py:996-1039)
+__global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n || cutting == 0) return;
+  int64_t code = S.codes[p];
+  if ((code / digit) % 3 != 0) return;
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  quat q = ldq(S.tool_rot);
+  f3 pos = ld3(S.tool_pos, 0);
+  f3 xp = ld3(S.x, p);
+  f3 edge = pos + rotate(q, f3{0.f, 0.f, -hf.z});
+  f3 local = rotate_inv(q, xp - edge);
+  float front = -local.z;
+  f3 edge_v = ld3(S.tool_vel, 0) + cross(ld3(S.tool_omega, 0), edge - pos);
+  f3 d = rotate(q, f3{0.f, 0.f, -1.f});
+  float advance = dot(edge_v - ld3(S.v, p), d);
+  float spacing = float(P.spacing), r = float(P.radius), dt = float(P.dt);
+  float back = 2.0f * spacing + fmaxf(advance * dt, 0.0f);
+  bool behind = front >= -back && front <= 0.0f;
+  bool on_board = (P.board_half[0] == 0.0 || fabsf(xp.x) <= float(P.board_half[0])) &&
+                  (P.board_half[1] == 0.0 || fabsf(xp.y) <= float(P.board_half[1]));
+  f3 edge_here = edge + rotate(q, f3{0.f, local.y, 0.f});
+  bool pinched = on_board && edge_here.z <= r && xp.z <= edge_here.z + r;
+  if (fabsf(local.x) <= 6.0f * spacing && fabsf(local.y) <= hf.y && (behind || pinched)) {
+    int64_t side = local.x < 0.0f ? 1 : 2;
+    S.codes[p] = code + digit * side;
+    S.topology_dirty[0] = 1;
+  }
+}
+
+// ------------------------------------------------------------------------ port kernels
+__global__ void k_load_external(bm_state S, const float* packed) {
+  for (int i = 0; i < 3; ++i) {
+    S.tool_pos[i] = packed[i];
+    S.tool_vel[i] = packed[7 + i];
+    S.tool_omega[i] = packed[10 + i];
+    S.interval_impulse[i] = 0.0f;
+    S.interval_angular[i] = 0.0f;
+  }
+  for (int i = 0; i < 4; ++i) S.tool_rot[i] = packed[3 + i];
+  for (int i = 0; i < 36; ++i) S.mobility[i] = packed[13 + i];
+}
+__global__ void k_record_pose(bm_state S, int index, float* out) {
+  int offset = 9 + 7 * index;
+  for (int i = 0; i < 3; ++i) out[offset + i] = S.tool_pos[i];
+  for (int i = 0; i < 4; ++i) out[offset + 3 + i] = S.tool_rot[i];
+}
+__global__ void k_store_interval(bm_state S, float* out) {
+  for (int i = 0; i < 3; ++i) {
+    out[i] = S.interval_impulse[i];
+    out[3 + i] = S.interval_angular[i];
+    out[6 + i] = float(S.errors[i]);
+  }
+}
+// validate_interval (material.py:6-20)
+__global__ void k_validate(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float d = mdet(ld9(S.F, p));
+  int bad = (!(d > 0.0f) || !isfinite(d)) ? 1 : 0;
+  f3 x = ld3(S.x, p), v = ld3(S.v, p);
+  bad += !isfinite(x.x) + !isfinite(x.y) + !isfinite(x.z) + !isfinite(v.x) + !isfinite(v.y) + !isfinite(v.z);
+  if (bad) atomicAdd(S.errors + 1, bad);
+}
+
+__global__ void k_count_embedded(const bm_params P, bm_state S, int32_t* count) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  f3 local = rotate_inv(ldq(S.tool_rot), ld3(S.x, p) - ld3(S.tool_pos, 0));
+  if (fabsf(local.x) <= 6.0f * float(P.spacing) && fabsf(local.y) <= hf.y && fabsf(local.z) <= hf.z)
+    atomicAdd(count, 1);
+}
+
+// metrics (diagnostics.py:7-35)
+__global__ void __launch_bounds__(kBlock) k_metrics(const bm_params P, bm_state S, float* out) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float kin = 0.f, grav = 0.f, el = 0.f, vmax = 0.f, zmax = -3.0e38f, detmin = 3.0e38f;
+  if (p < P.n) {
+    m3 F = ld9(S.F, p);
+    m3 fe = mmul(F, minv(ld9(S.Fp, p)));
+    m3 U, V;
+    f3 sg;
+    svd3(fe, U, sg, V);
+    m3 R = mmul_t(U, V);
+    float e = 0.f;
+    float mu = float(P.mu), lam = float(P.lam);
+    for (int k = 0; k < 9; ++k) {
+      float d = fe.a[k] - R.a[k];
+      e += mu * d * d;
+    }
+    float J = mdet(fe);
+    e += 0.5f * lam * (J - 1.0f) * (J - 1.0f);
+    f3 v = ld3(S.v, p), x = ld3(S.x, p);
+    kin = 0.5f * float(P.mass) * dot(v, v);
+    grav = float(P.mass) * 9.81f * x.z;
+    el = float(P.volume) * e;
+    vmax = len(v);
+    zmax = x.z;
+    detmin = mdet(F);
+  }
+  kin = wave_sum(kin);
+  grav = wave_sum(grav);
+  el = wave_sum(el);
+  vmax = wave_max(vmax);
+  zmax = wave_max(zmax);
+  float dm = detmin;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dm = fminf(dm, __shfl_xor(dm, off, 64));
+  if (lane0()) {
+    atomicAdd(out + 0, kin);
+    atomicAdd(out + 1, grav);
+    atomicAdd(out + 2, el);
+    atomicMax(out + 3, vmax);
+    atomicMax(out + 4, zmax);
+    atomicMin(out + 5, dm);
+  }
+}
+
+// ------------------------------------------------------------------------ probes
+__global__ void k_probe_blade_query(const float* local, f3 hf, float bevel, float* out, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  f3 c, nrm;
+  float d;
+  blade_query(ld3(local, i), hf, bevel, c, nrm, d);
+  out[7 * i] = c.x; out[7 * i + 1] = c.y; out[7 * i + 2] = c.z;
+  out[7 * i + 3] = nrm.x; out[7 * i + 4] = nrm.y; out[7 * i + 5] = nrm.z;
+  out[7 * i + 6] = d;
+}
+__global__ void k_probe_friction(const float* in, Contact c, float* out, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float* a = in + 13 * i;
+  f3 jt, ht;
+  friction(c, f3{a[0], a[1], a[2]}, f3{a[3], a[4], a[5]}, a[6], f3{a[7], a[8], a[9]}, a[10], jt, ht);
+  st3(out, 2 * i, jt);
+  st3(out, 2 * i + 1, ht);
+}
+__global__ void k_probe_svd(const float* A, float* U, float* s, float* V, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  m3 u, v;
+  f3 sg;
+  svd3(ld9(A, i), u, sg, v);
+  st9(U, i, u);
+  st9(V, i, v);
+  st3(s, i, sg);
+}
+
+inline int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : int(e);
+}
+
+bool state_ok(const bm_params* p, const bm_state* s) {
+  return p != nullptr && s != nullptr && p->n >= 0 && s->x != nullptr && s->v != nullptr;
+}
+
+}  // namespace
+
+// ======================================================================== C-ABI
+extern "C" {
+
+int bm_version(void) { return 27; }
+
+int bm_refresh_stress(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s) || !s->F || !s->Fp || !s->stress) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_refresh_stress, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_clear_step(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_clear_step, dim3(1), dim3(1), 0, (hipStream_t)stream, *s);
+  return launch_status();
+}
+
+int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, void* stream) {
+  if (!state_ok(p, s) || !s->impulses || !s->histories || !s->damage) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_rigid, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
+  return launch_status();
+}
+
+int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void* stream) {
+  if (!state_ok(p, s) || !s->pair_keys || !s->sorted_x || p->pair_capacity <= 0 ||
+      (p->pair_capacity & (p->pair_capacity - 1)) != 0 || p->list_buckets <= 0 ||
+      (p->list_buckets & (p->list_buckets - 1)) != 0 || p->list_buckets > (int64_t(4096) * 4096))
+    return -1;
+  if (p->n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int old_p = parity & 1, new_p = 1 - old_p;
+  hipLaunchKernelGGL(k_clear_pair_slots, dim3(256), dim3(kBlock), 0, st, *s, p->pair_capacity, new_p);
+  hipLaunchKernelGGL(k_zero_pair_count, dim3(1), dim3(1), 0, st, *s, new_p);
+  int64_t B = p->list_buckets;
+  int nscan = int((B + 4095) / 4096);
+  hipLaunchKernelGGL(k_list_reset, dim3(blocks_for(B + 1, 1024)), dim3(1024), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_count, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_scan_partial, dim3(nscan), dim3(1024), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, *p, *s, nscan);
+  hipLaunchKernelGGL(k_scan_final, dim3(nscan), dim3(1024), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_scatter, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_build, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1), 0, st, *s);
+  hipLaunchKernelGGL(k_pairs_listed, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
+  return launch_status();
+}
+
+int bm_register(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s) || !s->field_code || !s->node_nf || !s->field_idx || !s->transfer_codes) return -1;
+  if (p->fields < 1 || p->fields > 255) return -2;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t total = p->fields * p->node_count;
+  hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
+  if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_topo_sort, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  if (p->n > 0) hipLaunchKernelGGL(k_topo_index, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_topo_done, dim3(1), dim3(1), 0, st, *s);
+  return launch_status();
+}
+
+int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s) || !s->grid_mass) return -1;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_grid_clear, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  if (p->n > 0) hipLaunchKernelGGL(k_p2g, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  return launch_status();
+}
+
+int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+                     *s);
+  return launch_status();
+}
+
+int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_g2p, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+                     *s);
+  return launch_status();
+}
+
+int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_update_F, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_material_step(const bm_params* p, const bm_state* s, void* stream) {
+  int r;
+  if ((r = bm_register(p, s, stream))) return r;
+  if ((r = bm_p2g(p, s, stream))) return r;
+  if ((r = bm_couple_and_velocity(p, s, stream))) return r;
+  if ((r = bm_g2p_remap(p, s, stream))) return r;
+  if ((r = bm_couple_remapped(p, s, stream))) return r;
+  return bm_update_F(p, s, stream);
+}
+
+int bm_tool_step(const bm_params* p, const bm_state* s, const float* targets, const float* rotations, int32_t index,
+                 void* stream) {
+  if (!state_ok(p, s) || !targets || !rotations || index < 0) return -1;
+  hipLaunchKernelGGL(k_tool_step, dim3(1), dim3(1), 0, (hipStream_t)stream, *p, *s, targets, rotations, int(index));
+  return launch_status();
+}
+
+int bm_finish_external(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_accumulate_reaction(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_accumulate_reaction, dim3(1), dim3(1), 0, (hipStream_t)stream, *s);
+  return launch_status();
+}
+
+int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutting, void* stream) {
+  if (!state_ok(p, s) || digit < 1) return -1;
+  if (p->n == 0 || !cutting) return 0;
+  hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, digit,
+                     int(cutting));
+  return launch_status();
+}
+
+int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
+               int32_t parity, int32_t accumulate, const float* targets, const float* rotations, int32_t index,
+               void* stream) {
+  int r;
+  bool external = p && p->tool_model < 0;
+  if (!external && (r = bm_clear_step(p, s, stream))) return r;
+  if ((r = bm_rigid_contacts(p, s, cutting ? cut_bits : 0, stream))) return r;
+  if ((r = bm_pair_contacts(p, s, parity, stream))) return r;
+  if ((r = bm_material_step(p, s, stream))) return r;
+  if (external) {
+    if ((r = bm_finish_external(p, s, stream))) return r;
+  } else {
+    if (accumulate && (r = bm_accumulate_reaction(p, s, stream))) return r;
+    if ((r = bm_tool_step(p, s, targets, rotations, index, stream))) return r;
+  }
+  return bm_assign(p, s, digit, cutting, stream);
+}
+
+int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream) {
+  if (!state_ok(p, s) || !packed49) return -1;
+  hipLaunchKernelGGL(k_load_external, dim3(1), dim3(1), 0, (hipStream_t)stream, *s, packed49);
+  return launch_status();
+}
+
+int bm_record_pose(const bm_params* p, const bm_state* s, int32_t index, float* out, void* stream) {
+  if (!state_ok(p, s) || !out || index < 0) return -1;
+  hipLaunchKernelGGL(k_record_pose, dim3(1), dim3(1), 0, (hipStream_t)stream, *s, int(index), out);
+  return launch_status();
+}
+
+int bm_store_interval(const bm_params* p, const bm_state* s, float* out, void* stream) {
+  if (!state_ok(p, s) || !out) return -1;
+  hipLaunchKernelGGL(k_store_interval, dim3(1), dim3(1), 0, (hipStream_t)stream, *s, out);
+  return launch_status();
+}
+
+int bm_validate_interval(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_validate, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_count_embedded(const bm_params* p, const bm_state* s, int32_t* count, void* stream) {
+  if (!state_ok(p, s) || !count) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_count_embedded, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, count);
+  return launch_status();
+}
+
+int bm_metrics(const bm_params* p, const bm_state* s, float* out, void* stream) {
+  if (!state_ok(p, s) || !out) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_metrics, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, out);
+  return launch_status();
+}
+
+int bm_plastic_project(const bm_params* p, const bm_state* s, void* stream) {
+  if (!state_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_plastic_project, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_probe_blade_query(const float* local, const float* half, float bevel, float* out, int32_t count,
+                         void* stream) {
+  if (!local || !half || !out || count < 0) return -1;
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_probe_blade_query, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, local,
+                     f3{half[0], half[1], half[2]}, bevel, out, int(count));
+  return launch_status();
+}
+
+int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk, float gc, float* out, int32_t count,
+                      void* stream) {
+  if (!in || !out || count < 0) return -1;
+  if (count == 0) return 0;
+  Contact c{dt, 0.0f, mus, mud, gk, gc};
+  hipLaunchKernelGGL(k_probe_friction, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, in, c, out,
+                     int(count));
+  return launch_status();
+}
+
+int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream) {
+  if (!A || !U || !s || !V || count < 0) return -1;
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_probe_svd, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, A, U, s, V,
+                     int(count));
+  return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+/* C-ABI of libbm_hip.so: the BladeMaster TLMPM cutting substep on MI355X (gfx950).
+ *
+ * The reference implements this path as Warp kernels launched from Python
+ * (blademaster_warp/solver.py:318-940); its "FFI" is wp.launch(kernel, dim, args).  Each
+ * entry point below replaces one such launch group and cites it.  All device buffers are
+ * owned by the caller (the Python host keeps them as PyTorch-ROCm tensors) and passed as
+ * plain pointers in bm_state; scalars travel in bm_params.  Every call is asynchronous on
+ * the given stream (a hipStream_t passed as void*), never allocates, never synchronises,
+ * and is safe to record into a HIP graph.  Stages that the reference guards with Warp
+ * conditional graph nodes (capture_if on topology_dirty / pair_list_dirty) read the
+ * corresponding device flag themselves and do nothing while it is zero.
+ *
+ * Return value: 0 on success, negative for an argument error (null pointer, bad size),
+ * positive hipError_t for a launch failure.  Physics failures are counted on the device in
+ * errors[3] = [storage overflow, inverted material, coincident proxies], the layout of
+ * solver.py:307.
+ *
+ * Layout (float32 unless noted; n particles, N grid nodes, F field slots per node):
+ *   particle vec3 arrays  [n][3], matrices [n][3][3] row-major, codes int64[n]
+ *   local   [n][3] cell fractions of the reference position, base_node int32[n]
+ *   field_idx uint8[n][8]  field slot of each stencil corner
+ *   field_code int64[F][N] (-1 = empty slot), node_nf int32[N]
+ *   grid scalars [F][N], grid vectors [F][N][3]
+ *   histories [2][n][3]  (row 0 board, row 1 blade friction history)
+ */
+#ifndef BM_HIP_H
+#define BM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bits of cut_bits (contact.py:15-18). */
+#define BM_CUT_ON 1
+#define BM_CUT_ADVANCE 2
+#define BM_CUT_HOLD 4
+
+typedef struct bm_params {
+  int64_t n;              /* particles */
+  int64_t node_count;     /* N = prod(dims) */
+  int64_t fields;         /* F field slots per node */
+  int64_t dims[3];        /* grid nodes per axis (cells + 1) */
+  int64_t pair_capacity;  /* slots per pair-history table, power of two */
+  int64_t list_buckets;   /* cell-list buckets, power of two */
+  int64_t list_lanes;     /* listed partners per particle (16) */
+  int64_t tool_model;     /* -1 external, 0 guided translation, 1 free rigid / servo */
+  int64_t free_blade;     /* tool.step: no drive */
+  int64_t lock_rotation;  /* tool.step: guided, translation only */
+  int64_t predict;        /* finish_external_step: mobility predictor */
+  int64_t plastic;        /* yield_stress > 0 */
+  double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
+  double sn, gk, mus, mud, gc, board_sn, threshold;
+  double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
+  double drive_k, drive_c, drive_limit, rot_k, rot_c;
+  double pair_skin;
+} bm_params;
+
+typedef struct bm_state {
+  /* particles */
+  float *x, *v, *F, *Fp, *stress, *damage, *ref, *local, *impulses, *histories;
+  int64_t *codes;          /* side codes (contact.py:996-1039) */
+  int64_t *transfer_codes; /* codes used by the grid transfers: codes, or zeros for single field */
+  int32_t *base_node;
+  uint8_t *field_idx;
+  /* grid field table */
+  int64_t *field_code;
+  int32_t *node_nf;
+  float *grid_mass, *momentum, *grid_impulse, *old_v, *new_v, *remapped, *grid_vel;
+  /* tool and scalars */
+  float *tool_pos, *tool_vel, *tool_rot, *tool_omega, *mobility;
+  float *reaction, *torque, *board, *work, *speed, *table;
+  float *interval_impulse, *interval_angular;
+  int32_t *errors, *topology_dirty;
+  /* pair contacts: ping-pong history tables [2][P] and the cell list */
+  int64_t *pair_keys;      /* [2][P], -1 empty */
+  float *pair_history;     /* [2][P][3] */
+  int32_t *pair_used;      /* [2][P] */
+  int32_t *pair_count;     /* [2] */
+  int32_t *pair_dirty;     /* [1] rebuild request */
+  int32_t *pair_builds;    /* [1] */
+  float *list_reach;       /* [1] */
+  float *built_x;          /* [n][3] */
+  int64_t *built_codes;    /* [n] */
+  int32_t *partners;       /* [n][lanes] */
+  int32_t *partner_count;  /* [n] */
+  int32_t *bucket_count;   /* [B + 1] */
+  int32_t *bucket_start;   /* [B + 1] */
+  int32_t *scan_partial;   /* [ceil(B / 4096) + 1] */
+  int32_t *particle_bucket;/* [n] */
+  int32_t *particle_rank;  /* [n] */
+  float *sorted_x;         /* [n][4]: position and particle index (bits) in bucket order */
+} bm_state;
+
+/* Library identification; returns the number of exported stage entry points. */
+int bm_version(void);
+
+/* Stress of the current F, Fp (material.py:94-124, compute_stress; solver.py:318-329). */
+int bm_refresh_stress(const bm_params* p, const bm_state* s, void* stream);
+
+/* Zero reaction, torque, board, speed (contact.py:1058-1068, clear_step). */
+int bm_clear_step(const bm_params* p, const bm_state* s, void* stream);
+
+/* Gravity, board and blade contact with cutting resistance; max speed; pair-list
+ * staleness check (contact.py:332-497 rigid_and_speed, contact.py:716-735 check_pair_list;
+ * solver.py:588-630). */
+int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, void* stream);
+
+/* Particle-pair contact with friction history over a sort-based cell list
+ * (contact.py:569-925; solver.py:670-786).  parity = step_count % 2 selects the tables. */
+int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void* stream);
+
+/* Multi-field topology: rebuild the node field table and the per-corner field slots when
+ * topology_dirty is set (material.py:37-75, 621-653; solver.py:391-429). */
+int bm_register(const bm_params* p, const bm_state* s, void* stream);
+
+/* P2G of mass, momentum and stress + contact impulse into the field table
+ * (material.py:127-152; clears the used fields first, material.py:656-669). */
+int bm_p2g(const bm_params* p, const bm_state* s, void* stream);
+
+/* Compatible-field exchange (Eq. 8) and grid velocities (material.py:252-318). */
+int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream);
+
+/* G2P with PIC/FLIP and TLMPM position update, MUSL re-scatter (material.py:343-401). */
+int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream);
+
+/* Eq. 8 exchange of the re-scattered momentum, then field velocity = p / m
+ * (solver.py:494, material.py:155-209). */
+int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream);
+
+/* MUSL F update, optional Hencky von Mises return map, stress for the next step
+ * (material.py:404-475). */
+int bm_update_F(const bm_params* p, const bm_state* s, void* stream);
+
+/* Whole material step: register, p2g, couple_and_velocity, g2p_remap, couple_remapped,
+ * update_F (solver.py:334-535). */
+int bm_material_step(const bm_params* p, const bm_state* s, void* stream);
+
+/* Knife drive and knife/board contact, one thread (tool.py:146-259; solver.py:824-864).
+ * targets[3*index], rotations[4*index] give the servo target of this substep. */
+int bm_tool_step(const bm_params* p, const bm_state* s, const float* targets, const float* rotations,
+                 int32_t index, void* stream);
+
+/* External tool: accumulate the interval impulse, optional mobility prediction, reset
+ * the per-step sums (tool.py:70-115). */
+int bm_finish_external(const bm_params* p, const bm_state* s, void* stream);
+
+/* Interval sums for a driven knife (tool.py:6-16). */
+int bm_accumulate_reaction(const bm_params* p, const bm_state* s, void* stream);
+
+/* Side labels behind the blade edge; digit = 3^cut_index (contact.py:996-1039). */
+int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutting, void* stream);
+
+/* One full substep in the reference order (solver.py:920-940).  external selects
+ * finish_external_step instead of tool.step. */
+int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
+               int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
+               int32_t index, void* stream);
+
+/* External-tool port kernels (tool.py:19-68, material.py:6-20). */
+int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream);
+int bm_record_pose(const bm_params* p, const bm_state* s, int32_t index, float* out, void* stream);
+int bm_store_interval(const bm_params* p, const bm_state* s, float* out, void* stream);
+int bm_validate_interval(const bm_params* p, const bm_state* s, void* stream);
+
+/* Particles beside the blade body (contact.py:1042-1055); count is int32[1]. */
+int bm_count_embedded(const bm_params* p, const bm_state* s, int32_t* count, void* stream);
+
+/* Energy diagnostics (diagnostics.py:7-35): out[6] = kinetic, gravity, elastic, max speed,
+ * max z, min det F; the caller initialises out. */
+int bm_metrics(const bm_params* p, const bm_state* s, float* out, void* stream);
+
+/* Hencky von Mises projection of every particle (material.py:434-442, test probe). */
+int bm_plastic_project(const bm_params* p, const bm_state* s, void* stream);
+
+/* Probes of the contact functions (contact.py:21-43, 46-120, 181-232), count lanes each.
+ * blade_query: in local[3*count] (device) and half[3] (host) -> out[7*count] = closest 3,
+ * normal 3, distance. */
+int bm_probe_blade_query(const float* local, const float* half, float bevel, float* out, int32_t count,
+                         void* stream);
+/* friction: in[13*count] = normal 3, relative 3, jn, history 3, me, (pad 2); params
+ * dt, mus, mud, gk, gc; out[6*count] = impulse 3, history 3. */
+int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk, float gc, float* out,
+                      int32_t count, void* stream);
+/* 3x3 SVD of A[9*count] -> U[9], s[3], V[9] per lane. */
+int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM_HIP_H */

@@ -1,0 +1,371 @@
+"""Pins of the CPU oracle (oracle/tlmpm_oracle.cpp) before it is trusted as the checker.
+
+Each oracle function is checked against the reference's own known answers and golden
+vectors, or against a property the mathematics fixes, chosen so that a dropped term, a
+wrong sign or index fails:
+
+  blade_query      exact bevel-face results (reference tests/test_mechanics.py:151-180);
+                   flat faces against a brute-force closest point on a dense surface sample
+  friction         slip magnitude mu_d * j_n (test_mechanics.py:148); stick case closed form
+  resist_cut       damage [1,1,1,.5,.54,1], work 4e-5, returned impulses
+                   (tests/test_cut_resistance.py:45-67)
+  couple (Eq. 8)   momentum conservation; all-compatible fields equalise to m_a S / M;
+                   incompatible fields untouched (test_mechanics.py:80-105 fixture)
+  g2p              known answers of tests/test_particle_transport.py
+  svd3             reconstruction, orthogonality, singular values of numpy.linalg.svd
+  stress           P = 2 mu (F - R) + lam (J - 1) J F^-T with R from scipy.linalg.polar
+  von Mises        det Fp = 1, deviatoric norm on the yield surface (test_mechanics.py:303-314)
+  rigid contact    material + reaction - board - gravity = 0 (test_mechanics.py:423-437)
+  effective mass   articulated / prescribed = 1 / (1 + m u^T M u) (tests/test_coupling.py:234-257)
+  pair contact     equal-opposite impulses, compatible neighbours excluded, exactly one
+                   pair for the reference's two-particle fixture (tests/test_pair_list.py:11-46)
+  assign           codes in {0, 1, 2}, digit persistence (test_mechanics.py:228-300)
+  tool.step        free-fall closed form (tests/test_multi_tool.py:68-79)
+  predict mode     x = 0.01 t, q_z = sin(0.1 t) (tests/test_coupling.py:322-339)
+  compatible       tests/test_topology.py:10-12
+
+Parity unpinned: the exact float32 bits of the reference's wp.svd3 and of Warp's quaternion
+code generation (no Warp in this environment); the oracle's SVD is evaluated in float64.
+"""
+
+import numpy as np
+import pytest
+from scipy.linalg import polar
+from scipy.spatial.transform import Rotation
+
+import oracle
+from oracle import OracleSim
+from blademaster_hip.config import Config
+from blademaster_hip.geometry import points, stencil
+
+
+def sim(cfg, precision=64):
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, weights, gradients = stencil(ref, origin, cfg.grid_spacing, cells)
+    return OracleSim(cfg, ref, nodes, weights, gradients, int(np.prod(cells + 1)), precision), ref
+
+
+def test_compatible_known_cases():
+    assert not oracle.compatible(1, 2)
+    assert not oracle.compatible(4, 7)
+    assert oracle.compatible(1, 4)
+    assert oracle.compatible(0, 3**30 * 2)
+    assert not oracle.compatible(3**33 * 2 + 1, 3**33 * 1 + 1)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_bevel_face_is_exact_near_contact(precision):
+    half = np.array([0.0005, 0.04, 0.02], np.float32)
+    x = np.array(
+        [[s * (half[0] + gap), y, z] for s in (-1, 1) for gap in (1e-8, 1e-7, 1e-6) for y in (-0.03, 0.01)
+         for z in (-0.01476572, -0.005, 0.00126425, 0.019)],
+        np.float32,
+    )
+    for p in x:
+        c, nrm, d = oracle.blade_query(p, half, 0.005, precision)
+        if precision == 32:
+            assert np.float32(c[0]) == np.sign(p[0]) * half[0] and c[1] == p[1] and c[2] == p[2]
+            assert np.float32(d) == np.abs(p[0]) - half[0]
+        np.testing.assert_allclose(nrm, [np.sign(p[0]), 0, 0], atol=1e-7)
+
+
+@pytest.mark.parametrize("bevel", [0.0, 0.005])
+def test_blade_distance_against_brute_force_surface(bevel):
+    """Points outside the cross-section: distance to the polygon boundary by point-segment projection."""
+    half = np.array([0.0005, 0.025, 0.02])
+    rng = np.random.default_rng(2)
+    # dense surface sample of the cross-section polygon in (x, z), extruded in y
+    if bevel:
+        poly = np.array([[0, -half[2]], [half[0], -half[2] + bevel], [half[0], half[2]], [-half[0], half[2]],
+                         [-half[0], -half[2] + bevel], [0, -half[2]]])
+    else:
+        poly = np.array([[-half[0], -half[2]], [half[0], -half[2]], [half[0], half[2]], [-half[0], half[2]],
+                         [-half[0], -half[2]]])
+    def segment_distance(q):
+        best = np.inf
+        for a, b in zip(poly[:-1], poly[1:]):
+            t = np.clip(np.dot(q - a, b - a) / np.dot(b - a, b - a), 0.0, 1.0)
+            best = min(best, np.linalg.norm(q - (a + t * (b - a))))
+        return best
+
+    checked = 0
+    while checked < 60:
+        p = rng.uniform([-0.004, -0.02, -0.025], [0.004, 0.02, 0.025])
+        inside_xz = np.all(np.abs(p[[0, 2]]) <= half[[0, 2]])
+        if inside_xz:
+            continue
+        c, nrm, d = oracle.blade_query(p, half, bevel, 64)
+        brute = segment_distance(p[[0, 2]])
+        assert d == pytest.approx(brute, abs=1e-12)
+        np.testing.assert_allclose(np.linalg.norm(nrm), 1.0, atol=1e-12)
+        np.testing.assert_allclose(p - c, d * nrm, atol=1e-10)
+        checked += 1
+
+
+def test_friction_slip_magnitude_and_stick_history():
+    args = dict(dt=0.001, mus=0.3, mud=0.2, gk=0.05, gc=0.1)
+    n = np.array([0.0, 0.0, 1.0])
+    j, h = oracle.friction(n, [1.0, 0.1, 0.3], 0.003, [0.0002, 0.0003, 0.0], 0.02, **args)
+    assert np.linalg.norm(j) == pytest.approx(0.2 * 0.003, rel=2e-6)
+    assert j[2] == 0.0
+    # slip: the returned history reproduces the applied impulse through the spring-damper
+    k = 0.05 * 0.02 / 0.001**2
+    damping = 2 * 0.1 * np.sqrt(0.02 * k)
+    vt = np.array([1.0, 0.1, 0.0])
+    np.testing.assert_allclose((-k * h - damping * vt) * 0.001, j, rtol=1e-12, atol=1e-18)
+    # stick: small sliding speed, no history
+    j, h = oracle.friction(n, [0.001, 0.0, 0.0], 0.003, [0.0, 0.0, 0.0], 0.02, **args)
+    np.testing.assert_allclose(h, [0.001 * 0.001, 0.0, 0.0], rtol=1e-12)
+    np.testing.assert_allclose(j, [-damping * 0.001 * 0.001, 0.0, 0.0], rtol=1e-12)
+    # normal history and velocity components are projected out
+    j, h = oracle.friction(n, [0.0, 0.0, 5.0], 0.003, [0.0, 0.0, 0.2], 0.02, **args)
+    np.testing.assert_array_equal(j, 0.0)
+    np.testing.assert_array_equal(h, 0.0)
+
+
+@pytest.mark.parametrize("angle", [0, 35])
+def test_resist_cut_known_answer(angle):
+    cfg = Config(size=(0.012, 0.012, 0.012), grid_spacing=0.006, blade_half_size=(0.0005, 0.025, 0.02))
+    rot = Rotation.from_euler("y", angle, degrees=True)
+    edge = np.array([0.0, 0.0, 0.02]) + rot.apply([0.0, 0.0, -0.02])
+    positions = edge + rot.apply([[0, 0, -0.001]] * 5 + [[0, 0, 0.001]])
+    velocities = rot.apply([[0, 0, -s] for s in [0.01, 0, -0.01, -0.01, 0.01, 0]])
+    o = _resist_probe_sim(cfg, threshold=0.001)
+    x = np.zeros((o.n, 3))
+    x[:6] = positions
+    damage = np.zeros(o.n)
+    damage[:6] = [1, 1, 1, 0.5, 0.5, 1]
+    o.set("x", x)
+    o.set("v", np.zeros_like(x))
+    o.set("damage", damage)
+    q = rot.as_quat()
+    j = rot.apply([0.002, 0.001, -0.004])
+    results = [o.resist_cut(p, j, [0, 0, 0.02], velocities[p], q, [0, 0, 0], 0.0015, 1) for p in range(6)]
+    expected = rot.apply([[0.002, 0.001, 0]] * 3 + [[0.002, 0.001, -0.004]] * 3)
+    np.testing.assert_allclose(results, expected, atol=2e-9)
+    np.testing.assert_allclose(o.get("damage")[:6], [1, 1, 1, 0.5, 0.54, 1], atol=1e-7)
+    assert o.get("work")[0] == pytest.approx(0.004 * 0.01, rel=1e-6)
+
+
+def _resist_probe_sim(cfg, threshold):
+    """Oracle simulation whose damage threshold is replaced by the probe's value."""
+    import ctypes
+
+    o, _ = sim(cfg, 64)
+    o.c.threshold = threshold
+    o.lib.oracle_destroy(o.handle)
+    o.handle = o.lib.oracle_create(64, ctypes.byref(o.c))
+    return o
+
+
+def _couple_fixture():
+    codes = np.array([0, 1, 2, 4, 7], np.int64)
+    masses = np.array([0.2, 0.5, 0.7, 0.3, 0.4])
+    states = np.random.default_rng(3).normal(size=(5, 3))
+    heads = np.array([0], np.int32)
+    links = np.array([1, 2, 3, 4, -1], np.int32)
+    return codes, masses, states, heads, links
+
+
+def test_couple_conserves_momentum_and_respects_compatibility():
+    codes, masses, states, heads, links = _couple_fixture()
+    out = oracle.couple(codes, heads, links, masses, states, 1)
+    np.testing.assert_allclose(out.sum(0), states.sum(0), atol=1e-14)
+    # field 0 (uncut) is compatible with all: it gains from every other field
+    assert np.abs(out[0] - states[0]).max() > 1e-3
+
+
+def test_couple_all_compatible_equalises_and_incompatible_is_untouched():
+    masses = np.array([0.2, 0.5, 0.7])
+    states = np.random.default_rng(4).normal(size=(3, 3))
+    heads = np.array([0], np.int32)
+    links = np.array([1, 2, -1], np.int32)
+    out = oracle.couple(np.zeros(3, np.int64), heads, links, masses, states, 1)
+    np.testing.assert_allclose(out, masses[:, None] * states.sum(0) / masses.sum(), rtol=1e-13)
+    out = oracle.couple(np.array([4, 5, 7], np.int64), heads, links, masses, states, 1)
+    np.testing.assert_allclose(out, states, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("flip", [0.0, 0.95, 1.0])
+def test_g2p_known_answers(flip):
+    slots = np.tile(np.arange(8), (2, 1))
+    weights = np.full((2, 8), 1 / 8)
+    x0 = np.array([[0.0, 0.0, 0.0], [0.003, 0.0, 0.0]])
+    v0 = np.array([[10.0, 0.0, 0.0], [-10.0, 0.0, 0.0]])
+    rest = np.zeros((8, 3))
+    x, v = oracle.g2p(x0, v0, slots, weights, rest, rest, 0.001, flip)
+    np.testing.assert_array_equal(x, x0)
+    np.testing.assert_allclose(v, flip * v0, atol=1e-12)
+    old = np.tile([1.0, 2.0, 3.0], (8, 1))
+    new = np.tile([2.0, 4.0, 6.0], (8, 1))
+    x, v = oracle.g2p(np.zeros((1, 3)), [[7.0, 2.0, -1.0]], slots[:1], weights[:1], old, new, 0.001, flip)
+    np.testing.assert_allclose(x, [[0.002, 0.004, 0.006]], rtol=1e-12)
+    np.testing.assert_allclose(v, [[2.0 + 6 * flip, 4.0, 6.0 - 4 * flip]], rtol=1e-12)
+
+
+def test_svd3_against_numpy():
+    rng = np.random.default_rng(7)
+    mats = [np.eye(3), np.diag([1.3, 1 / 1.3, 1.0]), np.diag([2.0, 2.0, 0.5]), -np.eye(3)]
+    mats += [np.eye(3) + 0.2 * rng.normal(size=(3, 3)) for _ in range(200)]
+    mats += [rng.normal(size=(3, 3)) for _ in range(200)]
+    for A in mats:
+        U, s, V = oracle.svd3(A)
+        np.testing.assert_allclose(U @ np.diag(s) @ V.T, A, atol=1e-12)
+        np.testing.assert_allclose(U @ U.T, np.eye(3), atol=1e-12)
+        np.testing.assert_allclose(V @ V.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(U) > 0 and np.linalg.det(V) > 0
+        np.testing.assert_allclose(np.abs(s), np.linalg.svd(A, compute_uv=False), rtol=1e-12, atol=1e-14)
+        assert s[0] >= 0 and s[1] >= 0
+        assert np.sign(s.prod()) == np.sign(np.linalg.det(A)) or abs(np.linalg.det(A)) < 1e-14
+
+
+def test_corotated_stress_closed_form():
+    cfg = Config()
+    o, _ = sim(cfg, 64)
+    rng = np.random.default_rng(11)
+    F = np.eye(3) + 0.05 * rng.normal(size=(o.n, 3, 3))
+    o.set("F", F)
+    o.stage("refresh_stress")
+    P = o.get("stress").reshape(-1, 3, 3)
+    k = o.k
+    for f, p in zip(F[:50], P[:50]):
+        R, _ = polar(f)
+        J = np.linalg.det(f)
+        expected = 2 * k["mu"] * (f - R) + k["lam"] * (J - 1) * J * np.linalg.inv(f).T
+        np.testing.assert_allclose(p, expected, rtol=1e-9, atol=1e-9 * np.abs(expected).max())
+
+
+def test_von_mises_projection():
+    cfg = Config(yield_stress=100.0)
+    o, _ = sim(cfg, 32)
+    total = np.tile(np.diag([1.3, 1 / 1.3, 1.0]), (o.n, 1, 1)).astype(np.float32)
+    o.set("F", total)
+    o.stage("plastic_project")
+    fp = o.get("Fp").reshape(-1, 3, 3)
+    np.testing.assert_array_equal(o.get("F").reshape(-1, 3, 3), total)
+    np.testing.assert_allclose(np.linalg.det(fp), 1, atol=3e-6)
+    fe = total @ np.linalg.inv(fp)
+    logs = np.log(np.linalg.svd(fe, compute_uv=False))
+    dev = logs - logs.mean(1)[:, None]
+    np.testing.assert_allclose(np.linalg.norm(dev, axis=1), 100 / (2 * o.k["mu"]) * np.sqrt(2 / 3), atol=2e-6)
+
+
+def test_rigid_contact_momentum_balance_with_rotated_blade():
+    cfg = Config(free_blade=True)
+    o, _ = sim(cfg, 64)
+    theta = 0.3
+    o.set("tool_rot", [0, np.sin(theta / 2), 0, np.cos(theta / 2)])
+    o.set("tool_pos", [0.0, 0.0, 0.02])
+    o.stage("rigid", 0)
+    gravity = np.array([0.0, 0.0, -9.81 * o.k["mass"] * cfg.dt]) * o.n
+    material = o.get("impulses").reshape(-1, 3).sum(0)
+    np.testing.assert_allclose(material + o.get("reaction") - o.get("board") - gravity, 0, atol=1e-15)
+    assert np.linalg.norm(o.get("reaction")) > 0
+
+
+def test_articulated_effective_mass_ratio():
+    cfg = Config(external_tool=True)
+    o, _ = sim(cfg, 64)
+    o.set("tool_pos", [0, 0, 0.15])
+    x = o.get("x").reshape(-1, 3)
+    x[0] = [cfg.blade_half_size[0] + o.k["radius"] / 2, 0.01, 0.15]
+    o.set("x", x)
+    mobility = np.diag([1000, 2000, 3000, 4000, 5000, 6000.0])
+
+    def impulse(a):
+        o.set("tool_mobility", a)
+        o.set("reaction", np.zeros(3))
+        o.set("torque", np.zeros(3))
+        o.set("histories", np.zeros(6 * o.n))
+        o.stage("rigid", 0)
+        return o.get("reaction")[0]
+
+    prescribed = impulse(np.zeros((6, 6)))
+    articulated = impulse(mobility)
+    direction = np.array([1, 0, 0, 0, 0, -0.01])
+    assert prescribed < 0
+    assert articulated / prescribed == pytest.approx(1 / (1 + o.k["mass"] * (direction @ mobility @ direction)),
+                                                     rel=1e-12)
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_pair_contact_equal_opposite_and_reference_exclusion(split):
+    cfg = Config(size=(0.018, 0.018, 0.018), grid_spacing=0.006, dt=0.00025)
+    o, _ = sim(cfg, 64)
+    x = np.zeros((o.n, 3))
+    x[:, 0] = np.arange(o.n) * 0.1 + 2
+    x[:, 2] = 0.1
+    x[:2] = [[0, 0, 0.1], [0.0048, 0, 0.1]]
+    v = np.zeros_like(x)
+    v[:2] = [[16.0, 0, 0], [-16.0, 0, 0]]
+    codes = np.zeros(o.n, np.int64)
+    codes[:2] = [1, 2] if split else [0, 0]
+    # the two particles are reference neighbours (< 2h apart)
+    ref = o.get("ref").reshape(-1, 3)
+    assert np.linalg.norm(ref[0] - ref[1]) < 2 * cfg.grid_spacing
+    o.set("x", x)
+    o.set("v", v)
+    o.set("codes", codes)
+    o.set("impulses", np.zeros((o.n, 3)))
+    o.set("speed", [16.0])
+    o.stage("pairs")
+    imp = o.get("impulses").reshape(-1, 3)
+    assert o.scalars()["pair_count"] == (1 if split else 0)
+    np.testing.assert_allclose(imp.sum(0), 0, atol=1e-18)
+    assert bool(np.any(imp)) == split
+    if split:
+        assert imp[0, 0] < 0 < imp[1, 0]
+
+
+def test_assign_labels_persist():
+    cfg = Config()
+    o, ref = sim(cfg, 32)
+    o.set("tool_pos", [0.0, 0.0, 0.03])
+    o.stage("assign", 1, 1)
+    first = o.get("codes")
+    assert set(np.unique(first)) == {0, 1, 2}
+    assert ref[first == 1, 0].max() < 0 < ref[first == 2, 0].min()
+    o.set("tool_pos", [0.02, 0.0, 0.1])
+    o.stage("assign", 1, 1)
+    np.testing.assert_array_equal(o.get("codes"), first)
+    o.set("tool_pos", [0.012, 0.0, 0.03])
+    o.stage("assign", 1, 3)
+    np.testing.assert_array_equal(o.get("codes") % 3, first)
+    assert o.get("codes").max() > 2
+
+
+def test_free_knife_beside_the_block_falls_freely():
+    cfg = Config(size=(0.06, 0.03, 0.03), grid_spacing=0.0075, dt=0.0005, free_blade=True,
+                 blade_half_size=(0.0005, 0.025, 0.02))
+    o, _ = sim(cfg, 32)
+    z0 = 0.2
+    o.set("tool_pos", [0.2, 0.0, z0])
+    n = 100
+    for _ in range(n):
+        o.substep(cutting=True)
+    dt, g = cfg.dt, 9.81
+    assert o.get("tool_vel")[2] == pytest.approx(-g * n * dt, rel=1.0e-5)
+    assert o.get("tool_pos")[2] == pytest.approx(z0 - g * dt * dt * n * (n + 1) / 2, abs=1.0e-6)
+    np.testing.assert_allclose(o.get("tool_pos")[:2], [0.2, 0.0], atol=1e-9)
+
+
+def test_predict_mode_integrates_the_twist():
+    cfg = Config(external_tool=True, tool_substep="predict")
+    o, _ = sim(cfg, 64)
+    o.set("tool_pos", [0, 0, 0.15])
+    o.set("tool_vel", [0.01, 0, 0])
+    o.set("tool_omega", [0, 0, 0.2])
+    o.set("tool_rot", [0, 0, 0, 1])
+    steps = 7
+    for _ in range(steps):
+        o.stage("finish_external")
+    t = steps * cfg.dt
+    assert o.get("tool_pos")[0] == pytest.approx(0.01 * t, rel=1e-12)
+    assert o.get("tool_rot")[2] == pytest.approx(np.sin(0.2 * t / 2), rel=1e-12)
+
+
+def test_max_cut_index_capacity():
+    from blademaster_hip.config import max_cut_index
+
+    assert max_cut_index(225) == 33
+    assert 225 * 3**34 - 1 <= np.iinfo(np.int64).max < 225 * 3**35 - 1
