@@ -1,0 +1,246 @@
+"""Throughput of the TLMPM cutting substep on one MI355X.
+
+Workload (N=1): a 200 x 100 x 50 = 1,000,000-particle block on a 1.5 mm lattice, 3 mm grid
+(133,926 nodes), fixed-corotated elasticity with E = 10**U(4, 6) drawn from seed 0 and
+nu = 0.3, a blade tilted 10 degrees about its thickness axis plunging at 0.1 m/s
+(prescribed kinematics: external tool re-posed every 1 ms coupling interval, integrated at
+its twist inside the interval), 25 us substeps, cutting on.  One bench step is one substep
+of all particles; intervals of 40 substeps are replayed as HIP graphs.
+
+Prints one JSON line with particle-substeps/s, the HBM roofline of the dominant kernel
+(live HIP-event timing of that kernel on the bench stream) and the CPU oracle timed on a
+bounded sample of the same workload on this host.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "blademaster-lab_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle-substep of the reference substep (x, v, F, Fp, stress,
+# codes, friction histories, stencil slots/weights/gradients), attributed to the kernel of
+# this build that first reads / last writes each array; the four shares sum to 444 B.
+ALG_BYTES = {"k_rigid": 80, "k_p2g": 196, "k_g2p": 24, "k_update_F": 144}
+SUBSTEP_BYTES = 444
+INTERVAL_SUBSTEPS = 40
+
+
+def workload(seed=0, n_side=(0.3, 0.15, 0.075)):
+    from blademaster_hip import Config
+
+    rng = np.random.default_rng(seed)
+    E = float(10 ** rng.uniform(4, 6))
+    cfg = Config(
+        size=n_side,
+        grid_spacing=0.003,
+        dt=25e-6,
+        young_modulus=E,
+        poisson_ratio=0.3,
+        external_tool=True,
+        tool_substep="predict",
+        cache_topology=True,
+        grid_storage="compact",
+        blade_half_size=(0.0005, 0.08, 0.03),
+    )
+    tilt = np.deg2rad(10.0)
+    q = np.array([np.sin(tilt / 2), 0.0, 0.0, np.cos(tilt / 2)])
+    hz = cfg.blade_half_size[2]
+    # lowest edge point 2 mm above the top face
+    edge_z0 = cfg.size[2] + 0.002 + cfg.blade_half_size[1] * np.sin(tilt)
+    return cfg, q, edge_z0, hz, E
+
+
+def pose_at(t, q, edge_z0, hz, speed=0.1):
+    from scipy.spatial.transform import Rotation
+
+    edge = np.array([0.0, 0.0, edge_z0 - speed * t])
+    origin = edge + Rotation.from_quat(q).apply([0.0, 0.0, hz])
+    return origin
+
+
+def run_gpu(args):
+    import torch
+    from blademaster_hip import Solver, ExternalToolPort, ToolState
+    from blademaster_hip import _lib
+
+    cfg, q, edge_z0, hz, E = workload()
+    s = Solver(cfg)
+    port = ExternalToolPort(s)
+    interval = INTERVAL_SUBSTEPS * cfg.dt
+    vel = np.array([0.0, 0.0, -0.1])
+
+    def submit(k):
+        t = k * interval
+        state = ToolState(pose_at(t, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0])
+        port.submit(state, interval, cutting=True, cut_index=0)
+        return state
+
+    total = args.steps + args.warmup
+    intervals = -(-total // INTERVAL_SUBSTEPS)
+    warm_intervals = -(-args.warmup // INTERVAL_SUBSTEPS)
+    timed_intervals = intervals - warm_intervals
+    for k in range(warm_intervals):
+        submit(k)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    start.record(stream)
+    last = None
+    for k in range(warm_intervals, intervals):
+        last = submit(k)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    gpu_ms = start.elapsed_time(stop)
+    steps_timed = timed_intervals * INTERVAL_SUBSTEPS
+    reaction = port.collect(last, interval, 2)
+    st = s.state()
+    ms_per_step = gpu_ms / steps_timed
+    rate = s.n * steps_timed / (gpu_ms * 1e-3)
+    # Live per-kernel timing of the dominant kernels on the bench stream.
+    kernels = {
+        "k_rigid": ("bm_rigid_contacts", (1,)),
+        "k_p2g": ("bm_p2g", ()),
+        "k_g2p": ("bm_g2p_remap", ()),
+        "k_update_F": ("bm_update_F", ()),
+    }
+    per_kernel_ms = {}
+    reps = args.kernel_reps
+    saved = {name: t.clone() for name, t in s._t.items() if name in ("x", "v", "F", "Fp", "stress", "impulses",
+                                                                        "histories", "damage", "codes", "work")}
+    for kname, (entry, extra) in kernels.items():
+        s.launch(entry, *extra)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            s.launch(entry, *extra)
+        b.record(stream)
+        torch.cuda.synchronize()
+        per_kernel_ms[kname] = a.elapsed_time(b) / reps
+    for name, t in saved.items():
+        s._t[name].copy_(t)
+    dominant = max(per_kernel_ms, key=per_kernel_ms.get)
+    t_dom = per_kernel_ms[dominant] * 1e-3
+    achieved = ALG_BYTES[dominant] * s.n / t_dom / 1e9
+    traffic = None
+    if args.traffic_file and os.path.exists(args.traffic_file):
+        traffic = json.load(open(args.traffic_file)).get(dominant)
+    return dict(
+        solver=s,
+        cfg=cfg,
+        E=E,
+        rate=rate,
+        ms_per_step=ms_per_step,
+        wall=wall,
+        gpu_ms=gpu_ms,
+        steps=steps_timed,
+        per_kernel_ms=per_kernel_ms,
+        dominant=dominant,
+        achieved=achieved,
+        traffic=traffic,
+        reaction=reaction,
+        codes_nonzero=int(np.count_nonzero(st["sides"])),
+    )
+
+
+def run_cpu_baseline(cfg, q, edge_z0, hz, substeps):
+    sys.path.insert(0, ROOT)
+    from oracle import OracleSim
+    from blademaster_hip.geometry import points, stencil
+    from blademaster_hip.config import max_cut_index
+
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, w, g = stencil(ref, origin, cfg.grid_spacing, cells)
+    node_count = int(np.prod(cells + 1))
+    o = OracleSim(cfg, ref, nodes, w, g, node_count, 32, max_cut_index(node_count))
+    o.set("tool_pos", pose_at(0.0, q, edge_z0, hz))
+    o.set("tool_rot", q)
+    o.set("tool_vel", [0.0, 0.0, -0.1])
+    o.substep(cutting=True)  # first substep registers the topology (setup)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        o.substep(cutting=True)
+    dt = time.perf_counter() - t0
+    return len(ref) * substeps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--cpu-substeps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_1M_block.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the path is single-device, see DESIGN.md (sharding)", file=sys.stderr)
+        sys.exit(2)
+    r = run_gpu(args)
+    s, cfg = r["solver"], r["cfg"]
+    cpu = None
+    if not args.no_cpu:
+        _, q, edge_z0, hz, _ = workload()
+        cpu_rate, cpu_s = run_cpu_baseline(cfg, q, edge_z0, hz, args.cpu_substeps)
+        cpu = dict(
+            value=cpu_rate,
+            unit="particle-substeps/s",
+            cores=1,
+            kind="port",
+            sample=f"{args.cpu_substeps} substeps of the same 1M-particle workload after one setup substep, "
+            f"float32 oracle, one thread, {cpu_s:.1f} s",
+        )
+    peak = HBM_PEAK_GBS
+    line = {
+        "metric": "particle-substeps/s",
+        "value": r["rate"],
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "block 200x100x50 = 1M particles, 1.5 mm lattice, 3 mm grid, blade 10 deg tilt plunging "
+            "0.1 m/s, dt 25 us, cutting",
+            "particles": s.n,
+            "grid_nodes": s.node_count,
+            "young_modulus": r["E"],
+            "poisson_ratio": cfg.poisson_ratio,
+            "interval_substeps": INTERVAL_SUBSTEPS,
+        },
+        "hbm_peak_frac": SUBSTEP_BYTES * r["rate"] / (peak * 1e9),
+        "sim_per_wall": r["steps"] * cfg.dt / r["wall"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": r["dominant"],
+            "achieved": r["achieved"],
+            "peak": peak,
+            "unit": "GB/s",
+            "frac": r["achieved"] / peak,
+            "traffic": r["traffic"],
+            "alg_bytes_per_particle": ALG_BYTES[r["dominant"]],
+        },
+        "kernel_ms": {k: round(v, 5) for k, v in r["per_kernel_ms"].items()},
+        "cut_particles": r["codes_nonzero"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,6 +31,10 @@ class Params(ctypes.Structure):
         ("lock_rotation", ctypes.c_int64),
         ("predict", ctypes.c_int64),
         ("plastic", ctypes.c_int64),
+        ("brick", ctypes.c_int64 * 3),
+        ("bricks", ctypes.c_int64 * 3),
+        ("rigid_blocks", ctypes.c_int64),
+        ("blade_hold", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)
         for name in (
@@ -67,7 +71,12 @@ STATE_FIELDS = [
     ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
     ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("bucket_count", c_i32p),
     ("bucket_start", c_i32p), ("scan_partial", c_i32p), ("particle_bucket", c_i32p), ("particle_rank", c_i32p),
-    ("sorted_x", c_f32p),
+    ("sorted_x", c_f32p), ("perm", c_i32p), ("cell_start", c_i32p), ("p2g_partial", c_f32p),
+    ("remap_partial", c_f32p), ("rigid_partial", c_f32p), ("counters", ctypes.POINTER(ctypes.c_uint32)),
+    ("bc_side", c_f32p), ("bc_normal", c_f32p), ("bc_lever", c_f32p), ("bc_rhs", c_f32p), ("bc_bound", c_f32p),
+    ("bc_impulse", c_f32p), ("bc_extrapolated", c_f32p), ("bc_active", c_i32p), ("bc_nodes", c_i32p),
+    ("bc_node_mark", c_i32p), ("bc_grid", c_f32p), ("bc_coupled", c_f32p), ("bc_counts", c_i32p),
+    ("bc_sums", c_f32p),
 ]
 
 
@@ -75,6 +84,29 @@ class State(ctypes.Structure):
     _fields_ = STATE_FIELDS
 
 
+class MlsParams(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("dims", ctypes.c_int64 * 3),
+        ("plastic", ctypes.c_int64),
+        ("origin", ctypes.c_double * 3),
+    ] + [(name, ctypes.c_double) for name in
+         "dx dt mass volume mu lam damaged_shear softening compressed stretched".split()]
+
+
+MLS_STATE_FIELDS = [
+    (name, c_f32p) for name in
+    "x v C F damage damage_reference yield_stress impulse grid_m grid_p grid_v interval_impulse "
+    "interval_angular".split()
+] + [("inversions", c_i32p)]
+
+
+class MlsState(ctypes.Structure):
+    _fields_ = MLS_STATE_FIELDS
+
+
+MP = ctypes.POINTER(MlsParams)
+MS = ctypes.POINTER(MlsState)
 P = ctypes.POINTER(Params)
 S = ctypes.POINTER(State)
 VP = ctypes.c_void_p
@@ -111,6 +143,12 @@ SIGNATURES = {
     "bm_probe_blade_query": [VP, ctypes.POINTER(ctypes.c_float), F32, VP, I32, VP],
     "bm_probe_friction": [VP, F32, F32, F32, F32, F32, VP, I32, VP],
     "bm_probe_svd3": [VP, VP, VP, VP, I32, VP],
+    "bm_blade_constraint": [P, S, VP],
+    "bm_store_residual": [P, S, VP, I32, VP],
+    "bm_mls_p2g": [MP, MS, VP],
+    "bm_mls_grid_update": [MP, MS, VP],
+    "bm_mls_g2p": [MP, MS, VP],
+    "bm_mls_substep": [P, S, MP, MS, VP],
 }
 
 _lib = None

@@ -95,6 +95,16 @@ class ExternalToolPort:
     def advance(self, state, duration_s, *, cutting=True, cut_index=0, graph=True):
         """Run one coupling interval and return its reaction (coupling.py:88-189)."""
         submission_begin = time.perf_counter()
+        state, steps, samples = self.submit(state, duration_s, cutting=cutting, cut_index=cut_index, graph=graph)
+        return self.collect(state, duration_s, samples, submission_begin)
+
+    def submit(self, state, duration_s, *, cutting=True, cut_index=0, graph=True):
+        """Enqueue one interval on the current stream without waiting for it.
+
+        Returns (checked state, substeps, trajectory samples); `collect` reads the result.
+        Consecutive submissions are ordered on the stream, so a caller may enqueue many
+        intervals and read back only the last one.
+        """
         state = state.checked()
         s = self.solver
         if not np.isfinite(duration_s) or duration_s <= 0:
@@ -155,9 +165,14 @@ class ExternalToolPort:
                 if sample is not None:
                     sample(i)
             finish()
+        return state, steps, samples
+
+    def collect(self, state, duration_s, samples, submission_begin=None):
+        """Wait for the last submitted interval and return its Reaction."""
         readback_begin = time.perf_counter()
-        self.timing_s["submission"] += readback_begin - submission_begin
-        out = result.cpu().numpy().astype(np.float64)
+        if submission_begin is not None:
+            self.timing_s["submission"] += readback_begin - submission_begin
+        out = self.result.cpu().numpy().astype(np.float64)
         self.timing_s["completion_and_readback"] += time.perf_counter() - readback_begin
         impulse, angular, errors = out[:3], out[3:6], out[6:9]
         if self.record_trajectory:

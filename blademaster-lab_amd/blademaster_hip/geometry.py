@@ -120,3 +120,44 @@ def compact_stencil(reference, origin, spacing, cells):
     dims = np.asarray(cells) + 1
     base_node = (base[:, 0] * dims[1] + base[:, 1]) * dims[2] + base[:, 2]
     return base_node.astype(np.int32), local.astype(np.float32), base.astype(np.int32)
+
+
+# Cells per brick along x and y of the particle bins; z bricks are as tall as the LDS window
+# of the G2P kernel allows (at most 32 cells).
+BRICK_XY = 2
+LDS_BYTES = 65536
+
+
+def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY):
+    """Particle bins of the brick kernels.
+
+    Returns (brick cells per axis, bricks per axis, perm, cell_start): bricks are numbered
+    x-major, the cells of a brick z-fastest, and the particles of the c-th cell of the grid in
+    this order are perm[cell_start[c]:cell_start[c + 1]] (reference order within a cell).
+    The G2P kernel keeps 9 floats per (field slot, window node) in LDS, which bounds the
+    brick's window.
+    """
+    cells = np.asarray(cells, dtype=np.int64)
+
+    def fits(bx, by, bz):
+        return 9 * fields * (bx + 1) * (by + 1) * (bz + 1) * 4 <= LDS_BYTES
+
+    bx = by = brick_xy
+    bz = int(min(cells[2], 32))
+    while bz > 1 and not fits(bx, by, bz):
+        bz -= 1
+    while not fits(bx, by, bz) and (bx > 1 or by > 1):
+        bx, by = max(1, bx - 1), max(1, by - 1)
+    if not fits(bx, by, bz):
+        raise ValueError("fields_per_node too large for the LDS window of one brick")
+    brick = np.array([bx, by, bz], dtype=np.int64)
+    bricks = -(-cells // brick)
+    base_cell = np.asarray(base_cell, dtype=np.int64)
+    ids = ((base_cell[:, 0] // bx) * bricks[1] + base_cell[:, 1] // by) * bricks[2] + base_cell[:, 2] // bz
+    local = ((base_cell[:, 0] % bx) * by + base_cell[:, 1] % by) * bz + base_cell[:, 2] % bz
+    per_brick = int(bx * by * bz)
+    key = ids * per_brick + local
+    perm = np.argsort(key, kind="stable").astype(np.int32)
+    nb = int(np.prod(bricks))
+    cell_start = np.searchsorted(key[perm], np.arange(nb * per_brick + 1)).astype(np.int32)
+    return brick, bricks, perm, cell_start

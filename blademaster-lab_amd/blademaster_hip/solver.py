@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .arrays import DArray
 from .config import Config, derived, max_cut_index
-from .geometry import points, stencil, compact_stencil
+from .geometry import points, stencil, compact_stencil, bin_bricks
 
 # Listed pair partners per particle before the bucket fallback (solver.py:152-154).
 PAIR_LIST_LANES = 16
@@ -33,6 +33,26 @@ class _Device:
         self.torch = torch.device(device)
         self.name = str(self.torch)
         self.is_cuda = self.torch.type == "cuda"
+
+
+class BladeConstraint:
+    """Held-blade grid constraint state (blade_constraint.py:213-324); the solve itself runs
+    inside the material step of the C-ABI."""
+
+    def __init__(self, solver):
+        self.solver = solver
+        T = solver._t
+        self.side = DArray(T["bc_side"])
+        self.impulse = DArray(T["bc_impulse"])
+        self.max_violation = DArray(T["bc_sums"][6:7])
+        self.residual = DArray(T["bc_sums"][7:8])
+        self.batches = DArray(T["bc_counts"][3:4])
+
+    def solve(self):
+        self.solver._call("bm_blade_constraint")
+
+    def store_residual(self, result):
+        self.solver._call("bm_store_residual", ctypes.c_void_p(result.data_ptr()), int(result.numel()))
 
 
 class Solver:
@@ -83,7 +103,8 @@ class Solver:
         def arr(tensor, hook=None):
             return DArray(tensor, hook)
 
-        base_node, local, _ = compact_stencil(self.reference, self.origin, cfg.grid_spacing, cells)
+        base_node, local, base_cell = compact_stencil(self.reference, self.origin, cfg.grid_spacing, cells)
+        self._bin_bricks(base_cell)
         eye = torch.eye(3, dtype=f32, device=dev).expand(n, 3, 3).contiguous()
         self._t = {}
         T = self._t
@@ -143,11 +164,37 @@ class Solver:
         T["particle_bucket"] = t((n,), i32)
         T["particle_rank"] = t((n,), i32)
         T["sorted_x"] = t((n, 4))
+        T["perm"] = torch.tensor(self.brick_perm, dtype=i32, device=dev)
+        T["cell_start"] = torch.tensor(self.cell_start, dtype=i32, device=dev)
+        window = int(np.prod(np.asarray(self.brick) + 1))
+        nb = int(np.prod(self.bricks))
+        T["p2g_partial"] = t((max(nb * F_ * window * 7, 1),))
+        T["remap_partial"] = t((max(nb * F_ * window * 3, 1),))
+        self.rigid_blocks = max(1, -(-n // 256))
+        T["rigid_partial"] = t((self.rigid_blocks * 12,))
+        T["counters"] = t((4,), i32)
+        # Held-blade grid constraint (blade_constraint.py): allocated for an externally held tool.
+        self.blade_hold = bool(cfg.external_tool and cfg.tool_substep == "hold")
+        bn = n if self.blade_hold else 1
+        bN = N_ if self.blade_hold else 1
+        T["bc_side"] = t((bn,))
+        T["bc_normal"] = t((bn, 3))
+        T["bc_lever"] = t((bn, 3))
+        for name in ("bc_rhs", "bc_bound", "bc_impulse", "bc_extrapolated"):
+            T[name] = t((bn,))
+        T["bc_active"] = t((bn,), i32)
+        T["bc_nodes"] = t((bN,), i32)
+        T["bc_node_mark"] = t((bN,), i32)
+        T["bc_grid"] = t((F_ if self.blade_hold else 1, bN, 3))
+        T["bc_coupled"] = t((F_ if self.blade_hold else 1, bN, 3))
+        T["bc_counts"] = t((4,), i32)
+        T["bc_sums"] = t((8,))
         T["single_target"] = t((1, 3))
         T["single_rotation"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]], dtype=f32, device=dev)
 
         mark_topology = self._mark_topology
-        self.x, self.v = arr(T["x"]), arr(T["v"])
+        mark_pairs = self._mark_pairs
+        self.x, self.v = arr(T["x"], mark_pairs), arr(T["v"], mark_pairs)
         self.ref = arr(T["ref"])
         self.F, self.Fp, self.stress = arr(T["F"]), arr(T["Fp"]), arr(T["stress"])
         self.codes = arr(T["codes"], mark_topology)
@@ -186,7 +233,7 @@ class Solver:
         ]
         self._params = self._make_params()
         self._state = self._make_state()
-        self.blade_constraint = None
+        self.blade_constraint = BladeConstraint(self) if self.blade_hold else None
         self.refresh_stress()
 
     # ------------------------------------------------------------------ C-ABI plumbing
@@ -219,6 +266,11 @@ class Solver:
         p.drive_k, p.drive_c, p.drive_limit = cfg.drive_stiffness, cfg.drive_damping, cfg.drive_force_limit
         p.rot_k, p.rot_c = cfg.rotational_stiffness, cfg.rotational_damping
         p.pair_skin = self.pair_skin
+        for i in range(3):
+            p.brick[i] = int(self.brick[i])
+            p.bricks[i] = int(self.bricks[i])
+        p.rigid_blocks = self.rigid_blocks
+        p.blade_hold = int(self.blade_hold)
         return p
 
     def _make_state(self):
@@ -246,6 +298,16 @@ class Solver:
 
     def _mark_topology(self):
         self._t["topology_dirty"].fill_(1)
+        self._t["pair_dirty"].fill_(1)
+
+    def _mark_pairs(self):
+        # Host writes to x or v: rebuild the pair list before the next contact pass.
+        self._t["pair_dirty"].fill_(1)
+
+    def _bin_bricks(self, base_cell):
+        self.brick, self.bricks, self.brick_perm, self.cell_start = bin_bricks(
+            self.cells, base_cell, int(self.config.fields_per_node)
+        )
 
     def launch(self, name, *args):
         """Launch one C-ABI stage by name with the solver's params/state (cf. solver.py:331)."""

@@ -215,8 +215,35 @@ __device__ __forceinline__ float effective_mass(const bm_params& P, const float*
   return 1.0f / (1.0f / float(P.mass) + inv);
 }
 
-// rigid_and_speed (contact.py:332-497) with check_pair_list (contact.py:716-735) fused in.
-__global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_state S, int cut_bits) {
+// Modes of the contact kernel.
+enum RigidMode : int {
+  kAccumulate = 0,        // add the step sums to reaction/torque/board/work (standalone launch)
+  kOverwrite = 1,         // the sums replace them (clear_step + accumulate of one substep)
+  kToolStep = 2,          // then tool.step of a driven or free knife (last block)
+  kFinishExternal = 4,    // then finish_external_step (last block)
+  kAccumulateInterval = 8,// accumulate_reaction before the tool step
+  kPairPrep = 16,         // clear stale pair slots and bin particles for a pending list rebuild
+};
+
+struct ToolTarget {
+  const float* targets;
+  const float* rotations;
+  int index;
+};
+
+__device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float* targets, const float* rotations,
+                              int index);
+__device__ void finish_external_dev(const bm_params& P, const bm_state& S);
+__device__ void accumulate_reaction_dev(const bm_state& S);
+
+__device__ __forceinline__ int bucket_of_point(f3 x, float cell, int mask);
+
+constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
+
+// rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
+// Per-block sums go to rigid_partial; k_rigid_finish reduces them.
+__global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
+                                                  int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
   const float dt = float(P.dt);
@@ -226,13 +253,22 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   f3 origin = ld3(S.tool_pos, 0), tvel = ld3(S.tool_vel, 0), tom = ld3(S.tool_omega, 0);
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
+  if ((mode & kPairPrep) != 0) {
+    // clear_pair_slots (contact.py:643-652): empty the slots of the table used two steps ago.
+    int count = S.pair_count[parity_new];
+    int64_t* keys = S.pair_keys + int64_t(parity_new) * P.pair_capacity;
+    const int32_t* used = S.pair_used + int64_t(parity_new) * P.pair_capacity;
+    for (int64_t a = p; a < count; a += int64_t(gridDim.x) * blockDim.x) keys[used[a]] = -1;
+  }
   if (live) {
     f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
     spd = len(vp);
-    // pair list staleness (check_pair_list)
-    if (S.built_x != nullptr) {
-      if (len(xp - ld3(S.built_x, p)) + dt * spd > 0.5f * float(P.pair_skin) || S.codes[p] != S.built_codes[p])
-        S.pair_dirty[0] = 1;
+    if ((mode & kPairPrep) != 0 && S.pair_dirty[0] != 0) {
+      // bin_particles for the rebuild of this step (cells of the list's base size)
+      float cell = 2.0f * float(P.radius) + float(P.pair_skin);
+      int b = bucket_of_point(xp, cell, int(P.list_buckets - 1));
+      S.particle_bucket[p] = b;
+      S.particle_rank[p] = atomicAdd(S.bucket_count + b, 1);
     }
     f3 total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
     // board: plane z = 0 inside its rectangle
@@ -307,14 +343,83 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
     }
     st3(S.impulses, p, total);
   }
-  // Wave-level reductions, one atomic per wave.
-  float m = wave_max(spd);
-  if (lane0()) atomic_max_nonneg(S.speed, m);
-  wave_add3(S.board, board_j);
-  wave_add3(S.reaction, react);
-  wave_add3(S.torque, torq);
-  float w = wave_sum(work);
-  if (lane0() && w != 0.0f) atomicAdd(S.work, w);
+  // Block sums: wave reduction, then the waves of the block in order.
+  __shared__ float wsum[kBlock / 64][kRigidSums];
+  float v[kRigidSums] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z,
+                         torq.x,    torq.y,    torq.z,    work,    spd,     0.0f};
+#pragma unroll
+  for (int k = 0; k < kRigidSums - 2; ++k) v[k] = wave_sum(v[k]);
+  v[10] = wave_max(v[10]);
+  int wave = threadIdx.x >> 6;
+  if (lane0()) {
+#pragma unroll
+    for (int k = 0; k < kRigidSums; ++k) wsum[wave][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < kRigidSums) {
+    int k = threadIdx.x;
+    float acc = wsum[0][k];
+    for (int w = 1; w < kBlock / 64; ++w) acc = (k == 10) ? fmaxf(acc, wsum[w][k]) : acc + wsum[w][k];
+    S.rigid_partial[int64_t(blockIdx.x) * kRigidSums + k] = acc;
+  }
+}
+
+// Reduction of the contact kernel's block sums in block order (deterministic), then the
+// single-thread work of the step: apply the sums, list reach, knife update.
+__global__ void __launch_bounds__(kBlock) k_rigid_finish(const bm_params P, bm_state S, int nblocks, int mode,
+                                                         ToolTarget tt, int parity_new) {
+  __shared__ float part[kBlock][kRigidSums];
+  float acc[kRigidSums];
+#pragma unroll
+  for (int k = 0; k < kRigidSums; ++k) acc[k] = 0.0f;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    const float* src = S.rigid_partial + int64_t(b) * kRigidSums;
+#pragma unroll
+    for (int k = 0; k < kRigidSums; ++k) acc[k] = (k == 10) ? fmaxf(acc[k], src[k]) : acc[k] + src[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kRigidSums; ++k) part[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+#pragma unroll
+      for (int k = 0; k < kRigidSums; ++k) {
+        float o = part[threadIdx.x + stride][k];
+        part[threadIdx.x][k] = (k == 10) ? fmaxf(part[threadIdx.x][k], o) : part[threadIdx.x][k] + o;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const float* t = part[0];
+  const float dt = float(P.dt);
+  if (mode & kOverwrite) {
+    for (int i = 0; i < 3; ++i) {
+      S.board[i] = t[i];
+      S.reaction[i] = t[3 + i];
+      S.torque[i] = t[6 + i];
+    }
+    S.speed[0] = t[10];
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      S.board[i] += t[i];
+      S.reaction[i] += t[3 + i];
+      S.torque[i] += t[6 + i];
+    }
+    S.speed[0] = fmaxf(S.speed[0], t[10]);
+  }
+  S.work[0] += t[9];
+  if (mode & kPairPrep) {
+    // set_pair_reach (contact.py:749-761) for a rebuild in this step; reset the new table count.
+    float r = float(P.radius), skin = float(P.pair_skin), sp = S.speed[0];
+    float value = 2.0f * r + skin;
+    if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
+    S.list_reach[0] = value;
+    S.pair_count[parity_new] = 0;
+  }
+  if (mode & kAccumulateInterval) accumulate_reaction_dev(S);
+  if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
+  if (mode & kFinishExternal) finish_external_dev(P, S);
 }
 
 __global__ void k_clear_step(bm_state S) {
@@ -363,6 +468,17 @@ __device__ __forceinline__ Cell cell_of(f3 x, float size) {
 __device__ __forceinline__ int bucket_of(Cell c, int mask) {
   return int((uint32_t(c.x) * 73856093u) ^ (uint32_t(c.y) * 19349663u) ^ (uint32_t(c.z) * 83492791u)) & mask;
 }
+__device__ __forceinline__ int bucket_of_point(f3 x, float cell, int mask) { return bucket_of(cell_of(x, cell), mask); }
+// The cell list bins particles in cells of the list's base reach 2 r + skin; a rebuild
+// whose reach is larger (fast particles) searches ceil(reach / cell) rings of cells.
+__device__ __forceinline__ float list_cell_size(const bm_params& P) { return 2.0f * float(P.radius) + float(P.pair_skin); }
+// At most 4 rings: only a diverged run (non-finite or absurd speeds) asks for more, and its
+// contacts are void anyway (the run fails its state checks).
+__device__ __forceinline__ int list_rings(const bm_params& P, float reach) {
+  float r = reach / list_cell_size(P);
+  if (!(r <= 4.0f)) r = 4.0f;
+  return max(1, int(ceilf(r)));
+}
 
 __global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
   int count = S.pair_count[parity_new];
@@ -390,8 +506,7 @@ __global__ void k_list_count(const bm_params P, bm_state S) {
   if (S.pair_dirty[0] == 0) return;
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  float reach = S.list_reach[0];
-  int b = bucket_of(cell_of(ld3(S.x, p), reach), int(P.list_buckets - 1));
+  int b = bucket_of_point(ld3(S.x, p), list_cell_size(P), int(P.list_buckets - 1));
   S.particle_bucket[p] = b;
   S.particle_rank[p] = atomicAdd(S.bucket_count + b, 1);
 }
@@ -475,6 +590,8 @@ __global__ void __launch_bounds__(1024) k_scan_final(const bm_params P, bm_state
 __global__ void k_list_scatter(const bm_params P, bm_state S) {
   if (S.pair_dirty[0] == 0) return;
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  // The scan has consumed the counts: leave them zero for the next rebuild's binning.
+  for (int64_t b = p; b <= P.list_buckets; b += int64_t(gridDim.x) * blockDim.x) S.bucket_count[b] = 0;
   if (p >= P.n) return;
   int dst = S.bucket_start[S.particle_bucket[p]] + S.particle_rank[p];
   f3 xp = ld3(S.x, p);
@@ -496,30 +613,34 @@ __global__ void k_list_build(const bm_params P, bm_state S) {
   S.built_codes[p] = cp;
   f3 rp = ld3(S.ref, p);
   float two_h = 2.0f * float(P.spacing);
-  Cell home = cell_of(xp, reach);
+  float cell = list_cell_size(P);
+  int rings = list_rings(P, reach);
+  Cell home = cell_of(xp, cell);
   int count = 0;
   const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
-  for (int i = 0; i < 27; ++i) {
-    Cell c = Cell{home.x + i % 3 - 1, home.y + (i / 3) % 3 - 1, home.z + i / 9 - 1};
-    int b = bucket_of(c, mask);
-    int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
-    for (int k = lo; k < hi; ++k) {
-      float4 e = sx[k];
-      int q = __float_as_int(e.w);
-      if (q <= p || count < 0) continue;
-      f3 xq = f3{e.x, e.y, e.z};
-      Cell cq = cell_of(xq, reach);
-      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
-      if (!(len(xq - xp) < reach)) continue;
-      if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
-      if (count == lanes) {
-        count = -1;
-      } else {
-        S.partners[p * lanes + count] = q;
-        count += 1;
+  for (int dx = -rings; dx <= rings; ++dx)
+    for (int dy = -rings; dy <= rings; ++dy)
+      for (int dz = -rings; dz <= rings; ++dz) {
+        Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+        int b = bucket_of(c, mask);
+        int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
+        for (int k = lo; k < hi; ++k) {
+          float4 e = sx[k];
+          int q = __float_as_int(e.w);
+          if (q <= p || count < 0) continue;
+          f3 xq = f3{e.x, e.y, e.z};
+          Cell cq = cell_of(xq, cell);
+          if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+          if (!(len(xq - xp) < reach)) continue;
+          if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
+          if (count == lanes) {
+            count = -1;
+          } else {
+            S.partners[p * lanes + count] = q;
+            count += 1;
+          }
+        }
       }
-    }
-  }
   S.partner_count[p] = count;
 }
 __global__ void k_list_done(bm_state S) { S.pair_dirty[0] = 0; }
@@ -574,6 +695,8 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 // pairs_listed (contact.py:825-925)
 __global__ void k_pairs_listed(const bm_params P, bm_state S, int parity_old) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  // The list is current from here on (list_done, solver.py:786).
+  if (p == 0) S.pair_dirty[0] = 0;
   if (p >= P.n) return;
   int count = S.partner_count[p];
   if (count == 0) return;
@@ -588,22 +711,37 @@ __global__ void k_pairs_listed(const bm_params P, bm_state S, int parity_old) {
   }
   // Overflowing particle: search the buckets of this build at the build positions.
   float reach = S.list_reach[0];
+  float cell = list_cell_size(P);
+  int rings = list_rings(P, reach);
   int mask = int(P.list_buckets - 1);
   const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
-  Cell home = cell_of(ld3(S.built_x, p), reach);
-  for (int i = 0; i < 27; ++i) {
-    Cell c = Cell{home.x + i % 3 - 1, home.y + (i / 3) % 3 - 1, home.z + i / 9 - 1};
-    int b = bucket_of(c, mask);
-    int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
-    for (int k = lo; k < hi; ++k) {
-      float4 e = sx[k];
-      int q = __float_as_int(e.w);
-      if (q <= p) continue;
-      Cell cq = cell_of(f3{e.x, e.y, e.z}, reach);
-      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
-      pair_at(P, S, T, p, q);
-    }
-  }
+  Cell home = cell_of(ld3(S.built_x, p), cell);
+  for (int dx = -rings; dx <= rings; ++dx)
+    for (int dy = -rings; dy <= rings; ++dy)
+      for (int dz = -rings; dz <= rings; ++dz) {
+        Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+        int b = bucket_of(c, mask);
+        int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
+        for (int k = lo; k < hi; ++k) {
+          float4 e = sx[k];
+          int q = __float_as_int(e.w);
+          if (q <= p) continue;
+          Cell cq = cell_of(f3{e.x, e.y, e.z}, cell);
+          if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+          pair_at(P, S, T, p, q);
+        }
+      }
+}
+
+// check_pair_list (contact.py:716-735) as its own pass for a standalone pair-contact call.
+__global__ void k_check_pair_list(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  float dt = float(P.dt);
+  f3 xp = ld3(S.x, p);
+  if (len(xp - ld3(S.built_x, p)) + dt * len(ld3(S.v, p)) > 0.5f * float(P.pair_skin) ||
+      S.codes[p] != S.built_codes[p])
+    S.pair_dirty[0] = 1;
 }
 
 // ------------------------------------------------------------------------ topology
@@ -702,67 +840,217 @@ __device__ __forceinline__ int field_of(uint2 idx, int a) {
 }
 
 // ------------------------------------------------------------------------ material
-// Zero the used field slots of every node (clear_compact_grid, material.py:656-669).
-__global__ void k_grid_clear(const bm_params P, bm_state S) {
-  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= P.node_count) return;
-  int nf = S.node_nf[node];
-  int64_t N = P.node_count;
-  for (int f = 0; f < nf; ++f) {
-    int64_t s = int64_t(f) * N + node;
-    S.grid_mass[s] = 0.0f;
-    st3(S.momentum, s, f3{0.f, 0.f, 0.f});
-    st3(S.grid_impulse, s, f3{0.f, 0.f, 0.f});
-    st3(S.remapped, s, f3{0.f, 0.f, 0.f});
-  }
+// Particles are binned into bricks of brick[0] x brick[1] x brick[2] cells (perm lists the
+// particles of each brick).  One workgroup per brick accumulates its particles' stencil
+// contributions in LDS over the brick's node window (brick + 1 nodes per axis) and writes
+// the window to a per-brick partial buffer with plain stores.  The grid kernels gather, for
+// every node, the windows of the (at most 8) bricks that contain it in a fixed order: P2G and
+// the MUSL re-scatter need no global atomics and their sums are deterministic.
+struct Window {
+  int x0, y0, z0;
+  int wy, wz, W;
+};
+__device__ __forceinline__ Window window_of(const bm_params& P, int b) {
+  int nby = int(P.bricks[1]), nbz = int(P.bricks[2]);
+  int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nby * nbz);
+  Window w;
+  w.x0 = bx * int(P.brick[0]);
+  w.y0 = by * int(P.brick[1]);
+  w.z0 = bz * int(P.brick[2]);
+  w.wy = int(P.brick[1]) + 1;
+  w.wz = int(P.brick[2]) + 1;
+  w.W = (int(P.brick[0]) + 1) * w.wy * w.wz;
+  return w;
 }
-
-// p2g (material.py:127-152): one thread per particle scatters its 8 corners.
-__global__ void __launch_bounds__(kBlock) k_p2g(const bm_params P, bm_state S) {
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  float mass = float(P.mass);
-  float sv = float(P.dt) * float(P.volume);
-  f3 vp = ld3(S.v, p), imp = ld3(S.impulses, p);
-  m3 Pk = ld9(S.stress, p);
-  Stencil st;
-  stencil_full(ld3(S.local, p), 1.0f / float(P.spacing), st);
-  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-  int32_t base = S.base_node[p];
+__device__ __forceinline__ int window_size(const bm_params& P) {
+  return int((P.brick[0] + 1) * (P.brick[1] + 1) * (P.brick[2] + 1));
+}
+// Window index of corner 0 of the particle's base cell.
+__device__ __forceinline__ int window_base(const bm_params& P, const Window& w, int32_t base) {
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
-  int64_t N = P.node_count;
-  m3 sP;
+  int cx = base / dyz, rem = base - cx * dyz;
+  int cy = rem / dz, cz = rem - cy * dz;
+  return ((cx - w.x0) * w.wy + (cy - w.y0)) * w.wz + (cz - w.z0);
+}
+__device__ __forceinline__ int window_corner(const Window& w, int a) {
+  return ((a >> 2) & 1) * w.wy * w.wz + ((a >> 1) & 1) * w.wz + (a & 1);
+}
+// Global node of window index i, or -1 outside the grid.
+__device__ __forceinline__ int64_t window_node(const bm_params& P, const Window& w, int i) {
+  int iz = i % w.wz, iy = (i / w.wz) % w.wy, ix = i / (w.wz * w.wy);
+  int nx = w.x0 + ix, ny = w.y0 + iy, nz = w.z0 + iz;
+  if (nx >= P.dims[0] || ny >= P.dims[1] || nz >= P.dims[2]) return -1;
+  return (int64_t(nx) * P.dims[1] + ny) * P.dims[2] + nz;
+}
+// Largest used field count over the window's nodes (block-wide).
+__device__ int window_fields(const bm_params& P, const bm_state& S, const Window& w, int* shared_max) {
+  if (threadIdx.x == 0) *shared_max = 0;
+  __syncthreads();
+  int m = 0;
+  for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
+    int64_t node = window_node(P, w, i);
+    if (node >= 0) m = max(m, S.node_nf[node]);
+  }
+  atomicMax(shared_max, m);
+  __syncthreads();
+  return *shared_max;
+}
+
+// Window index of corner 0 of the c-th cell of a brick (cells z-fastest).
+__device__ __forceinline__ int cell_window_base(const bm_params& P, const Window& w, int c) {
+  int bz = int(P.brick[2]), by = int(P.brick[1]);
+  int lz = c % bz, ly = (c / bz) % by, lx = c / (bz * by);
+  return (lx * w.wy + ly) * w.wz + lz;
+}
+__device__ __forceinline__ int cells_per_brick(const bm_params& P) { return int(P.brick[0] * P.brick[1] * P.brick[2]); }
+
+// p2g (material.py:127-152) for one brick: mass, momentum and stress + contact impulse.
+// One thread per cell sums its particles' contributions per stencil corner in registers and
+// adds each corner once to the LDS window (a particle whose field differs from the cell's
+// first particle at that corner adds its own).  Lanes hold different cells, so the LDS adds
+// of one instruction never share an address.
+__global__ void __launch_bounds__(256) k_p2g(const bm_params P, bm_state S, int clear_topology_flag) {
+  extern __shared__ float lds[];
+  __shared__ int fmax_shared;
+  int b = blockIdx.x;
+  if (clear_topology_flag && b == 0 && threadIdx.x == 0) S.topology_dirty[0] = 0;
+  Window w = window_of(P, b);
+  int F = int(P.fields);
+  int Fb = window_fields(P, S, w, &fmax_shared);
+  int FW = F * w.W;
+  for (int k = 0; k < 7; ++k)
+    for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) lds[k * FW + i] = 0.0f;
+  __syncthreads();
+  float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
+  int cpb = cells_per_brick(P);
+  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
+    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
+    if (lo == hi) continue;
+    int wb = cell_window_base(P, w, c);
+    float acc[8][7];
+    int f0[8];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) sP.a[k] = sv * Pk.a[k];
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    int64_t s = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
-    float w = st.w[a];
-    float mw = mass * w;
-    atomicAdd(S.grid_mass + s, mw);
-    atomic_add3(S.momentum, s, mw * vp);
-    atomic_add3(S.grid_impulse, s, w * imp - mvec(sP, st.g[a]));
+      for (int k = 0; k < 7; ++k) acc[a][k] = 0.0f;
+    for (int i = lo; i < hi; ++i) {
+      int64_t p = S.perm[i];
+      f3 vp = ld3(S.v, p), imp = ld3(S.impulses, p);
+      m3 Pk = ld9(S.stress, p);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
+      Stencil st;
+      stencil_full(ld3(S.local, p), inv_h, st);
+      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        int f = field_of(fidx, a);
+        if (i == lo) f0[a] = f;
+        float wt = st.w[a];
+        float mw = mass * wt;
+        f3 jm = wt * imp - mvec(Pk, st.g[a]);
+        float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
+        if (f == f0[a]) {
+#pragma unroll
+          for (int k = 0; k < 7; ++k) acc[a][k] += cv[k];
+        } else {
+          int idx = f * w.W + wb + window_corner(w, a);
+#pragma unroll
+          for (int k = 0; k < 7; ++k) atomicAdd(lds + k * FW + idx, cv[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int idx = f0[a] * w.W + wb + window_corner(w, a);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) atomicAdd(lds + k * FW + idx, acc[a][k]);
+    }
+  }
+  __syncthreads();
+  float* out = S.p2g_partial + int64_t(b) * FW * 7;
+  for (int g = threadIdx.x; g < Fb * w.W * 7; g += blockDim.x) {
+    int k = g % 7, fw = g / 7;
+    out[g] = lds[k * FW + fw];
   }
 }
 
-// Compatible-field exchange of one node (Eq. 8, material.py:252-285): for field a,
-// x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b.
+// Bricks along one axis whose window holds node coordinate c: c / B, and c / B - 1 when c
+// is on a brick boundary.
+__device__ __forceinline__ int bricks_at(int c, int B, int nb, int* out) {
+  int k = 0, b0 = c / B;
+  if (b0 < nb) out[k++] = b0;
+  if (c % B == 0 && b0 > 0) out[k++] = b0 - 1;
+  return k;
+}
+
+// Sum of the window entries of node (nx, ny, nz), field f, components [0, C) over its bricks.
+template <int C>
+__device__ __forceinline__ void gather_node(const bm_params& P, const float* partial, int nx, int ny, int nz, int f,
+                                            float* out) {
+  int bx[2], by[2], bz[2];
+  int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
+  int kx = bricks_at(nx, BX, int(P.bricks[0]), bx);
+  int ky = bricks_at(ny, BY, int(P.bricks[1]), by);
+  int kz = bricks_at(nz, BZ, int(P.bricks[2]), bz);
+  int wy = BY + 1, wz = BZ + 1, W = (BX + 1) * wy * wz;
+  int F = int(P.fields);
+#pragma unroll
+  for (int c = 0; c < C; ++c) out[c] = 0.0f;
+  for (int i = 0; i < kx; ++i)
+    for (int j = 0; j < ky; ++j)
+      for (int l = 0; l < kz; ++l) {
+        int64_t b = (int64_t(bx[i]) * P.bricks[1] + by[j]) * P.bricks[2] + bz[l];
+        int widx = ((nx - bx[i] * BX) * wy + (ny - by[j] * BY)) * wz + (nz - bz[l] * BZ);
+        const float* src = partial + ((b * F + f) * W + widx) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) out[c] += src[c];
+      }
+}
+
+__device__ __forceinline__ void node_coords(const bm_params& P, int64_t node, int& nx, int& ny, int& nz) {
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  nx = int(node / dyz);
+  int rem = int(node - int64_t(nx) * dyz);
+  ny = rem / dz;
+  nz = rem - ny * dz;
+}
+
+// Compatible-field exchange of one node (Eq. 8, material.py:252-285) on the gathered P2G
+// sums: x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b; then
+// old_v = p / m, new_v = (p + j) / m.
 __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S) {
   int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= P.node_count) return;
   int nf = S.node_nf[node];
+  if (nf == 0) return;
   int64_t N = P.node_count;
+  int nx, ny, nz;
+  node_coords(P, node, nx, ny, nz);
+  constexpr int kMaxLocal = 8;
   if (nf == 1) {
-    float m = S.grid_mass[node];
-    if (m > 0.0f) {
-      f3 pm = ld3(S.momentum, node), j = ld3(S.grid_impulse, node);
-      st3(S.old_v, node, pm / m);
-      st3(S.new_v, node, (pm + j) / m);
+    float g[7];
+    gather_node<7>(P, S.p2g_partial, nx, ny, nz, 0, g);
+    S.grid_mass[node] = g[0];
+    if (g[0] > 0.0f) {
+      f3 pm = f3{g[1], g[2], g[3]}, j = f3{g[4], g[5], g[6]};
+      st3(S.old_v, node, pm / g[0]);
+      st3(S.new_v, node, (pm + j) / g[0]);
     }
     return;
   }
+  // Several fields: gather all, then exchange (fields beyond kMaxLocal re-read global sums).
   float total = 0.0f;
-  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  for (int f = 0; f < nf; ++f) {
+    float g[7];
+    gather_node<7>(P, S.p2g_partial, nx, ny, nz, f, g);
+    int64_t s = int64_t(f) * N + node;
+    S.grid_mass[s] = g[0];
+    st3(S.momentum, s, f3{g[1], g[2], g[3]});
+    st3(S.grid_impulse, s, f3{g[4], g[5], g[6]});
+    total += g[0];
+  }
+  (void)kMaxLocal;
   for (int a = 0; a < nf; ++a) {
     int64_t sa = int64_t(a) * N + node;
     float ma = S.grid_mass[sa];
@@ -784,48 +1072,128 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   }
 }
 
-// g2p_remap (material.py:384-401)
-__global__ void __launch_bounds__(kBlock) k_g2p(const bm_params P, bm_state S) {
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  float w[8];
-  stencil_weights(ld3(S.local, p), w);
-  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-  int32_t base = S.base_node[p];
-  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+// g2p_remap (material.py:384-401) for one brick: grid velocities of the window are staged
+// in LDS; one thread per cell gathers each particle's 8 corners from there, updates it, and
+// re-scatters m w v per corner through registers (as k_p2g) into an LDS window that is
+// written to remap_partial.
+__global__ void __launch_bounds__(256) k_g2p(const bm_params P, bm_state S) {
+  extern __shared__ float lds[];
+  __shared__ int fmax_shared;
+  int b = blockIdx.x;
+  Window w = window_of(P, b);
+  int F = int(P.fields);
+  int Fb = window_fields(P, S, w, &fmax_shared);
+  int FW = F * w.W;
+  float* vel = lds;            // [6][F][W]: old_v xyz, new_v xyz
+  float* acc_l = lds + 6 * FW; // [3][F][W]
   int64_t N = P.node_count;
-  int64_t slot[8];
-  f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    slot[a] = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
-    f3 nv = ld3(S.new_v, slot[a]);
-    pic = pic + w[a] * nv;
-    dv = dv + w[a] * (nv - ld3(S.old_v, slot[a]));
+  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
+    int f = i / w.W, wi = i - f * w.W;
+    int64_t node = window_node(P, w, wi);
+    f3 o = f3{0.f, 0.f, 0.f}, nv = o;
+    if (node >= 0 && f < S.node_nf[node]) {
+      o = ld3(S.old_v, int64_t(f) * N + node);
+      nv = ld3(S.new_v, int64_t(f) * N + node);
+    }
+    vel[i] = o.x;
+    vel[FW + i] = o.y;
+    vel[2 * FW + i] = o.z;
+    vel[3 * FW + i] = nv.x;
+    vel[4 * FW + i] = nv.y;
+    vel[5 * FW + i] = nv.z;
+    acc_l[i] = 0.0f;
+    acc_l[FW + i] = 0.0f;
+    acc_l[2 * FW + i] = 0.0f;
   }
+  __syncthreads();
   float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
-  f3 vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
-  st3(S.v, p, vp);
-  st3(S.x, p, ld3(S.x, p) + dt * pic);
+  int cpb = cells_per_brick(P);
+  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
+    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
+    if (lo == hi) continue;
+    int wb = cell_window_base(P, w, c);
+    float acc[8][3];
+    int f0[8];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) atomic_add3(S.remapped, slot[a], (mass * w[a]) * vp);
+    for (int a = 0; a < 8; ++a) acc[a][0] = acc[a][1] = acc[a][2] = 0.0f;
+    for (int i = lo; i < hi; ++i) {
+      int64_t p = S.perm[i];
+      float wt[8];
+      stencil_weights(ld3(S.local, p), wt);
+      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+      int fa[8];
+      f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        fa[a] = field_of(fidx, a);
+        if (i == lo) f0[a] = fa[a];
+        int idx = fa[a] * w.W + wb + window_corner(w, a);
+        f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
+        f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
+        pic = pic + wt[a] * nv;
+        dv = dv + wt[a] * (nv - o);
+      }
+      f3 vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
+      st3(S.v, p, vp);
+      st3(S.x, p, ld3(S.x, p) + dt * pic);
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        f3 r = (mass * wt[a]) * vp;
+        if (fa[a] == f0[a]) {
+          acc[a][0] += r.x;
+          acc[a][1] += r.y;
+          acc[a][2] += r.z;
+        } else {
+          int idx = fa[a] * w.W + wb + window_corner(w, a);
+          atomicAdd(acc_l + idx, r.x);
+          atomicAdd(acc_l + FW + idx, r.y);
+          atomicAdd(acc_l + 2 * FW + idx, r.z);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int idx = f0[a] * w.W + wb + window_corner(w, a);
+      atomicAdd(acc_l + idx, acc[a][0]);
+      atomicAdd(acc_l + FW + idx, acc[a][1]);
+      atomicAdd(acc_l + 2 * FW + idx, acc[a][2]);
+    }
+  }
+  __syncthreads();
+  float* out = S.remap_partial + int64_t(b) * FW * 3;
+  for (int g = threadIdx.x; g < Fb * w.W * 3; g += blockDim.x) {
+    int k = g % 3, fw = g / 3;
+    out[g] = acc_l[k * FW + fw];
+  }
 }
 
-// couple (material.py:155-179) of the re-scattered momentum, then field velocity.
+// couple (material.py:155-179) of the gathered re-scattered momentum, then the field
+// velocity p / m read by the F update (0 on massless fields, which the update skips).
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S) {
   int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= P.node_count) return;
   int nf = S.node_nf[node];
+  if (nf == 0) return;
   int64_t N = P.node_count;
+  int nx, ny, nz;
+  node_coords(P, node, nx, ny, nz);
   if (nf == 1) {
+    float g[3];
+    gather_node<3>(P, S.remap_partial, nx, ny, nz, 0, g);
     float m = S.grid_mass[node];
     f3 vel = f3{0.f, 0.f, 0.f};
-    if (m > 0.0f) vel = ld3(S.remapped, node) / m;
+    if (m > 0.0f) vel = f3{g[0], g[1], g[2]} / m;
     st3(S.grid_vel, node, vel);
     return;
   }
   float total = 0.0f;
-  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  for (int f = 0; f < nf; ++f) {
+    float g[3];
+    gather_node<3>(P, S.remap_partial, nx, ny, nz, f, g);
+    int64_t s = int64_t(f) * N + node;
+    st3(S.remapped, s, f3{g[0], g[1], g[2]});
+    total += S.grid_mass[s];
+  }
   for (int a = 0; a < nf; ++a) {
     int64_t sa = int64_t(a) * N + node;
     float ma = S.grid_mass[sa];
@@ -902,32 +1270,67 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp);
 }
 
-// update_F_plastic (material.py:445-475)
-__global__ void __launch_bounds__(kBlock) k_update_F(const bm_params P, bm_state S) {
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  Stencil st;
-  stencil_full(ld3(S.local, p), 1.0f / float(P.spacing), st);
-  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-  int32_t base = S.base_node[p];
-  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit);
+
+// update_F_plastic (material.py:445-475) for one brick, with the field velocities of the
+// window staged in LDS.  In a substep it also labels sides (assign, contact.py:996-1039) and
+// flags the pair list of the next step as stale (check_pair_list, contact.py:716-735) from
+// the same particle pass.
+__global__ void __launch_bounds__(256) k_update_F(const bm_params P, bm_state S, int64_t digit, int flags) {
+  extern __shared__ float lds[];
+  __shared__ int fmax_shared;
+  int b = blockIdx.x;
+  Window w = window_of(P, b);
+  int F = int(P.fields);
+  int Fb = window_fields(P, S, w, &fmax_shared);
+  int FW = F * w.W;
   int64_t N = P.node_count;
-  m3 rate = mzero();
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    int64_t s = int64_t(field_of(fidx, a)) * N + corner_node(base, a, dyz, dz);
-    f3 vel = ld3(S.grid_vel, s);
-    f3 g = st.g[a];
-    rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
-    rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
-    rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
+  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
+    int f = i / w.W, wi = i - f * w.W;
+    int64_t node = window_node(P, w, wi);
+    f3 v = f3{0.f, 0.f, 0.f};
+    if (node >= 0 && f < S.node_nf[node]) v = ld3(S.grid_vel, int64_t(f) * N + node);
+    lds[i] = v.x;
+    lds[FW + i] = v.y;
+    lds[2 * FW + i] = v.z;
   }
-  m3 F = ld9(S.F, p);
-  float dt = float(P.dt);
+  __syncthreads();
+  float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
+  float half_skin = 0.5f * float(P.pair_skin);
+  bool stale = false;
+  int cpb = cells_per_brick(P);
+  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
+    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
+    int wb = cell_window_base(P, w, c);
+    for (int i = lo; i < hi; ++i) {
+      int64_t p = S.perm[i];
+      Stencil st;
+      stencil_full(ld3(S.local, p), inv_h, st);
+      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+      m3 rate = mzero();
 #pragma unroll
-  for (int k = 0; k < 9; ++k) F.a[k] += dt * rate.a[k];
-  st9(S.F, p, F);
-  st9(S.stress, p, stress_of(P, S, p, F));
+      for (int a = 0; a < 8; ++a) {
+        int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+        f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
+        f3 g = st.g[a];
+        rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
+        rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
+        rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
+      }
+      m3 Fm = ld9(S.F, p);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
+      st9(S.F, p, Fm);
+      st9(S.stress, p, stress_of(P, S, p, Fm));
+      if (flags & 1) assign_dev(P, S, p, digit);
+      if (flags & 2) {
+        f3 xp = ld3(S.x, p);
+        if (len(xp - ld3(S.built_x, p)) + dt * len(ld3(S.v, p)) > half_skin || S.codes[p] != S.built_codes[p])
+          stale = true;
+      }
+    }
+  }
+  if (stale) S.pair_dirty[0] = 1;
 }
 
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
@@ -970,7 +1373,8 @@ __device__ __forceinline__ f3 blade_vertex(int i, f3 hf, float bevel) {
 
 // tool.step (tool.py:146-259): drive, gyroscopic term, 6 sweeps x 10 vertices of the
 // knife/board velocity constraint with Coulomb friction, integration.
-__global__ void k_tool_step(const bm_params P, bm_state S, const float* targets, const float* rotations, int index) {
+__device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float* targets, const float* rotations,
+                              int index) {
   float dt = float(P.dt), mass = float(P.blade_mass), bevel = float(P.bevel);
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
   f3 x = ld3(S.tool_pos, 0);
@@ -1051,9 +1455,12 @@ __global__ void k_tool_step(const bm_params P, bm_state S, const float* targets,
   st3(S.tool_omega, 0, w);
   st3(S.table, 0, total);
 }
+__global__ void k_tool_step(const bm_params P, bm_state S, const float* targets, const float* rotations, int index) {
+  tool_step_dev(P, S, targets, rotations, index);
+}
 
 // finish_external_step (tool.py:70-115)
-__global__ void k_finish_external(const bm_params P, bm_state S) {
+__device__ void finish_external_dev(const bm_params& P, const bm_state& S) {
   f3 j = ld3(S.reaction, 0), tau = ld3(S.torque, 0);
   f3 pos = ld3(S.tool_pos, 0);
   st3(S.interval_impulse, 0, ld3(S.interval_impulse, 0) + j);
@@ -1091,17 +1498,18 @@ __global__ void k_finish_external(const bm_params P, bm_state S) {
   }
   S.speed[0] = 0.0f;
 }
+__global__ void k_finish_external(const bm_params P, bm_state S) { finish_external_dev(P, S); }
 
-__global__ void k_accumulate_reaction(bm_state S) {
+__device__ void accumulate_reaction_dev(const bm_state& S) {
   f3 j = ld3(S.reaction, 0);
   st3(S.interval_impulse, 0, ld3(S.interval_impulse, 0) + j);
   st3(S.interval_angular, 0, ld3(S.interval_angular, 0) + (ld3(S.torque, 0) + cross(ld3(S.tool_pos, 0), j)));
 }
+__global__ void k_accumulate_reaction(bm_state S) { accumulate_reaction_dev(S); }
 
-// assign (contact.py:996-1039)
-__global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting) {
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n || cutting == 0) return;
+// assign (contact.py:996-1039): side digit for a particle behind the edge band or pinched
+// on the board under the edge.
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit) {
   int64_t code = S.codes[p];
   if ((code / digit) % 3 != 0) return;
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
@@ -1126,6 +1534,12 @@ __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S
     S.codes[p] = code + digit * side;
     S.topology_dirty[0] = 1;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n || cutting == 0) return;
+  assign_dev(P, S, p, digit);
 }
 
 // ------------------------------------------------------------------------ port kernels
@@ -1278,33 +1692,86 @@ int bm_clear_step(const bm_params* p, const bm_state* s, void* stream) {
 }
 
 int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, void* stream) {
-  if (!state_ok(p, s) || !s->impulses || !s->histories || !s->damage) return -1;
+  if (!state_ok(p, s) || !s->impulses || !s->histories || !s->damage || !s->rigid_partial || !s->counters) return -1;
   if (p->n == 0) return 0;
-  hipLaunchKernelGGL(k_rigid, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
+  if (p->rigid_blocks < blocks_for(p->n)) return -2;
+  int nb = blocks_for(p->n);
+  hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
+                     int(kAccumulate), 0);
+  hipLaunchKernelGGL(k_rigid_finish, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
+                     ToolTarget{nullptr, nullptr, 0}, 0);
   return launch_status();
 }
 
-int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void* stream) {
-  if (!state_ok(p, s) || !s->pair_keys || !s->sorted_x || p->pair_capacity <= 0 ||
-      (p->pair_capacity & (p->pair_capacity - 1)) != 0 || p->list_buckets <= 0 ||
-      (p->list_buckets & (p->list_buckets - 1)) != 0 || p->list_buckets > (int64_t(4096) * 4096))
-    return -1;
-  if (p->n == 0) return 0;
-  hipStream_t st = (hipStream_t)stream;
-  int old_p = parity & 1, new_p = 1 - old_p;
-  hipLaunchKernelGGL(k_clear_pair_slots, dim3(256), dim3(kBlock), 0, st, *s, p->pair_capacity, new_p);
-  hipLaunchKernelGGL(k_zero_pair_count, dim3(1), dim3(1), 0, st, *s, new_p);
+namespace {
+bool pair_args_ok(const bm_params* p, const bm_state* s) {
+  return state_ok(p, s) && s->pair_keys && s->sorted_x && p->pair_capacity > 0 &&
+         (p->pair_capacity & (p->pair_capacity - 1)) == 0 && p->list_buckets > 0 &&
+         (p->list_buckets & (p->list_buckets - 1)) == 0 && p->list_buckets <= int64_t(4096) * 4096;
+}
+// Rebuild of the cell list and partner lists (predicated on pair_dirty inside), then the
+// listed contacts.
+void launch_pair_list(const bm_params* p, const bm_state* s, int old_p, hipStream_t st) {
   int64_t B = p->list_buckets;
   int nscan = int((B + 4095) / 4096);
-  hipLaunchKernelGGL(k_list_reset, dim3(blocks_for(B + 1, 1024)), dim3(1024), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_list_count, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_scan_partial, dim3(nscan), dim3(1024), 0, st, *p, *s);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, *p, *s, nscan);
   hipLaunchKernelGGL(k_scan_final, dim3(nscan), dim3(1024), 0, st, *p, *s);
   hipLaunchKernelGGL(k_list_scatter, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_list_build, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1), 0, st, *s);
   hipLaunchKernelGGL(k_pairs_listed, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
+}
+int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
+int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
+bool brick_args_ok(const bm_params* p, const bm_state* s, int lds_floats_per_fw) {
+  if (!state_ok(p, s) || !s->perm || !s->cell_start || !s->p2g_partial || !s->remap_partial) return false;
+  if (p->brick[0] < 1 || p->brick[1] < 1 || p->brick[2] < 1 || brick_count(p) < 1) return false;
+  if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
+      p->bricks[2] * p->brick[2] < p->dims[2] - 1)
+    return false;
+  return int64_t(lds_floats_per_fw) * p->fields * window_nodes(p) * 4 <= 65536;
+}
+// Threads of a brick workgroup: one per cell, rounded up to whole waves, at most 256.
+int brick_threads(const bm_params* p) {
+  int64_t cells = p->brick[0] * p->brick[1] * p->brick[2];
+  int64_t t = ((cells + 63) / 64) * 64;
+  return int(t < 64 ? 64 : (t > 256 ? 256 : t));
+}
+size_t lds_bytes(const bm_params* p, int floats_per_fw) {
+  return size_t(floats_per_fw) * size_t(p->fields) * size_t(window_nodes(p)) * sizeof(float);
+}
+void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
+  int64_t total = p->fields * p->node_count;
+  hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
+  if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_topo_sort, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  if (p->n > 0) hipLaunchKernelGGL(k_topo_index, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+}
+// P2G .. update_F with the brick kernels; update_flags as k_update_F.
+void launch_transfers(const bm_params* p, const bm_state* s, int clear_topology, int64_t digit, int update_flags,
+                      hipStream_t st) {
+  int nb = int(brick_count(p)), tb = brick_threads(p);
+  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7), st, *p, *s, clear_topology);
+  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  // Held blade: constrain new_v before it moves particles (solver.py:476-477).
+  if (p->blade_hold) bm_blade_constraint(p, s, st);
+  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9), st, *p, *s);
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3), st, *p, *s, digit, update_flags);
+}
+}  // namespace
+
+int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void* stream) {
+  if (!pair_args_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int old_p = parity & 1, new_p = 1 - old_p;
+  hipLaunchKernelGGL(k_check_pair_list, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_clear_pair_slots, dim3(256), dim3(kBlock), 0, st, *s, p->pair_capacity, new_p);
+  hipLaunchKernelGGL(k_zero_pair_count, dim3(1), dim3(1), 0, st, *s, new_p);
+  hipLaunchKernelGGL(k_list_reset, dim3(blocks_for(p->list_buckets + 1, 1024)), dim3(1024), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_count, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  launch_pair_list(p, s, old_p, st);
   return launch_status();
 }
 
@@ -1312,59 +1779,52 @@ int bm_register(const bm_params* p, const bm_state* s, void* stream) {
   if (!state_ok(p, s) || !s->field_code || !s->node_nf || !s->field_idx || !s->transfer_codes) return -1;
   if (p->fields < 1 || p->fields > 255) return -2;
   hipStream_t st = (hipStream_t)stream;
-  int64_t total = p->fields * p->node_count;
-  hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
-  if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_topo_sort, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
-  if (p->n > 0) hipLaunchKernelGGL(k_topo_index, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  launch_register(p, s, st);
   hipLaunchKernelGGL(k_topo_done, dim3(1), dim3(1), 0, st, *s);
   return launch_status();
 }
 
 int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
-  if (!state_ok(p, s) || !s->grid_mass) return -1;
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_grid_clear, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
-  if (p->n > 0) hipLaunchKernelGGL(k_p2g, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  if (!brick_args_ok(p, s, 9)) return -1;
+  hipLaunchKernelGGL(k_p2g, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 7), (hipStream_t)stream,
+                     *p, *s, 0);
   return launch_status();
 }
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
-  if (!state_ok(p, s)) return -1;
+  if (!brick_args_ok(p, s, 9)) return -1;
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
-  if (!state_ok(p, s)) return -1;
-  if (p->n == 0) return 0;
-  hipLaunchKernelGGL(k_g2p, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  if (!brick_args_ok(p, s, 9)) return -1;
+  hipLaunchKernelGGL(k_g2p, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 9), (hipStream_t)stream,
+                     *p, *s);
   return launch_status();
 }
 
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
-  if (!state_ok(p, s)) return -1;
+  if (!brick_args_ok(p, s, 9)) return -1;
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
-  if (!state_ok(p, s)) return -1;
-  if (p->n == 0) return 0;
-  hipLaunchKernelGGL(k_update_F, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  if (!brick_args_ok(p, s, 9)) return -1;
+  hipLaunchKernelGGL(k_update_F, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 3),
+                     (hipStream_t)stream, *p, *s, int64_t(1), 0);
   return launch_status();
 }
 
 int bm_material_step(const bm_params* p, const bm_state* s, void* stream) {
-  int r;
-  if ((r = bm_register(p, s, stream))) return r;
-  if ((r = bm_p2g(p, s, stream))) return r;
-  if ((r = bm_couple_and_velocity(p, s, stream))) return r;
-  if ((r = bm_g2p_remap(p, s, stream))) return r;
-  if ((r = bm_couple_remapped(p, s, stream))) return r;
-  return bm_update_F(p, s, stream);
+  if (!brick_args_ok(p, s, 9) || !s->field_code || !s->node_nf || !s->field_idx || !s->transfer_codes) return -1;
+  hipStream_t st = (hipStream_t)stream;
+  launch_register(p, s, st);
+  launch_transfers(p, s, 1, 1, 0, st);
+  return launch_status();
 }
 
 int bm_tool_step(const bm_params* p, const bm_state* s, const float* targets, const float* rotations, int32_t index,
@@ -1397,19 +1857,33 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
 int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
                int32_t parity, int32_t accumulate, const float* targets, const float* rotations, int32_t index,
                void* stream) {
-  int r;
-  bool external = p && p->tool_model < 0;
-  if (!external && (r = bm_clear_step(p, s, stream))) return r;
-  if ((r = bm_rigid_contacts(p, s, cutting ? cut_bits : 0, stream))) return r;
-  if ((r = bm_pair_contacts(p, s, parity, stream))) return r;
-  if ((r = bm_material_step(p, s, stream))) return r;
+  if (!brick_args_ok(p, s, 9) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
+  if (p->rigid_blocks < blocks_for(p->n)) return -2;
+  bool external = p->tool_model < 0;
+  if (!external && (!targets || !rotations || index < 0)) return -1;
+  if (p->n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int old_p = parity & 1, new_p = 1 - old_p;
+  // 1. contacts with the knife and the board; the last block applies the step sums and moves
+  //    the knife (its update reads only this step's rigid reaction, solver.py:824-864).
+  int mode = kOverwrite | kPairPrep;
   if (external) {
-    if ((r = bm_finish_external(p, s, stream))) return r;
+    // With a held blade the grid constraint adds to the reaction first (finish runs last).
+    if (!p->blade_hold) mode |= kFinishExternal;
   } else {
-    if (accumulate && (r = bm_accumulate_reaction(p, s, stream))) return r;
-    if ((r = bm_tool_step(p, s, targets, rotations, index, stream))) return r;
+    mode |= kToolStep | (accumulate ? kAccumulateInterval : 0);
   }
-  return bm_assign(p, s, digit, cutting, stream);
+  int nb = blocks_for(p->n);
+  hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
+  hipLaunchKernelGGL(k_rigid_finish, dim3(1), dim3(kBlock), 0, st, *p, *s, nb, mode,
+                     ToolTarget{targets, rotations, int(index)}, new_p);
+  // 2. particle pairs across cuts
+  launch_pair_list(p, s, old_p, st);
+  // 3. topology when labels changed, transfers, F update + side labels + list staleness
+  launch_register(p, s, st);
+  launch_transfers(p, s, 1, digit, (cutting ? 1 : 0) | 2, st);
+  if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
+  return launch_status();
 }
 
 int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream) {

@@ -22,6 +22,7 @@
  *   field_code int64[F][N] (-1 = empty slot), node_nf int32[N]
  *   grid scalars [F][N], grid vectors [F][N][3]
  *   histories [2][n][3]  (row 0 board, row 1 blade friction history)
+ *   W = (brick[0] + 1) (brick[1] + 1) (brick[2] + 1) window nodes per brick
  */
 #ifndef BM_HIP_H
 #define BM_HIP_H
@@ -50,6 +51,10 @@ typedef struct bm_params {
   int64_t lock_rotation;  /* tool.step: guided, translation only */
   int64_t predict;        /* finish_external_step: mobility predictor */
   int64_t plastic;        /* yield_stress > 0 */
+  int64_t brick[3];       /* cells per brick along x, y, z (particle bins of the P2G/G2P kernels) */
+  int64_t bricks[3];      /* bricks along x, y, z */
+  int64_t rigid_blocks;   /* blocks of the contact kernel (partial sums per block) */
+  int64_t blade_hold;     /* external tool held over the interval: grid constraint active */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
@@ -91,6 +96,26 @@ typedef struct bm_state {
   int32_t *particle_bucket;/* [n] */
   int32_t *particle_rank;  /* [n] */
   float *sorted_x;         /* [n][4]: position and particle index (bits) in bucket order */
+  /* brick bins: bricks x-major, cells of a brick z-fastest; the particles of the c-th cell
+   * in this order are perm[cell_start[c] .. cell_start[c+1]) */
+  int32_t *perm;           /* [n] */
+  int32_t *cell_start;     /* [bricks * cells per brick + 1] */
+  float *p2g_partial;      /* [bricks][F][W][7] window sums of mass, momentum, impulse */
+  float *remap_partial;    /* [bricks][F][W][3] window sums of the re-scattered momentum */
+  float *rigid_partial;    /* [rigid_blocks][12] per-block contact sums */
+  uint32_t *counters;      /* [4] last-block tickets (zero between launches) */
+  /* held-blade grid constraint (blade_constraint.py), used when blade_hold is set */
+  float *bc_side;          /* [n] face side of the previous step (0, +1, -1) */
+  float *bc_normal;        /* [n][3] */
+  float *bc_lever;         /* [n][3] */
+  float *bc_rhs, *bc_bound, *bc_impulse, *bc_extrapolated; /* [n] */
+  int32_t *bc_active;      /* [n] particles with a face contact this step */
+  int32_t *bc_nodes;       /* [N] nodes touched by them */
+  int32_t *bc_node_mark;   /* [N] 0 between steps */
+  float *bc_grid;          /* [F][N][3] 0 between steps */
+  float *bc_coupled;       /* [F][N][3] */
+  int32_t *bc_counts;      /* [4] active particles, active nodes, needs_solve, batches */
+  float *bc_sums;          /* [8] linear 3, angular 3, max residual of the run, last residual */
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */
@@ -158,6 +183,14 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
                int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
                int32_t index, void* stream);
 
+/* Held-blade grid constraint (blade_constraint.py:13-324): face contacts of the particles
+ * beside the blade's broad faces, projected-gradient solve of W lambda >= rhs, lambda >= 0 on
+ * the grid velocity new_v, reaction added to the tool.  Runs between couple_and_velocity and
+ * g2p_remap; a no-op without face contacts that move into the blade. */
+int bm_blade_constraint(const bm_params* p, const bm_state* s, void* stream);
+/* result[size - 1] = largest unresolved inward velocity so far (blade_constraint.py:208-210). */
+int bm_store_residual(const bm_params* p, const bm_state* s, float* result, int32_t size, void* stream);
+
 /* External-tool port kernels (tool.py:19-68, material.py:6-20). */
 int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream);
 int bm_record_pose(const bm_params* p, const bm_state* s, int32_t index, float* out, void* stream);
@@ -185,6 +218,37 @@ int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk,
                       int32_t count, void* stream);
 /* 3x3 SVD of A[9*count] -> U[9], s[3], V[9] per lane. */
 int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * MLS-MPM cutting baseline (blademaster_warp/mlsmpm.py:42-229, MlsSolver._substep
+ * mlsmpm.py:293-393).  Dense grid of dims nodes with planes [3][nodes] for momentum and
+ * velocity; grid_m and grid_p must be zero before the first P2G (the grid update leaves
+ * them zero for the next one). */
+typedef struct bm_mls_params {
+  int64_t n;
+  int64_t dims[3];
+  int64_t plastic;         /* yield_stress > 0 */
+  double origin[3];
+  double dx, dt, mass, volume, mu, lam, damaged_shear, softening, compressed, stretched;
+} bm_mls_params;
+
+typedef struct bm_mls_state {
+  float *x, *v, *C, *F, *damage, *damage_reference, *yield_stress, *impulse;
+  float *grid_m, *grid_p, *grid_v;
+  float *interval_impulse, *interval_angular;
+  int32_t *inversions;
+} bm_mls_state;
+
+/* Quadratic B-spline APIC P2G with the corotated stress (mlsmpm.py:75-119). */
+int bm_mls_p2g(const bm_mls_params* m, const bm_mls_state* s, void* stream);
+/* Nodal velocity with sticky walls (mlsmpm.py:122-137); zeroes grid_m and grid_p. */
+int bm_mls_grid_update(const bm_mls_params* m, const bm_mls_state* s, void* stream);
+/* APIC G2P, F update with singular-value clamp, return map, damage (mlsmpm.py:157-229). */
+int bm_mls_g2p(const bm_mls_params* m, const bm_mls_state* s, void* stream);
+/* One MLS substep: knife/board contact on the MLS particles (p, t as for bm_rigid_contacts
+ * with tool_model -1), interval sums, P2G, grid update, G2P (mlsmpm.py:293-393). */
+int bm_mls_substep(const bm_params* p, const bm_state* t, const bm_mls_params* m, const bm_mls_state* s,
+                   void* stream);
 
 #ifdef __cplusplus
 }

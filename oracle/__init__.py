@@ -21,11 +21,12 @@ REAL_ARRAYS = {
     "impulses": 3, "histories": None, "tool_pos": None, "tool_vel": None, "tool_rot": None, "tool_omega": None,
     "tool_mobility": None, "reaction": None, "torque": None, "board": None, "work": None, "speed": None,
     "tool_table_impulse": None, "interval_impulse": None, "interval_angular_impulse": None,
+    "bc_side": 1, "bc_impulse": 1, "bc_normal": 3,
 }
 FIXED = {
     "histories": lambda n: 6 * n, "tool_pos": 3, "tool_vel": 3, "tool_rot": 4, "tool_omega": 3, "tool_mobility": 36,
     "reaction": 3, "torque": 3, "board": 3, "work": 1, "speed": 1, "tool_table_impulse": 3, "interval_impulse": 3,
-    "interval_angular_impulse": 3,
+    "interval_angular_impulse": 3, "bc_max_violation": 1,
 }
 
 
@@ -283,3 +284,78 @@ def g2p(x, v, slots, weights, old_v, new_v, dt, flip):
     lib.oracle_g2p(x.ctypes.data, v.ctypes.data, slots.ctypes.data, weights.ctypes.data, old_v.ctypes.data,
                    new_v.ctypes.data, float(dt), float(flip), len(x))
     return x, v
+
+
+class MlsConfig(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int64 * 3), ("plastic", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("origin", ctypes.c_double * 3)] + [
+        (k, ctypes.c_double) for k in "dx dt mass volume mu lam damaged_shear softening compressed stretched".split()]
+
+
+class OracleMls:
+    """MLS-MPM baseline oracle (mlsmpm.py:293-393) for config `cfg`, TopoCut parameters
+    `params`, on the grid (origin, dims) of the solver under test."""
+
+    def __init__(self, cfg, params, reference, origin, dims, precision=32, damage_from_start=True):
+        lib = load()
+        lib.oracle_mls_create.restype = ctypes.c_void_p
+        lib.oracle_mls_create.argtypes = [ctypes.c_int, ctypes.POINTER(OracleConfig), ctypes.POINTER(MlsConfig)]
+        lib.oracle_mls_destroy.argtypes = [ctypes.c_void_p]
+        lib.oracle_mls_substep.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.oracle_mls_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int]
+        self.lib = lib
+        self.n = n = len(reference)
+        k = constants(cfg)
+        c = OracleConfig()
+        c.n, c.node_count, c.key_capacity, c.pair_capacity = n, 1, 16, 16
+        c.dt, c.spacing, c.mass, c.volume = cfg.dt, cfg.grid_spacing, k["mass"], k["volume"]
+        c.radius, c.blade_offset, c.mu, c.lam = k["radius"], k["blade_offset"], k["mu"], k["lam"]
+        c.yield_stress = cfg.yield_stress
+        c.sn, c.gk, c.mus, c.mud, c.gc = k["sn"], k["gk"], cfg.friction_static, cfg.friction_dynamic, cfg.tangent_damping
+        c.board_sn, c.threshold = k["board_sn"], 1.0
+        for i in range(3):
+            c.half[i] = cfg.blade_half_size[i]
+        c.bevel, c.blade_mass = cfg.blade_bevel_height, cfg.blade_mass
+        c.board_half[0], c.board_half[1] = cfg.board_half_extent
+        c.external_tool = 1
+        m = MlsConfig()
+        for i in range(3):
+            m.dims[i] = int(dims[i])
+            m.origin[i] = float(origin[i])
+        m.plastic = int(cfg.yield_stress > 0)
+        m.dx, m.dt, m.mass, m.volume = cfg.grid_spacing, cfg.dt, k["mass"], k["volume"]
+        m.mu, m.lam, m.damaged_shear, m.softening = k["mu"], k["lam"], params.damaged_shear, params.softening
+        m.compressed = (1 - params.compression) ** params.sensitivity
+        m.stretched = (1 + params.stretch) ** params.sensitivity
+        self.c, self.m = c, m
+        self.handle = lib.oracle_mls_create(precision, ctypes.byref(c), ctypes.byref(m))
+        self.set("x", reference)
+        self.set("damage_reference", np.full(n, 1.0 if damage_from_start else 0.0))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.oracle_mls_destroy(self.handle)
+            self.handle = None
+
+    def _count(self, name):
+        n = self.n
+        return {"x": 3 * n, "v": 3 * n, "C": 9 * n, "F": 9 * n, "damage": n, "damage_reference": n,
+                "yield_stress": n, "impulse": 3, "angular_impulse": 3, "tool_pos": 3, "tool_vel": 3, "tool_rot": 4,
+                "tool_omega": 3, "inversions": 1, "histories": 6 * n}[name]
+
+    def set(self, name, value):
+        buf = np.ascontiguousarray(np.asarray(value, dtype=np.float64).ravel())
+        code = self.lib.oracle_mls_array(self.handle, name.encode(), buf.ctypes.data, self._count(name), 1)
+        if code:
+            raise RuntimeError(f"oracle_mls_array({name}) failed: {code}")
+
+    def get(self, name):
+        buf = np.zeros(self._count(name))
+        code = self.lib.oracle_mls_array(self.handle, name.encode(), buf.ctypes.data, self._count(name), 0)
+        if code:
+            raise RuntimeError(f"oracle_mls_array({name}) failed: {code}")
+        return buf
+
+    def substep(self, steps=1):
+        self.lib.oracle_mls_substep(self.handle, int(steps))

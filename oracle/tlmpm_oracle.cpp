@@ -353,6 +353,10 @@ template <class R> struct Sim {
   int64_t pair_count = 0;
   int64_t step_count = 0;
   int64_t cut_index = 0;
+  // Held-blade grid constraint state (blade_constraint.py:221-240).
+  std::vector<R> bc_side, bc_normal, bc_lever, bc_rhs, bc_bound, bc_impulse, bc_extrap, bc_grid, bc_coupled;
+  R bc_max_violation = R(0);
+  int64_t bc_batches = 0;
 
   explicit Sim(const OracleConfig& cfg) : c(cfg), n(cfg.n) {
     x.assign(3 * n, R(0)); v.assign(3 * n, R(0)); ref.assign(3 * n, R(0));
@@ -372,6 +376,9 @@ template <class R> struct Sim {
     heads.assign(cfg.node_count, -1); links.assign(8 * n, -1);
     for (auto* g : {&grid_mass}) g->assign(8 * n, R(0));
     for (auto* g : {&momentum, &grid_impulse, &old_v, &new_v, &remapped, &coupled_remapped}) g->assign(24 * n, R(0));
+    bc_side.assign(n, R(0)); bc_normal.assign(3 * n, R(0)); bc_lever.assign(3 * n, R(0));
+    bc_rhs.assign(n, R(0)); bc_bound.assign(n, R(0)); bc_impulse.assign(n, R(0)); bc_extrap.assign(n, R(0));
+    bc_grid.assign(24 * n, R(0)); bc_coupled.assign(24 * n, R(0));
   }
 
   V3<R> g3(const std::vector<R>& a, int64_t i) const { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
@@ -850,10 +857,122 @@ template <class R> struct Sim {
     }
   }
 
+
+  // ---------------------------------------------------------------- held-blade constraint
+  // BladeConstraint.solve (blade_constraint.py:254-324) with prepare (:13-70), row_bound
+  // (:73-90), scatter (:93-111), project (:130-152), record_residual (:155-168),
+  // finish_refinement (:171-183), apply_grid (:186-194), apply_reaction (:197-205).
+  V3<R> bc_response(int64_t p) {
+    V3<R> r = {R(0), R(0), R(0)};
+    for (int a = 0; a < 8; ++a) {
+      int32_t s = slots[8 * p + a];
+      if (s >= 0 && grid_mass[s] > R(0)) r = r + weights[8 * p + a] * g3(bc_coupled, s) / grid_mass[s];
+    }
+    return r;
+  }
+  void bc_scatter(const std::vector<R>& values, V3<R>& linear, V3<R>& angular) {
+    std::fill(bc_grid.begin(), bc_grid.begin() + 3 * active_count, R(0));
+    linear = {R(0), R(0), R(0)};
+    angular = linear;
+    for (int64_t p = 0; p < n; ++p) {
+      V3<R> j = values[p] * g3(bc_normal, p);
+      for (int a = 0; a < 8; ++a) {
+        int32_t s = slots[8 * p + a];
+        if (s >= 0) add3(bc_grid, s, weights[8 * p + a] * j);
+      }
+      linear = linear + j;
+      angular = angular + cross(g3(bc_lever, p), j);
+    }
+    couple(bc_grid, bc_coupled);
+  }
+  void blade_constraint() {
+    std::fill(bc_grid.begin(), bc_grid.begin() + 3 * active_count, R(0));
+    std::fill(bc_coupled.begin(), bc_coupled.begin() + 3 * active_count, R(0));
+    bool needs = false;
+    Q<R> q = rot();
+    V3<R> pos = g3(tool_pos, 0), hf = half();
+    R bevel = R(c.bevel), radius = R(c.blade_offset), dt = R(c.dt);
+    for (int64_t p = 0; p < n; ++p) {
+      V3<R> local = rotate_inv(q, g3(x, p) - pos);
+      s3(bc_normal, p, {R(0), R(0), R(0)});
+      bc_impulse[p] = R(0);
+      bc_extrap[p] = R(0);
+      if (std::fabs(local.y) >= hf.y || local.z <= -hf.z + bevel || local.z >= hf.z) {
+        bc_side[p] = R(0);
+        continue;
+      }
+      V3<R> grid_v = {R(0), R(0), R(0)};
+      for (int a = 0; a < 8; ++a) {
+        int32_t s = slots[8 * p + a];
+        if (s >= 0) grid_v = grid_v + weights[8 * p + a] * g3(new_v, s);
+      }
+      R sd = bc_side[p];
+      if (sd == R(0) || std::fabs(local.x) > hf.x + radius) sd = local.x < R(0) ? R(-1) : R(1);
+      V3<R> nrm = rotate(q, V3<R>{sd, R(0), R(0)});
+      R gap = sd * local.x - hf.x;
+      if (gap > radius && gap + dt * dot(grid_v, nrm) > R(0)) {
+        bc_side[p] = R(0);
+        continue;
+      }
+      bc_side[p] = sd;
+      s3(bc_normal, p, nrm);
+      s3(bc_lever, p, rotate(q, V3<R>{sd * hf.x, local.y, local.z}));
+      bc_rhs[p] = -std::max(gap, R(0)) / dt - dot(grid_v, nrm);
+      for (int a = 0; a < 8; ++a) {
+        int32_t s = slots[8 * p + a];
+        if (s >= 0) bc_grid[3 * s] += weights[8 * p + a];
+      }
+      if (bc_rhs[p] > R(1e-7)) needs = true;
+    }
+    if (!needs) return;
+    couple(bc_grid, bc_coupled);
+    for (int64_t p = 0; p < n; ++p) {
+      R total = R(0);
+      for (int a = 0; a < 8; ++a) {
+        int32_t s = slots[8 * p + a];
+        if (s >= 0 && grid_mass[s] > R(0)) total += weights[8 * p + a] * bc_coupled[3 * s] / grid_mass[s];
+      }
+      bc_bound[p] = total;
+    }
+    bc_batches = 0;
+    V3<R> linear, angular;
+    while (true) {
+      bc_extrap = bc_impulse;
+      for (int it = 0; it < 48; ++it) {
+        bc_scatter(bc_extrap, linear, angular);
+        R accel = R(it) / R(it + 3);
+        for (int64_t p = 0; p < n; ++p) {
+          V3<R> nrm = g3(bc_normal, p);
+          if (dot(nrm, nrm) == R(0)) continue;
+          V3<R> r = bc_response(p);
+          R previous = bc_impulse[p];
+          R current = std::max(R(0), bc_extrap[p] + (bc_rhs[p] - dot(nrm, r)) / std::max(bc_bound[p], R(1e-12)));
+          bc_impulse[p] = current;
+          bc_extrap[p] = current + accel * (current - previous);
+        }
+      }
+      bc_scatter(bc_impulse, linear, angular);
+      R residual = R(0);
+      for (int64_t p = 0; p < n; ++p) {
+        V3<R> nrm = g3(bc_normal, p);
+        if (dot(nrm, nrm) > R(0)) residual = std::max(residual, std::max(R(0), bc_rhs[p] - dot(nrm, bc_response(p))));
+      }
+      bc_batches += 1;
+      if (residual > R(1e-6) && bc_batches < 8) continue;
+      bc_max_violation = std::max(bc_max_violation, residual);
+      break;
+    }
+    for (int64_t s = 0; s < active_count; ++s)
+      if (grid_mass[s] > R(0)) s3(new_v, s, g3(new_v, s) + g3(bc_coupled, s) / grid_mass[s]);
+    s3(reaction, 0, g3(reaction, 0) - linear);
+    s3(torque, 0, g3(torque, 0) - angular);
+  }
+
   void material_step() {
     if (!c.cache_topology || topology_dirty) register_topology();
     p2g();
     couple_and_velocity();
+    if (c.external_tool && !c.predict) blade_constraint();
     g2p_remap();
     couple(remapped, coupled_remapped);
     update_F();
@@ -1053,6 +1172,204 @@ template <class R> struct Sim {
   }
 };
 
+
+// ---------------------------------------------------------------- MLS-MPM baseline
+struct MlsConfig {
+  int64_t dims[3];
+  int32_t plastic, pad;
+  double origin[3];
+  double dx, dt, mass, volume, mu, lam, damaged_shear, softening, compressed, stretched;
+};
+
+// MlsSolver._substep (mlsmpm.py:293-393) on the particles of `contact` (x, v, tool state):
+// contact impulses with tool_model -1 and no cutting, interval sums, dense-grid P2G,
+// grid update and G2P.
+template <class R> struct MlsSim {
+  MlsConfig m;
+  Sim<R> contact;
+  std::vector<R> C, F, damage, damage_ref, yield, grid_m, grid_p, grid_v, impulse, angular;
+  int64_t inversions = 0;
+
+  MlsSim(const OracleConfig& oc, const MlsConfig& mc) : m(mc), contact(oc) {
+    int64_t n = oc.n, nodes = mc.dims[0] * mc.dims[1] * mc.dims[2];
+    C.assign(9 * n, R(0));
+    F.assign(9 * n, R(0));
+    for (int64_t p = 0; p < n; ++p) for (int i = 0; i < 3; ++i) F[9 * p + 4 * i] = R(1);
+    damage.assign(n, R(0));
+    damage_ref.assign(n, R(1));
+    yield.assign(n, R(oc.yield_stress));
+    grid_m.assign(nodes, R(0));
+    grid_p.assign(3 * nodes, R(0));
+    grid_v.assign(3 * nodes, R(0));
+    impulse.assign(3, R(0));
+    angular.assign(3, R(0));
+  }
+
+  bool base_of(const R* x, int64_t b[3], R fx[3]) const {
+    for (int i = 0; i < 3; ++i) {
+      if (!std::isfinite(double(x[i]))) return false;
+      R local = (x[i] - R(m.origin[i])) / R(m.dx);
+      R fb = std::floor(local - R(0.5));
+      if (!(fb >= R(0) && fb < R(m.dims[i] - 2))) return false;
+      b[i] = int64_t(fb);
+      fx[i] = local - fb;
+    }
+    return true;
+  }
+  static void spline(R f, R w[3]) {
+    w[0] = R(0.5) * (R(1.5) - f) * (R(1.5) - f);
+    w[1] = R(0.75) - (f - R(1)) * (f - R(1));
+    w[2] = R(0.5) * (f - R(0.5)) * (f - R(0.5));
+  }
+  int64_t node(int64_t i, int64_t j, int64_t k) const { return (i * m.dims[1] + j) * m.dims[2] + k; }
+
+  void p2g() {
+    int64_t n = contact.n, nodes = m.dims[0] * m.dims[1] * m.dims[2];
+    std::fill(grid_m.begin(), grid_m.end(), R(0));
+    std::fill(grid_p.begin(), grid_p.end(), R(0));
+    R dx = R(m.dx), mass = R(m.mass);
+    for (int64_t p = 0; p < n; ++p) {
+      int64_t b[3];
+      R fx[3];
+      if (!base_of(&contact.x[3 * p], b, fx)) continue;
+      R w[3][3];
+      for (int a = 0; a < 3; ++a) spline(fx[a], w[a]);
+      R mu = R(m.mu);
+      if (damage[p] > R(0)) mu = R(m.mu) * R(m.damaged_shear);
+      M3<R> Fm;
+      for (int i = 0; i < 9; ++i) Fm.m[i / 3][i % 3] = F[9 * p + i];
+      M3<R> U, V;
+      V3<R> sg;
+      svd3(Fm, U, sg, V);
+      R J = sg.x * sg.y * sg.z;
+      M3<R> D = zero3<R>();
+      D.m[0][0] = R(2) * mu * (sg.x - R(1)) + R(m.lam) * (J - R(1)) * J / sg.x;
+      D.m[1][1] = R(2) * mu * (sg.y - R(1)) + R(m.lam) * (J - R(1)) * J / sg.y;
+      D.m[2][2] = R(2) * mu * (sg.z - R(1)) + R(m.lam) * (J - R(1)) * J / sg.z;
+      M3<R> P = mul(mul(U, D), transpose(V));
+      R scale_ = -R(m.dt) * R(m.volume) * R(4) / (dx * dx);
+      M3<R> stress = mul(scale(scale_, P), transpose(Fm));
+      M3<R> Cm;
+      for (int i = 0; i < 9; ++i) Cm.m[i / 3][i % 3] = C[9 * p + i];
+      M3<R> affine = addm(stress, scale(mass, Cm));
+      V3<R> mom = mass * contact.g3(contact.v, p) + contact.g3(contact.impulses, p);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            R weight = w[0][i] * w[1][j] * w[2][k];
+            V3<R> dpos = {(R(i) - fx[0]) * dx, (R(j) - fx[1]) * dx, (R(k) - fx[2]) * dx};
+            V3<R> c = weight * (mom + mulv(affine, dpos));
+            int64_t nd = node(b[0] + i, b[1] + j, b[2] + k);
+            grid_p[nd] += c.x;
+            grid_p[nodes + nd] += c.y;
+            grid_p[2 * nodes + nd] += c.z;
+            grid_m[nd] += weight * mass;
+          }
+    }
+  }
+
+  void grid_update() {
+    int64_t nodes = m.dims[0] * m.dims[1] * m.dims[2];
+    for (int64_t i = 0; i < m.dims[0]; ++i)
+      for (int64_t j = 0; j < m.dims[1]; ++j)
+        for (int64_t k = 0; k < m.dims[2]; ++k) {
+          int64_t nd = node(i, j, k);
+          R mm = grid_m[nd];
+          R v[3] = {R(0), R(0), R(0)};
+          if (mm > R(0))
+            for (int c = 0; c < 3; ++c) v[c] = grid_p[c * nodes + nd] / mm;
+          if (i < 1 || j < 1 || k < 1 || i >= m.dims[0] - 1 || j >= m.dims[1] - 1 || k >= m.dims[2] - 1)
+            v[0] = v[1] = v[2] = R(0);
+          for (int c = 0; c < 3; ++c) grid_v[c * nodes + nd] = v[c];
+        }
+  }
+
+  void g2p() {
+    int64_t n = contact.n, nodes = m.dims[0] * m.dims[1] * m.dims[2];
+    R dx = R(m.dx), dt = R(m.dt);
+    const R minsv = R(0.05);
+    for (int64_t p = 0; p < n; ++p) {
+      int64_t b[3];
+      R fx[3];
+      if (!base_of(&contact.x[3 * p], b, fx)) continue;
+      R w[3][3];
+      for (int a = 0; a < 3; ++a) spline(fx[a], w[a]);
+      V3<R> nv = {R(0), R(0), R(0)};
+      M3<R> nC = zero3<R>();
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            R weight = w[0][i] * w[1][j] * w[2][k];
+            V3<R> dpos = {(R(i) - fx[0]) * dx, (R(j) - fx[1]) * dx, (R(k) - fx[2]) * dx};
+            int64_t nd = node(b[0] + i, b[1] + j, b[2] + k);
+            V3<R> gv = {grid_v[nd], grid_v[nodes + nd], grid_v[2 * nodes + nd]};
+            nv = nv + weight * gv;
+            nC = addm(nC, scale(R(4) * weight / (dx * dx), outer(gv, dpos)));
+          }
+      contact.s3(contact.v, p, nv);
+      for (int i = 0; i < 9; ++i) C[9 * p + i] = nC.m[i / 3][i % 3];
+      contact.s3(contact.x, p, contact.g3(contact.x, p) + dt * nv);
+      M3<R> Fm;
+      for (int i = 0; i < 9; ++i) Fm.m[i / 3][i % 3] = F[9 * p + i];
+      M3<R> f = mul(addm(eye3<R>(), scale(dt, nC)), Fm);
+      M3<R> U, V;
+      V3<R> s;
+      svd3(f, U, s, V);
+      R smin = std::min(std::min(s.x, s.y), s.z);
+      if (smin <= R(0)) inversions += 1;
+      auto rebuild = [&](V3<R> sv) {
+        M3<R> D = zero3<R>();
+        D.m[0][0] = sv.x; D.m[1][1] = sv.y; D.m[2][2] = sv.z;
+        return mul(mul(U, D), transpose(V));
+      };
+      if (smin < minsv) {
+        s = {std::max(s.x, minsv), std::max(s.y, minsv), std::max(s.z, minsv)};
+        f = rebuild(s);
+      }
+      R ref_J = damage_ref[p];
+      if (m.plastic) {
+        V3<R> e = {std::log(s.x), std::log(s.y), std::log(s.z)};
+        R mean = (e.x + e.y + e.z) / R(3);
+        V3<R> dev = e - V3<R>{mean, mean, mean};
+        R norm = length(dev);
+        R limit = yield[p] / (R(2) * R(m.mu)) * std::sqrt(R(2) / R(3));
+        if (norm > limit) {
+          e = e - (R(1) - limit / norm) * dev;
+          s = {std::exp(e.x), std::exp(e.y), std::exp(e.z)};
+          f = rebuild(s);
+          if (m.softening > 0 && ref_J > R(0)) {
+            yield[p] = std::max(R(0), yield[p] - R(m.softening) * (norm - limit));
+            if (yield[p] <= R(0)) damage[p] = R(1);
+          }
+        }
+      }
+      for (int i = 0; i < 9; ++i) F[9 * p + i] = f.m[i / 3][i % 3];
+      if (ref_J > R(0)) {
+        R J = s.x * s.y * s.z / ref_J;
+        if (J <= R(m.compressed) || J >= R(m.stretched)) damage[p] = R(1);
+      }
+    }
+  }
+
+  void substep() {
+    contact.rigid_contacts(-1, 0);
+    for (int i = 0; i < 3; ++i) {
+      impulse[i] += contact.reaction[i];
+      angular[i] += contact.torque[i];
+      contact.reaction[i] = contact.torque[i] = contact.board[i] = R(0);
+    }
+    contact.speed[0] = R(0);
+    p2g();
+    grid_update();
+    g2p();
+  }
+};
+
+struct MlsHandle {
+  int precision;
+  void* sim;
+};
+
 // ---------------------------------------------------------------- named array access
 struct Handle {
   int precision;
@@ -1069,7 +1386,8 @@ template <class R> bool real_array(Sim<R>& s, const std::string& name, std::vect
       {"tool_table_impulse", &s.table}, {"interval_impulse", &s.interval_impulse},
       {"interval_angular_impulse", &s.interval_angular}, {"grid_mass", &s.grid_mass}, {"momentum", &s.momentum},
       {"grid_impulse", &s.grid_impulse}, {"old_v", &s.old_v}, {"new_v", &s.new_v}, {"remapped", &s.remapped},
-      {"coupled_remapped", &s.coupled_remapped}};
+      {"coupled_remapped", &s.coupled_remapped}, {"bc_side", &s.bc_side}, {"bc_impulse", &s.bc_impulse},
+      {"bc_normal", &s.bc_normal}};
   auto it = table.find(name);
   if (it == table.end()) return false;
   out = it->second;
@@ -1105,6 +1423,12 @@ template <class R> int get_set(Sim<R>& s, const char* cname, void* data, int64_t
   if (name == "grid_keys") return ints(s.grid_keys);
   if (name == "heads") return ints(s.heads);
   if (name == "links") return ints(s.links);
+  if (name == "bc_max_violation") {
+    if (count != 1) return -2;
+    double* dd = static_cast<double*>(data);
+    if (set) s.bc_max_violation = R(dd[0]); else dd[0] = double(s.bc_max_violation);
+    return 0;
+  }
   if (name == "scalars") {
     // [active_count, pair_count, step_count, topology_dirty, single_field]
     if (count != 5) return -2;
@@ -1264,6 +1588,71 @@ int oracle_g2p(double* x, double* v, const int32_t* slots, const double* weights
                const double* new_v, double dt, double flip, int64_t n) {
   g2p_fields<double>(x, v, slots, weights, old_v, new_v, dt, flip, n);
   return 0;
+}
+
+// MLS oracle: created from the contact configuration (tool_model -1 semantics are taken from
+// external_tool = 1) and the grid description.
+void* oracle_mls_create(int precision, const OracleConfig* cfg, const MlsConfig* mc) {
+  MlsHandle* h = new MlsHandle{precision, nullptr};
+  if (precision == 32) h->sim = new MlsSim<float>(*cfg, *mc);
+  else h->sim = new MlsSim<double>(*cfg, *mc);
+  return h;
+}
+
+void oracle_mls_destroy(void* handle) {
+  MlsHandle* h = static_cast<MlsHandle*>(handle);
+  if (!h) return;
+  if (h->precision == 32) delete static_cast<MlsSim<float>*>(h->sim);
+  else delete static_cast<MlsSim<double>*>(h->sim);
+  delete h;
+}
+
+int oracle_mls_substep(void* handle, int steps) {
+  MlsHandle* h = static_cast<MlsHandle*>(handle);
+  for (int i = 0; i < steps; ++i) {
+    if (h->precision == 32) static_cast<MlsSim<float>*>(h->sim)->substep();
+    else static_cast<MlsSim<double>*>(h->sim)->substep();
+  }
+  return 0;
+}
+
+// Named arrays of the MLS oracle (float64 transfer): x, v, C, F, damage, damage_reference,
+// yield_stress, impulse, angular_impulse, tool_pos, tool_vel, tool_rot, tool_omega, inversions.
+int oracle_mls_array(void* handle, const char* cname, double* data, int64_t count, int set) {
+  MlsHandle* h = static_cast<MlsHandle*>(handle);
+  std::string name(cname);
+  auto go = [&](auto* s) -> int {
+    using RR = std::decay_t<decltype(s->C[0])>;
+    std::vector<RR>* a = nullptr;
+    if (name == "x") a = &s->contact.x;
+    else if (name == "v") a = &s->contact.v;
+    else if (name == "C") a = &s->C;
+    else if (name == "F") a = &s->F;
+    else if (name == "damage") a = &s->damage;
+    else if (name == "damage_reference") a = &s->damage_ref;
+    else if (name == "yield_stress") a = &s->yield;
+    else if (name == "impulse") a = &s->impulse;
+    else if (name == "angular_impulse") a = &s->angular;
+    else if (name == "tool_pos") a = &s->contact.tool_pos;
+    else if (name == "tool_vel") a = &s->contact.tool_vel;
+    else if (name == "tool_rot") a = &s->contact.tool_rot;
+    else if (name == "tool_omega") a = &s->contact.tool_omega;
+    else if (name == "histories") a = &s->contact.histories;
+    else if (name == "inversions") {
+      if (count != 1) return -2;
+      if (set) s->inversions = int64_t(data[0]); else data[0] = double(s->inversions);
+      return 0;
+    }
+    if (!a) return -1;
+    if (count != int64_t(a->size())) return -2;
+    for (int64_t i = 0; i < count; ++i) {
+      if (set) (*a)[i] = RR(data[i]);
+      else data[i] = double((*a)[i]);
+    }
+    return 0;
+  };
+  if (h->precision == 32) return go(static_cast<MlsSim<float>*>(h->sim));
+  return go(static_cast<MlsSim<double>*>(h->sim));
 }
 
 }  // extern "C"

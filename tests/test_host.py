@@ -237,3 +237,34 @@ def test_library_argument_errors_need_no_device():
     lib = _lib.load()
     assert lib.bm_refresh_stress(None, None, None) < 0
     assert lib.bm_substep(None, None, 0, 0, 1, 0, 0, None, None, 0, None) < 0
+
+
+@pytest.mark.parametrize("shape,size,spacing,fields", [("block", (0.3, 0.15, 0.075), 0.003, 4),
+                                                       ("banana", BANANA_DEMO_SIZE_M, 0.003, 4),
+                                                       ("block", (0.048, 0.024, 0.018), 0.006, 32)])
+def test_brick_bins_cover_every_particle_window(shape, size, spacing, fields):
+    from blademaster_hip.geometry import bin_bricks
+
+    ref, origin, cells = points(size, spacing, shape)
+    base, local, base_cell = compact_stencil(ref, origin, spacing, cells)
+    brick, bricks, perm, start = bin_bricks(cells, base_cell, fields)
+    np.testing.assert_array_equal(np.sort(perm), np.arange(len(ref)))
+    assert start[0] == 0 and start[-1] == len(ref) and np.all(np.diff(start) >= 0)
+    assert np.all(bricks * brick >= cells)
+    assert 9 * fields * np.prod(brick + 1) * 4 <= 65536
+    per_brick = int(np.prod(brick))
+    assert len(start) == int(np.prod(bricks)) * per_brick + 1
+    # the particles of slot c are exactly those of cell c of the brick order
+    slot = np.repeat(np.arange(len(start) - 1), np.diff(start))
+    nb, cl = slot // per_brick, slot % per_brick
+    bz = nb % bricks[2]
+    by = (nb // bricks[2]) % bricks[1]
+    bx = nb // (bricks[1] * bricks[2])
+    lz = cl % brick[2]
+    ly = (cl // brick[2]) % brick[1]
+    lx = cl // (brick[1] * brick[2])
+    cell = np.stack([bx * brick[0] + lx, by * brick[1] + ly, bz * brick[2] + lz], 1)
+    np.testing.assert_array_equal(base_cell[perm], cell)
+    # reference order inside a cell
+    same = slot[1:] == slot[:-1]
+    assert np.all(perm[1:][same] > perm[:-1][same])
