@@ -69,10 +69,9 @@ STATE_FIELDS = [
     ("errors", c_i32p), ("topology_dirty", c_i32p),
     ("pair_keys", c_i64p), ("pair_history", c_f32p), ("pair_used", c_i32p), ("pair_count", c_i32p),
     ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
-    ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("bucket_count", c_i32p),
-    ("bucket_start", c_i32p), ("scan_partial", c_i32p), ("particle_bucket", c_i32p), ("particle_rank", c_i32p),
-    ("sorted_x", c_f32p), ("perm", c_i32p), ("cell_start", c_i32p), ("p2g_partial", c_f32p),
-    ("remap_partial", c_f32p), ("rigid_partial", c_f32p), ("counters", ctypes.POINTER(ctypes.c_uint32)),
+    ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("list_heads", c_i64p),
+    ("list_links", c_i32p), ("perm", c_i32p), ("p2g_partial", c_f32p), ("remap_partial", c_f32p),
+    ("rigid_partial", c_f32p), ("counters", ctypes.POINTER(ctypes.c_uint32)),
     ("bc_side", c_f32p), ("bc_normal", c_f32p), ("bc_lever", c_f32p), ("bc_rhs", c_f32p), ("bc_bound", c_f32p),
     ("bc_impulse", c_f32p), ("bc_extrapolated", c_f32p), ("bc_active", c_i32p), ("bc_nodes", c_i32p),
     ("bc_node_mark", c_i32p), ("bc_grid", c_f32p), ("bc_coupled", c_f32p), ("bc_counts", c_i32p),

@@ -131,16 +131,19 @@ LDS_BYTES = 65536
 def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY):
     """Particle bins of the brick kernels.
 
-    Returns (brick cells per axis, bricks per axis, perm, cell_start): bricks are numbered
-    x-major, the cells of a brick z-fastest, and the particles of the c-th cell of the grid in
-    this order are perm[cell_start[c]:cell_start[c + 1]] (reference order within a cell).
-    The G2P kernel keeps 9 floats per (field slot, window node) in LDS, which bounds the
-    brick's window.
+    Returns (brick cells per axis, bricks per axis, slots): bricks are numbered x-major, the
+    cells of a brick z-fastest, and the c-th cell in this order holds the particles
+    slots[8c:8c + 8] in reference order, -1 for unused slots.  The sampling puts at most 8
+    particles in a cell (2 per axis, jitter stays inside the sub-cell).
+    The brick kernels keep per (field slot, window node) 7 (P2G) or 9 (G2P) floats in LDS,
+    plus 7 or 3 floats per (cell, corner); the larger must fit LDS_BYTES.
     """
     cells = np.asarray(cells, dtype=np.int64)
 
     def fits(bx, by, bz):
-        return 9 * fields * (bx + 1) * (by + 1) * (bz + 1) * 4 <= LDS_BYTES
+        window = fields * (bx + 1) * (by + 1) * (bz + 1)
+        corners = 8 * bx * by * bz
+        return max(7 * window + 7 * corners, 9 * window + 3 * corners) * 4 <= LDS_BYTES
 
     bx = by = brick_xy
     bz = int(min(cells[2], 32))
@@ -157,7 +160,13 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY):
     local = ((base_cell[:, 0] % bx) * by + base_cell[:, 1] % by) * bz + base_cell[:, 2] % bz
     per_brick = int(bx * by * bz)
     key = ids * per_brick + local
-    perm = np.argsort(key, kind="stable").astype(np.int32)
+    perm = np.argsort(key, kind="stable")
     nb = int(np.prod(bricks))
-    cell_start = np.searchsorted(key[perm], np.arange(nb * per_brick + 1)).astype(np.int32)
-    return brick, bricks, perm, cell_start
+    sorted_key = key[perm]
+    start = np.searchsorted(sorted_key, sorted_key)
+    rank = np.arange(len(perm)) - start
+    if len(rank) and rank.max() >= 8:
+        raise ValueError("more than 8 particles in a grid cell")
+    slots = np.full(8 * nb * per_brick, -1, dtype=np.int32)
+    slots[8 * sorted_key + rank] = perm
+    return brick, bricks, slots

@@ -158,14 +158,11 @@ class Solver:
         T["built_codes"] = t((n,), i64)
         T["partners"] = t((max(n, 1) * PAIR_LIST_LANES,), i32)
         T["partner_count"] = t((n,), i32)
-        T["bucket_count"] = t((self.list_buckets + 1,), i32)
-        T["bucket_start"] = t((self.list_buckets + 1,), i32)
-        T["scan_partial"] = t(((self.list_buckets + 4095) // 4096 + 1,), i32)
-        T["particle_bucket"] = t((n,), i32)
-        T["particle_rank"] = t((n,), i32)
-        T["sorted_x"] = t((n, 4))
-        T["perm"] = torch.tensor(self.brick_perm, dtype=i32, device=dev)
-        T["cell_start"] = torch.tensor(self.cell_start, dtype=i32, device=dev)
+        # Bucket heads carry the build count in their upper 32 bits; a head of an older build
+        # reads as an empty bucket, so the lists need no clearing pass.
+        T["list_heads"] = t((self.list_buckets,), i64)
+        T["list_links"] = t((max(n, 1),), i32, -1)
+        T["perm"] = torch.tensor(self.brick_slots, dtype=i32, device=dev)
         window = int(np.prod(np.asarray(self.brick) + 1))
         nb = int(np.prod(self.bricks))
         T["p2g_partial"] = t((max(nb * F_ * window * 7, 1),))
@@ -305,9 +302,7 @@ class Solver:
         self._t["pair_dirty"].fill_(1)
 
     def _bin_bricks(self, base_cell):
-        self.brick, self.bricks, self.brick_perm, self.cell_start = bin_bricks(
-            self.cells, base_cell, int(self.config.fields_per_node)
-        )
+        self.brick, self.bricks, self.brick_slots = bin_bricks(self.cells, base_cell, int(self.config.fields_per_node))
 
     def launch(self, name, *args):
         """Launch one C-ABI stage by name with the solver's params/state (cf. solver.py:331)."""

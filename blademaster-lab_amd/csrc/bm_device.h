@@ -307,6 +307,14 @@ __device__ __forceinline__ void stencil_full(f3 l, float inv_h, Stencil& st) {
     st.g[a] = f3{sx * (fy[iy] * fz[iz]) * inv_h, sy * (fx[ix] * fz[iz]) * inv_h, sz * (fx[ix] * fy[iy]) * inv_h};
   }
 }
+// Weight and reference gradient of one corner (the same products as stencil_full).
+__device__ __forceinline__ void stencil_corner(f3 l, float inv_h, int a, float& w, f3& g) {
+  int ix = (a >> 2) & 1, iy = (a >> 1) & 1, iz = a & 1;
+  float fx = ix ? l.x : 1.0f - l.x, fy = iy ? l.y : 1.0f - l.y, fz = iz ? l.z : 1.0f - l.z;
+  float sx = ix ? 1.0f : -1.0f, sy = iy ? 1.0f : -1.0f, sz = iz ? 1.0f : -1.0f;
+  w = fx * fy * fz;
+  g = f3{sx * (fy * fz) * inv_h, sy * (fx * fz) * inv_h, sz * (fx * fy) * inv_h};
+}
 // Node offset of corner a in a grid with (dy, dz) node counts along y and z.
 __device__ __forceinline__ int corner_offset(int a, int dyz, int dz) {
   return ((a >> 2) & 1) * dyz + ((a >> 1) & 1) * dz + (a & 1);

@@ -223,6 +223,7 @@ enum RigidMode : int {
   kFinishExternal = 4,    // then finish_external_step (last block)
   kAccumulateInterval = 8,// accumulate_reaction before the tool step
   kPairPrep = 16,         // clear stale pair slots and bin particles for a pending list rebuild
+  kTopologyPrep = 32,     // clear the field table for a pending registration
 };
 
 struct ToolTarget {
@@ -236,12 +237,14 @@ __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float
 __device__ void finish_external_dev(const bm_params& P, const bm_state& S);
 __device__ void accumulate_reaction_dev(const bm_state& S);
 
-__device__ __forceinline__ int bucket_of_point(f3 x, float cell, int mask);
+__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag);
 
 constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
 
 // rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
-// Per-block sums go to rigid_partial; k_rigid_finish reduces them.
+// Per-block sums go to rigid_partial; k_prep reduces them.  In a substep the same pass also
+// empties the pair slots of the new table, bins particles for a pair-list rebuild and clears
+// the field table for a re-registration.
 __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
                                                   int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -253,23 +256,24 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   f3 origin = ld3(S.tool_pos, 0), tvel = ld3(S.tool_vel, 0), tom = ld3(S.tool_omega, 0);
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
+  int64_t stride = int64_t(gridDim.x) * blockDim.x;
   if ((mode & kPairPrep) != 0) {
     // clear_pair_slots (contact.py:643-652): empty the slots of the table used two steps ago.
     int count = S.pair_count[parity_new];
     int64_t* keys = S.pair_keys + int64_t(parity_new) * P.pair_capacity;
     const int32_t* used = S.pair_used + int64_t(parity_new) * P.pair_capacity;
-    for (int64_t a = p; a < count; a += int64_t(gridDim.x) * blockDim.x) keys[used[a]] = -1;
+    for (int64_t a = p; a < count; a += stride) keys[used[a]] = -1;
+  }
+  if ((mode & kTopologyPrep) != 0 && S.topology_dirty[0] != 0) {
+    // register, first half (solver.py:392-393): empty field table before the inserts.
+    int64_t total = P.fields * P.node_count;
+    for (int64_t i = p; i < total; i += stride) S.field_code[i] = -1;
+    for (int64_t i = p; i < P.node_count; i += stride) S.node_nf[i] = 0;
   }
   if (live) {
     f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
     spd = len(vp);
-    if ((mode & kPairPrep) != 0 && S.pair_dirty[0] != 0) {
-      // bin_particles for the rebuild of this step (cells of the list's base size)
-      float cell = 2.0f * float(P.radius) + float(P.pair_skin);
-      int b = bucket_of_point(xp, cell, int(P.list_buckets - 1));
-      S.particle_bucket[p] = b;
-      S.particle_rank[p] = atomicAdd(S.bucket_count + b, 1);
-    }
+    if ((mode & kPairPrep) != 0 && S.pair_dirty[0] != 0) bin_particle(P, S, p, xp, S.pair_builds[0] + 1);
     f3 total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
     // board: plane z = 0 inside its rectangle
     float gap = xp.z - float(P.radius);
@@ -364,10 +368,19 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   }
 }
 
-// Reduction of the contact kernel's block sums in block order (deterministic), then the
-// single-thread work of the step: apply the sums, list reach, knife update.
-__global__ void __launch_bounds__(kBlock) k_rigid_finish(const bm_params P, bm_state S, int nblocks, int mode,
-                                                         ToolTarget tt, int parity_new) {
+__device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
+
+// Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
+// the single-thread work of the step: apply the sums, list reach, knife update.  Other
+// blocks: registration of every particle's field slots when the topology changed.
+__global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
+                                                 int parity_new) {
+  if (blockIdx.x > 0) {
+    if ((mode & kTopologyPrep) == 0 || S.topology_dirty[0] == 0) return;
+    int64_t stride = int64_t(gridDim.x - 1) * blockDim.x;
+    for (int64_t p = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; p < P.n; p += stride) insert_particle(P, S, p);
+    return;
+  }
   __shared__ float part[kBlock][kRigidSums];
   float acc[kRigidSums];
 #pragma unroll
@@ -416,6 +429,7 @@ __global__ void __launch_bounds__(kBlock) k_rigid_finish(const bm_params P, bm_s
     if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
     S.list_reach[0] = value;
     S.pair_count[parity_new] = 0;
+    if (S.pair_dirty[0] != 0) S.pair_builds[0] += 1;
   }
   if (mode & kAccumulateInterval) accumulate_reaction_dev(S);
   if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
@@ -486,164 +500,35 @@ __global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
   const int32_t* used = S.pair_used + int64_t(parity_new) * P;
   for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < count; a += gridDim.x * blockDim.x) keys[used[a]] = -1;
 }
-__global__ void k_zero_pair_count(bm_state S, int parity_new) { S.pair_count[parity_new] = 0; }
 
-// set_pair_reach (contact.py:749-761) + zero bucket counts; only when the list is stale.
-__global__ void k_list_reset(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
-  int64_t B = P.list_buckets;
-  for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b <= B; b += int64_t(gridDim.x) * blockDim.x)
-    S.bucket_count[b] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt), sp = S.speed[0];
-    float value = 2.0f * r + skin;
-    if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
-    S.list_reach[0] = value;
-  }
+// bin_particles (contact.py:764-775) into bucket lists without a clearing pass: a head
+// written by an earlier build carries an older tag and reads as an empty bucket.
+__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag) {
+  int b = bucket_of_point(xp, list_cell_size(P), int(P.list_buckets - 1));
+  unsigned long long entry = (uint64_t(uint32_t(tag)) << 32) | uint64_t(uint32_t(p));
+  unsigned long long old = atomicExch(reinterpret_cast<unsigned long long*>(S.list_heads + b), entry);
+  S.list_links[p] = (uint32_t(old >> 32) == uint32_t(tag)) ? int32_t(uint32_t(old)) : -1;
+}
+__device__ __forceinline__ int32_t bucket_head(const bm_state& S, int b, int tag) {
+  uint64_t e = uint64_t(S.list_heads[b]);
+  return (uint32_t(e >> 32) == uint32_t(tag)) ? int32_t(uint32_t(e)) : -1;
 }
 
-__global__ void k_list_count(const bm_params P, bm_state S) {
+// Standalone set_pair_reach + binning (contact.py:749-775) for bm_pair_contacts.
+__global__ void k_list_bin(const bm_params P, bm_state S) {
   if (S.pair_dirty[0] == 0) return;
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  int b = bucket_of_point(ld3(S.x, p), list_cell_size(P), int(P.list_buckets - 1));
-  S.particle_bucket[p] = b;
-  S.particle_rank[p] = atomicAdd(S.bucket_count + b, 1);
+  bin_particle(P, S, p, ld3(S.x, p), S.pair_builds[0] + 1);
 }
-
-// Exclusive scan of bucket_count[0..B) into bucket_start, three passes of 4096 entries per
-// block (1024 threads x 4).
-__global__ void __launch_bounds__(1024) k_scan_partial(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
-  __shared__ int wsum[16];
-  int64_t base = int64_t(blockIdx.x) * 4096;
-  int t = threadIdx.x;
-  int sum = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int64_t i = base + 4 * t + k;
-    if (i < P.list_buckets) sum += S.bucket_count[i];
-  }
-  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-  if ((t & 63) == 0) wsum[t >> 6] = sum;
-  __syncthreads();
-  if (t == 0) {
-    int total = 0;
-    for (int w = 0; w < 16; ++w) total += wsum[w];
-    S.scan_partial[blockIdx.x] = total;
-  }
+__global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
+  float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt), sp = S.speed[0];
+  float value = 2.0f * r + skin;
+  if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
+  S.list_reach[0] = value;
+  S.pair_count[parity_new] = 0;
+  if (S.pair_dirty[0] != 0) S.pair_builds[0] += 1;
 }
-__global__ void __launch_bounds__(1024) k_scan_top(const bm_params P, bm_state S, int nblocks) {
-  if (S.pair_dirty[0] == 0) return;
-  __shared__ int vals[4096];
-  int t = threadIdx.x;
-  for (int i = t; i < 4096; i += 1024) vals[i] = i < nblocks ? S.scan_partial[i] : 0;
-  __syncthreads();
-  if (t == 0) {
-    int run = 0;
-    for (int i = 0; i < nblocks; ++i) {
-      int v = vals[i];
-      vals[i] = run;
-      run += v;
-    }
-    S.bucket_start[P.list_buckets] = run;
-  }
-  __syncthreads();
-  for (int i = t; i < nblocks; i += 1024) S.scan_partial[i] = vals[i];
-}
-__global__ void __launch_bounds__(1024) k_scan_final(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
-  __shared__ int wsum[16];
-  int64_t base = int64_t(blockIdx.x) * 4096;
-  int t = threadIdx.x;
-  int v[4];
-  int local = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int64_t i = base + 4 * t + k;
-    v[k] = i < P.list_buckets ? S.bucket_count[i] : 0;
-    local += v[k];
-  }
-  // inclusive scan of `local` across the block
-  int lane = t & 63, wave = t >> 6;
-  int incl = local;
-  for (int off = 1; off < 64; off <<= 1) {
-    int o = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += o;
-  }
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  int wave_base = 0;
-  for (int w = 0; w < wave; ++w) wave_base += wsum[w];
-  int run = S.scan_partial[blockIdx.x] + wave_base + incl - local;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int64_t i = base + 4 * t + k;
-    if (i < P.list_buckets) S.bucket_start[i] = run;
-    run += v[k];
-  }
-}
-
-// Scatter particles into bucket order, then build the partner lists (build_pair_list,
-// contact.py:778-822): partners q > p within reach, in their own cell, not compatible
-// reference neighbours.
-__global__ void k_list_scatter(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  // The scan has consumed the counts: leave them zero for the next rebuild's binning.
-  for (int64_t b = p; b <= P.list_buckets; b += int64_t(gridDim.x) * blockDim.x) S.bucket_count[b] = 0;
-  if (p >= P.n) return;
-  int dst = S.bucket_start[S.particle_bucket[p]] + S.particle_rank[p];
-  f3 xp = ld3(S.x, p);
-  float4 e = make_float4(xp.x, xp.y, xp.z, __int_as_float(int(p)));
-  reinterpret_cast<float4*>(S.sorted_x)[dst] = e;
-}
-
-__global__ void k_list_build(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p == 0) S.pair_builds[0] += 1;
-  if (p >= P.n) return;
-  float reach = S.list_reach[0];
-  int mask = int(P.list_buckets - 1);
-  int lanes = int(P.list_lanes);
-  f3 xp = ld3(S.x, p);
-  st3(S.built_x, p, xp);
-  int64_t cp = S.codes[p];
-  S.built_codes[p] = cp;
-  f3 rp = ld3(S.ref, p);
-  float two_h = 2.0f * float(P.spacing);
-  float cell = list_cell_size(P);
-  int rings = list_rings(P, reach);
-  Cell home = cell_of(xp, cell);
-  int count = 0;
-  const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
-  for (int dx = -rings; dx <= rings; ++dx)
-    for (int dy = -rings; dy <= rings; ++dy)
-      for (int dz = -rings; dz <= rings; ++dz) {
-        Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-        int b = bucket_of(c, mask);
-        int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
-        for (int k = lo; k < hi; ++k) {
-          float4 e = sx[k];
-          int q = __float_as_int(e.w);
-          if (q <= p || count < 0) continue;
-          f3 xq = f3{e.x, e.y, e.z};
-          Cell cq = cell_of(xq, cell);
-          if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
-          if (!(len(xq - xp) < reach)) continue;
-          if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
-          if (count == lanes) {
-            count = -1;
-          } else {
-            S.partners[p * lanes + count] = q;
-            count += 1;
-          }
-        }
-      }
-  S.partner_count[p] = count;
-}
-__global__ void k_list_done(bm_state S) { S.pair_dirty[0] = 0; }
 
 struct PairCtx {
   const int64_t* old_keys;
@@ -692,46 +577,80 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
   atomic_add3(S.impulses, q, -impulse);
 }
 
-// pairs_listed (contact.py:825-925)
-__global__ void k_pairs_listed(const bm_params P, bm_state S, int parity_old) {
+// Particle-pair contacts of particle p with partners q > p.  When the list is stale, p first
+// rebuilds its own partner list (build_pair_list, contact.py:778-822) from the bucket lists
+// binned by the contact kernel: partners within reach that pair_at does not skip as
+// compatible reference neighbours, at most `lanes`, else -1 and a bucket search at every
+// step (pairs_listed, contact.py:825-925).
+__global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  // The list is current from here on (list_done, solver.py:786).
-  if (p == 0) S.pair_dirty[0] = 0;
   if (p >= P.n) return;
-  int count = S.partner_count[p];
+  int tag = S.pair_builds[0];
+  float reach = S.list_reach[0];
+  float cell = list_cell_size(P);
+  int rings = list_rings(P, reach);
+  int mask = int(P.list_buckets - 1);
+  int lanes = int(P.list_lanes);
+  int count;
+  bool dirty = S.pair_dirty[0] != 0;
+  if (dirty) {
+    f3 xp = ld3(S.x, p);
+    st3(S.built_x, p, xp);
+    int64_t cp = S.codes[p];
+    S.built_codes[p] = cp;
+    f3 rp = ld3(S.ref, p);
+    float two_h = 2.0f * float(P.spacing);
+    Cell home = cell_of(xp, cell);
+    count = 0;
+    for (int dx = -rings; dx <= rings; ++dx)
+      for (int dy = -rings; dy <= rings; ++dy)
+        for (int dz = -rings; dz <= rings; ++dz) {
+          Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+          for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
+            if (q <= p || count < 0) continue;
+            f3 xq = ld3(S.x, q);
+            Cell cq = cell_of(xq, cell);
+            if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+            if (!(len(xq - xp) < reach)) continue;
+            if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
+            if (count == lanes) {
+              count = -1;
+            } else {
+              S.partners[p * lanes + count] = q;
+              count += 1;
+            }
+          }
+        }
+    S.partner_count[p] = count;
+  } else {
+    count = S.partner_count[p];
+  }
   if (count == 0) return;
   int parity_new = 1 - parity_old;
   int64_t cap = P.pair_capacity;
   PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
             S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, S.pair_count + parity_new};
-  int lanes = int(P.list_lanes);
   if (count > 0) {
     for (int k = 0; k < count; ++k) pair_at(P, S, T, p, S.partners[p * lanes + k]);
     return;
   }
-  // Overflowing particle: search the buckets of this build at the build positions.
-  float reach = S.list_reach[0];
-  float cell = list_cell_size(P);
-  int rings = list_rings(P, reach);
-  int mask = int(P.list_buckets - 1);
-  const float4* sx = reinterpret_cast<const float4*>(S.sorted_x);
-  Cell home = cell_of(ld3(S.built_x, p), cell);
+  // Overflowing particle: search the buckets of the last build around its build position.
+  // In a build step built_x is being written by this kernel; the build positions are x.
+  const float* bx = dirty ? S.x : S.built_x;
+  Cell home = cell_of(ld3(bx, p), cell);
   for (int dx = -rings; dx <= rings; ++dx)
     for (int dy = -rings; dy <= rings; ++dy)
       for (int dz = -rings; dz <= rings; ++dz) {
         Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-        int b = bucket_of(c, mask);
-        int lo = S.bucket_start[b], hi = S.bucket_start[b + 1];
-        for (int k = lo; k < hi; ++k) {
-          float4 e = sx[k];
-          int q = __float_as_int(e.w);
+        for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
           if (q <= p) continue;
-          Cell cq = cell_of(f3{e.x, e.y, e.z}, cell);
+          Cell cq = cell_of(ld3(bx, q), cell);
           if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
           pair_at(P, S, T, p, q);
         }
       }
 }
+__global__ void k_pairs_done(bm_state S) { S.pair_dirty[0] = 0; }
 
 // check_pair_list (contact.py:716-735) as its own pass for a standalone pair-contact call.
 __global__ void k_check_pair_list(const bm_params P, bm_state S) {
@@ -745,92 +664,53 @@ __global__ void k_check_pair_list(const bm_params P, bm_state S) {
 }
 
 // ------------------------------------------------------------------------ topology
-// Rebuild of the field table when topology_dirty is set.
-__global__ void k_topo_clear(const bm_params P, bm_state S) {
-  if (S.topology_dirty[0] == 0) return;
-  int64_t total = P.fields * P.node_count;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
-    S.field_code[i] = -1;
-}
-
 __device__ __forceinline__ int corner_node(int32_t base, int a, int dyz, int dz) { return base + corner_offset(a, dyz, dz); }
 
-// Insert the transfer code of every particle at its 8 stencil nodes (register,
-// material.py:50-65); at most F codes per node, else errors[0].
-__global__ void k_topo_insert(const bm_params P, bm_state S) {
-  if (S.topology_dirty[0] == 0) return;
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
+// register (material.py:50-65): the field slot of the particle's transfer code at each of
+// its 8 nodes, claiming the first empty slot when the code is new there.  Used slots stay a
+// prefix of the node's F slots; a node needing more than F fields counts errors[0].
+__device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p) {
   int64_t code = S.transfer_codes[p];
   int32_t base = S.base_node[p];
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
   int64_t N = P.node_count;
   int F = int(P.fields);
-  for (int a = 0; a < 8; ++a) {
-    int64_t node = corner_node(base, a, dyz, dz);
-    bool done = false;
-    for (int f = 0; f < F && !done; ++f) {
-      int64_t* slot = S.field_code + int64_t(f) * N + node;
-      int64_t cur = *slot;
-      if (cur == code) {
-        done = true;
-        break;
-      }
-      if (cur == -1) {
-        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)(-1LL),
-                                            (unsigned long long)code);
-        if (int64_t(prev) == -1 || int64_t(prev) == code) done = true;
-      }
-    }
-    if (!done) atomicAdd(S.errors + 0, 1);
-  }
-}
-
-// Canonical slot order per node (ascending code) and the used-slot count.
-__global__ void k_topo_sort(const bm_params P, bm_state S) {
-  if (S.topology_dirty[0] == 0) return;
-  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= P.node_count) return;
-  int64_t N = P.node_count;
-  int F = int(P.fields);
-  int count = 0;
-  for (int f = 0; f < F; ++f) {
-    int64_t c = S.field_code[int64_t(f) * N + node];
-    if (c < 0) break;
-    // insertion into the sorted prefix [0, count)
-    int k = count;
-    while (k > 0 && S.field_code[int64_t(k - 1) * N + node] > c) {
-      S.field_code[int64_t(k) * N + node] = S.field_code[int64_t(k - 1) * N + node];
-      --k;
-    }
-    S.field_code[int64_t(k) * N + node] = c;
-    ++count;
-  }
-  S.node_nf[node] = count;
-}
-
-// Field slot of each stencil corner.
-__global__ void k_topo_index(const bm_params P, bm_state S) {
-  if (S.topology_dirty[0] == 0) return;
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  int64_t code = S.transfer_codes[p];
-  int32_t base = S.base_node[p];
-  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
-  int64_t N = P.node_count;
   uint32_t packed[2] = {0u, 0u};
   for (int a = 0; a < 8; ++a) {
     int64_t node = corner_node(base, a, dyz, dz);
-    int nf = S.node_nf[node];
-    int found = 0;
-    for (int f = 0; f < nf; ++f)
-      if (S.field_code[int64_t(f) * N + node] == code) {
+    int found = -1;
+    for (int f = 0; f < F; ++f) {
+      int64_t* slot = S.field_code + int64_t(f) * N + node;
+      int64_t cur = *slot;
+      if (cur == -1) {
+        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)(-1LL),
+                                            (unsigned long long)code);
+        cur = int64_t(prev) == -1 ? code : int64_t(prev);
+        if (int64_t(prev) == -1) atomicMax(S.node_nf + node, f + 1);
+      }
+      if (cur == code) {
         found = f;
         break;
       }
+    }
+    if (found < 0) {
+      atomicAdd(S.errors + 0, 1);
+      found = 0;
+    }
     packed[a >> 2] |= uint32_t(found & 0xff) << (8 * (a & 3));
   }
   reinterpret_cast<uint2*>(S.field_idx)[p] = make_uint2(packed[0], packed[1]);
+}
+
+__global__ void k_topo_clear(const bm_params P, bm_state S) {
+  int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t i = t; i < P.fields * P.node_count; i += stride) S.field_code[i] = -1;
+  for (int64_t i = t; i < P.node_count; i += stride) S.node_nf[i] = 0;
+}
+__global__ void k_topo_insert(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < P.n) insert_particle(P, S, p);
 }
 __global__ void k_topo_done(bm_state S) { S.topology_dirty[0] = 0; }
 
@@ -904,74 +784,139 @@ __device__ __forceinline__ int cell_window_base(const bm_params& P, const Window
 }
 __device__ __forceinline__ int cells_per_brick(const bm_params& P) { return int(P.brick[0] * P.brick[1] * P.brick[2]); }
 
+// Sum over the 8 lanes of a cell (lanes 8c .. 8c+7 of the wavefront) with DPP lane moves:
+// swap neighbours, swap pairs, then mirror the two quads; every lane of the group ends with
+// the total.  Call with the whole wavefront active.
+template <int Ctrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float cell_sum(float v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror: lane i <- 7 - i within 8
+  return v;
+}
+// Any particle in this lane's cell, from a wavefront ballot of `live`.
+__device__ __forceinline__ bool cell_occupied(bool live) {
+  uint64_t mask = __ballot(live);
+  return ((mask >> (threadIdx.x & 56)) & 0xffull) != 0;
+}
+// Sum over the (up to 8) cells of the brick around window node wi of their corner entries
+// cc[k * CS + 8 c + a], in fixed corner order.
+template <int K>
+__device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w, const float* cc, int CS, int wi,
+                                             float* out) {
+  int bx = int(P.brick[0]), by = int(P.brick[1]), bz = int(P.brick[2]);
+  int iz = wi % w.wz, iy = (wi / w.wz) % w.wy, ix = wi / (w.wz * w.wy);
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    int cx = ix - ((a >> 2) & 1), cy = iy - ((a >> 1) & 1), cz = iz - (a & 1);
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= bx || cy >= by || cz >= bz) continue;
+    int e = 8 * ((cx * by + cy) * bz + cz) + a;
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] += cc[k * CS + e];
+  }
+}
+
 // p2g (material.py:127-152) for one brick: mass, momentum and stress + contact impulse.
-// One thread per cell sums its particles' contributions per stencil corner in registers and
-// adds each corner once to the LDS window (a particle whose field differs from the cell's
-// first particle at that corner adds its own).  Lanes hold different cells, so the LDS adds
-// of one instruction never share an address.
-__global__ void __launch_bounds__(256) k_p2g(const bm_params P, bm_state S, int clear_topology_flag) {
+// One lane per particle slot, 8 slots per cell.  Per stencil corner the cell's lanes sum the
+// 7 values of its particles on field slot 0 (DPP reduction) and lane 8c + a stores the sums of
+// corner a in a per-(cell, corner) LDS table with plain stores; a particle on another field
+// slot (a cut through the node) adds its values to an LDS node window with atomics.  Each
+// window node then gathers its cells' corner sums in fixed order, adds the window, and the
+// brick's partial goes out with plain stores.
+__global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags) {
   extern __shared__ float lds[];
   __shared__ int fmax_shared;
   int b = blockIdx.x;
-  if (clear_topology_flag && b == 0 && threadIdx.x == 0) S.topology_dirty[0] = 0;
+  if (flags && b == 0 && threadIdx.x == 0) {
+    // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
+    if (flags & 1) S.topology_dirty[0] = 0;
+    if (flags & 2) S.pair_dirty[0] = 0;
+  }
   Window w = window_of(P, b);
   int F = int(P.fields);
   int Fb = window_fields(P, S, w, &fmax_shared);
   int FW = F * w.W;
-  for (int k = 0; k < 7; ++k)
-    for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) lds[k * FW + i] = 0.0f;
-  __syncthreads();
-  float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
   int cpb = cells_per_brick(P);
-  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
-    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
-    if (lo == hi) continue;
-    int wb = cell_window_base(P, w, c);
-    float acc[8][7];
-    int f0[8];
+  int CS = 8 * cpb;
+  float* win = lds;          // [7][F][W]
+  float* cc = lds + 7 * FW;  // [7][CS]
+  bool multi = Fb > 1;
+  if (multi) {
+    for (int k = 0; k < 7; ++k)
+      for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) win[k * FW + i] = 0.0f;
+    __syncthreads();
+  }
+  float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
+  int lane = int(threadIdx.x & 63);
+  for (int base_slot = 0; base_slot < CS; base_slot += blockDim.x) {
+    int slot = base_slot + int(threadIdx.x);
+    int c = slot >> 3;
+    int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+    bool live = p >= 0;
+    float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (__ballot(live) != 0) {
+      f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
+      m3 Pk = mzero();
+      uint2 fidx = make_uint2(0u, 0u);
+      if (live) {
+        vp = ld3(S.v, p);
+        imp = ld3(S.impulses, p);
+        Pk = ld9(S.stress, p);
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int k = 0; k < 7; ++k) acc[a][k] = 0.0f;
-    for (int i = lo; i < hi; ++i) {
-      int64_t p = S.perm[i];
-      f3 vp = ld3(S.v, p), imp = ld3(S.impulses, p);
-      m3 Pk = ld9(S.stress, p);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
-      Stencil st;
-      stencil_full(ld3(S.local, p), inv_h, st);
-      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-#pragma unroll
+        for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
+        lp = ld3(S.local, p);
+        fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+      }
+      int wb = cell_window_base(P, w, c < cpb ? c : 0);
+#pragma unroll 2
       for (int a = 0; a < 8; ++a) {
         int f = field_of(fidx, a);
-        if (i == lo) f0[a] = f;
-        float wt = st.w[a];
+        float wt;
+        f3 ga;
+        stencil_corner(lp, inv_h, a, wt, ga);
+        if (!live) wt = 0.0f;
         float mw = mass * wt;
-        f3 jm = wt * imp - mvec(Pk, st.g[a]);
+        f3 jm = wt * imp - mvec(Pk, ga);
         float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
-        if (f == f0[a]) {
-#pragma unroll
-          for (int k = 0; k < 7; ++k) acc[a][k] += cv[k];
-        } else {
+        bool own = live && f != 0;
+        if (own) {
           int idx = f * w.W + wb + window_corner(w, a);
 #pragma unroll
-          for (int k = 0; k < 7; ++k) atomicAdd(lds + k * FW + idx, cv[k]);
+          for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          float t = cell_sum(own ? 0.0f : cv[k]);
+          if ((lane & 7) == a) mine[k] = t;
         }
       }
     }
+    if (slot < CS) {
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      int idx = f0[a] * w.W + wb + window_corner(w, a);
-#pragma unroll
-      for (int k = 0; k < 7; ++k) atomicAdd(lds + k * FW + idx, acc[a][k]);
+      for (int k = 0; k < 7; ++k) cc[k * CS + slot] = mine[k];
     }
   }
   __syncthreads();
   float* out = S.p2g_partial + int64_t(b) * FW * 7;
-  for (int g = threadIdx.x; g < Fb * w.W * 7; g += blockDim.x) {
-    int k = g % 7, fw = g / 7;
-    out[g] = lds[k * FW + fw];
+  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
+    float sum[7];
+    if (i < w.W) {
+      gather_cells<7>(P, w, cc, CS, i, sum);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) sum[k] = 0.0f;
+    }
+    if (multi) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) sum[k] += win[k * FW + i];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out[int64_t(i) * 7 + k] = sum[k];
   }
 }
 
@@ -1019,6 +964,7 @@ __device__ __forceinline__ void node_coords(const bm_params& P, int64_t node, in
 // Compatible-field exchange of one node (Eq. 8, material.py:252-285) on the gathered P2G
 // sums: x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b; then
 // old_v = p / m, new_v = (p + j) / m.
+
 __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S) {
   int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= P.node_count) return;
@@ -1072,11 +1018,14 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   }
 }
 
-// g2p_remap (material.py:384-401) for one brick: grid velocities of the window are staged
-// in LDS; one thread per cell gathers each particle's 8 corners from there, updates it, and
-// re-scatters m w v per corner through registers (as k_p2g) into an LDS window that is
-// written to remap_partial.
-__global__ void __launch_bounds__(256) k_g2p(const bm_params P, bm_state S) {
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp);
+
+// g2p_remap (material.py:384-401) for one brick: grid velocities of the window are staged in
+// LDS, each particle lane gathers its 8 corners, updates v and x, and re-scatters m w v
+// through the same cell-corner table as k_p2g into remap_partial.  In a substep the updated
+// particle is also labelled (assign, contact.py:996-1039) and checked against the pair
+// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.
+__global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags) {
   extern __shared__ float lds[];
   __shared__ int fmax_shared;
   int b = blockIdx.x;
@@ -1084,8 +1033,12 @@ __global__ void __launch_bounds__(256) k_g2p(const bm_params P, bm_state S) {
   int F = int(P.fields);
   int Fb = window_fields(P, S, w, &fmax_shared);
   int FW = F * w.W;
-  float* vel = lds;            // [6][F][W]: old_v xyz, new_v xyz
-  float* acc_l = lds + 6 * FW; // [3][F][W]
+  int cpb = cells_per_brick(P);
+  int CS = 8 * cpb;
+  float* vel = lds;             // [6][F][W]: old_v xyz, new_v xyz
+  float* win = lds + 6 * FW;    // [3][F][W]
+  float* cc = lds + 9 * FW;     // [3][CS]
+  bool multi = Fb > 1;
   int64_t N = P.node_count;
   for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
     int f = i / w.W, wi = i - f * w.W;
@@ -1101,69 +1054,91 @@ __global__ void __launch_bounds__(256) k_g2p(const bm_params P, bm_state S) {
     vel[3 * FW + i] = nv.x;
     vel[4 * FW + i] = nv.y;
     vel[5 * FW + i] = nv.z;
-    acc_l[i] = 0.0f;
-    acc_l[FW + i] = 0.0f;
-    acc_l[2 * FW + i] = 0.0f;
+    if (multi) {
+      win[i] = 0.0f;
+      win[FW + i] = 0.0f;
+      win[2 * FW + i] = 0.0f;
+    }
   }
   __syncthreads();
   float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
-  int cpb = cells_per_brick(P);
-  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
-    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
-    if (lo == hi) continue;
-    int wb = cell_window_base(P, w, c);
-    float acc[8][3];
-    int f0[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) acc[a][0] = acc[a][1] = acc[a][2] = 0.0f;
-    for (int i = lo; i < hi; ++i) {
-      int64_t p = S.perm[i];
+  float half_skin = 0.5f * float(P.pair_skin);
+  bool stale = false;
+  int lane = int(threadIdx.x & 63);
+  for (int base_slot = 0; base_slot < CS; base_slot += blockDim.x) {
+    int slot = base_slot + int(threadIdx.x);
+    int c = slot >> 3;
+    int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+    bool live = p >= 0;
+    float mine[3] = {0.f, 0.f, 0.f};
+    if (__ballot(live) != 0) {
       float wt[8];
-      stencil_weights(ld3(S.local, p), wt);
-      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      int fa[8];
-      f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+      uint2 fidx = make_uint2(0u, 0u);
+      if (live) {
+        stencil_weights(ld3(S.local, p), wt);
+        fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+      } else {
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        fa[a] = field_of(fidx, a);
-        if (i == lo) f0[a] = fa[a];
-        int idx = fa[a] * w.W + wb + window_corner(w, a);
-        f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
-        f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
-        pic = pic + wt[a] * nv;
-        dv = dv + wt[a] * (nv - o);
+        for (int a = 0; a < 8; ++a) wt[a] = 0.0f;
       }
-      f3 vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
-      st3(S.v, p, vp);
-      st3(S.x, p, ld3(S.x, p) + dt * pic);
+      int wb = cell_window_base(P, w, c < cpb ? c : 0);
+      f3 vp = f3{0.f, 0.f, 0.f};
+      if (live) {
+        f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+          f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
+          f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
+          pic = pic + wt[a] * nv;
+          dv = dv + wt[a] * (nv - o);
+        }
+        vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
+        f3 xp = ld3(S.x, p) + dt * pic;
+        st3(S.v, p, vp);
+        st3(S.x, p, xp);
+        if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
+        if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > half_skin) stale = true;
+      }
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
+        int f = field_of(fidx, a);
         f3 r = (mass * wt[a]) * vp;
-        if (fa[a] == f0[a]) {
-          acc[a][0] += r.x;
-          acc[a][1] += r.y;
-          acc[a][2] += r.z;
-        } else {
-          int idx = fa[a] * w.W + wb + window_corner(w, a);
-          atomicAdd(acc_l + idx, r.x);
-          atomicAdd(acc_l + FW + idx, r.y);
-          atomicAdd(acc_l + 2 * FW + idx, r.z);
+        bool own = live && f != 0;
+        if (own) {
+          int idx = f * w.W + wb + window_corner(w, a);
+          atomicAdd(win + idx, r.x);
+          atomicAdd(win + FW + idx, r.y);
+          atomicAdd(win + 2 * FW + idx, r.z);
+        }
+        float tx = cell_sum(own ? 0.0f : r.x), ty = cell_sum(own ? 0.0f : r.y), tz = cell_sum(own ? 0.0f : r.z);
+        if ((lane & 7) == a) {
+          mine[0] = tx;
+          mine[1] = ty;
+          mine[2] = tz;
         }
       }
     }
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      int idx = f0[a] * w.W + wb + window_corner(w, a);
-      atomicAdd(acc_l + idx, acc[a][0]);
-      atomicAdd(acc_l + FW + idx, acc[a][1]);
-      atomicAdd(acc_l + 2 * FW + idx, acc[a][2]);
+    if (slot < CS) {
+      cc[slot] = mine[0];
+      cc[CS + slot] = mine[1];
+      cc[2 * CS + slot] = mine[2];
     }
   }
+  if (stale) S.pair_dirty[0] = 1;
   __syncthreads();
   float* out = S.remap_partial + int64_t(b) * FW * 3;
-  for (int g = threadIdx.x; g < Fb * w.W * 3; g += blockDim.x) {
-    int k = g % 3, fw = g / 3;
-    out[g] = acc_l[k * FW + fw];
+  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
+    float sum[3] = {0.f, 0.f, 0.f};
+    if (i < w.W) gather_cells<3>(P, w, cc, CS, i, sum);
+    if (multi) {
+      sum[0] += win[i];
+      sum[1] += win[FW + i];
+      sum[2] += win[2 * FW + i];
+    }
+    out[int64_t(i) * 3] = sum[0];
+    out[int64_t(i) * 3 + 1] = sum[1];
+    out[int64_t(i) * 3 + 2] = sum[2];
   }
 }
 
@@ -1270,13 +1245,9 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp);
 }
 
-__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit);
-
-// update_F_plastic (material.py:445-475) for one brick, with the field velocities of the
-// window staged in LDS.  In a substep it also labels sides (assign, contact.py:996-1039) and
-// flags the pair list of the next step as stale (check_pair_list, contact.py:716-735) from
-// the same particle pass.
-__global__ void __launch_bounds__(256) k_update_F(const bm_params P, bm_state S, int64_t digit, int flags) {
+// update_F_plastic (material.py:445-475) for one brick, one lane per particle slot, with the
+// field velocities of the window staged in LDS.
+__global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S) {
   extern __shared__ float lds[];
   __shared__ int fmax_shared;
   int b = blockIdx.x;
@@ -1296,41 +1267,30 @@ __global__ void __launch_bounds__(256) k_update_F(const bm_params P, bm_state S,
   }
   __syncthreads();
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  float half_skin = 0.5f * float(P.pair_skin);
-  bool stale = false;
   int cpb = cells_per_brick(P);
-  for (int c = threadIdx.x; c < cpb; c += blockDim.x) {
-    int lo = S.cell_start[int64_t(b) * cpb + c], hi = S.cell_start[int64_t(b) * cpb + c + 1];
-    int wb = cell_window_base(P, w, c);
-    for (int i = lo; i < hi; ++i) {
-      int64_t p = S.perm[i];
-      Stencil st;
-      stencil_full(ld3(S.local, p), inv_h, st);
-      uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      m3 rate = mzero();
+  for (int slot = threadIdx.x; slot < 8 * cpb; slot += blockDim.x) {
+    int64_t p = S.perm[(int64_t(b) * cpb) * 8 + slot];
+    if (p < 0) continue;
+    int wb = cell_window_base(P, w, slot >> 3);
+    Stencil st;
+    stencil_full(ld3(S.local, p), inv_h, st);
+    uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+    m3 rate = mzero();
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
-        f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
-        f3 g = st.g[a];
-        rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
-        rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
-        rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
-      }
-      m3 Fm = ld9(S.F, p);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
-      st9(S.F, p, Fm);
-      st9(S.stress, p, stress_of(P, S, p, Fm));
-      if (flags & 1) assign_dev(P, S, p, digit);
-      if (flags & 2) {
-        f3 xp = ld3(S.x, p);
-        if (len(xp - ld3(S.built_x, p)) + dt * len(ld3(S.v, p)) > half_skin || S.codes[p] != S.built_codes[p])
-          stale = true;
-      }
+    for (int a = 0; a < 8; ++a) {
+      int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+      f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
+      f3 g = st.g[a];
+      rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
+      rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
+      rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
     }
+    m3 Fm = ld9(S.F, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
+    st9(S.F, p, Fm);
+    st9(S.stress, p, stress_of(P, S, p, Fm));
   }
-  if (stale) S.pair_dirty[0] = 1;
 }
 
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
@@ -1509,19 +1469,18 @@ __global__ void k_accumulate_reaction(bm_state S) { accumulate_reaction_dev(S); 
 
 // assign (contact.py:996-1039): side digit for a particle behind the edge band or pinched
 // on the board under the edge.
-__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit) {
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp) {
   int64_t code = S.codes[p];
   if ((code / digit) % 3 != 0) return;
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
   quat q = ldq(S.tool_rot);
   f3 pos = ld3(S.tool_pos, 0);
-  f3 xp = ld3(S.x, p);
   f3 edge = pos + rotate(q, f3{0.f, 0.f, -hf.z});
   f3 local = rotate_inv(q, xp - edge);
   float front = -local.z;
   f3 edge_v = ld3(S.tool_vel, 0) + cross(ld3(S.tool_omega, 0), edge - pos);
   f3 d = rotate(q, f3{0.f, 0.f, -1.f});
-  float advance = dot(edge_v - ld3(S.v, p), d);
+  float advance = dot(edge_v - vp, d);
   float spacing = float(P.spacing), r = float(P.radius), dt = float(P.dt);
   float back = 2.0f * spacing + fmaxf(advance * dt, 0.0f);
   bool behind = front >= -back && front <= 0.0f;
@@ -1532,14 +1491,16 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   if (fabsf(local.x) <= 6.0f * spacing && fabsf(local.y) <= hf.y && (behind || pinched)) {
     int64_t side = local.x < 0.0f ? 1 : 2;
     S.codes[p] = code + digit * side;
+    // A new side code changes the grid fields and the pair candidates.
     S.topology_dirty[0] = 1;
+    S.pair_dirty[0] = 1;
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n || cutting == 0) return;
-  assign_dev(P, S, p, digit);
+  assign_dev(P, S, p, digit, ld3(S.x, p), ld3(S.v, p));
 }
 
 // ------------------------------------------------------------------------ port kernels
@@ -1698,66 +1659,61 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   int nb = blocks_for(p->n);
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
                      int(kAccumulate), 0);
-  hipLaunchKernelGGL(k_rigid_finish, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
+  hipLaunchKernelGGL(k_prep, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
                      ToolTarget{nullptr, nullptr, 0}, 0);
   return launch_status();
 }
 
 namespace {
 bool pair_args_ok(const bm_params* p, const bm_state* s) {
-  return state_ok(p, s) && s->pair_keys && s->sorted_x && p->pair_capacity > 0 &&
-         (p->pair_capacity & (p->pair_capacity - 1)) == 0 && p->list_buckets > 0 &&
-         (p->list_buckets & (p->list_buckets - 1)) == 0 && p->list_buckets <= int64_t(4096) * 4096;
-}
-// Rebuild of the cell list and partner lists (predicated on pair_dirty inside), then the
-// listed contacts.
-void launch_pair_list(const bm_params* p, const bm_state* s, int old_p, hipStream_t st) {
-  int64_t B = p->list_buckets;
-  int nscan = int((B + 4095) / 4096);
-  hipLaunchKernelGGL(k_scan_partial, dim3(nscan), dim3(1024), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, *p, *s, nscan);
-  hipLaunchKernelGGL(k_scan_final, dim3(nscan), dim3(1024), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_list_scatter, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_list_build, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_pairs_listed, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
+  return state_ok(p, s) && s->pair_keys && s->pair_history && s->pair_used && s->pair_count && s->pair_dirty &&
+         s->pair_builds && s->list_reach && s->built_x && s->built_codes && s->partners && s->partner_count &&
+         s->list_heads && s->list_links && p->pair_capacity > 0 && (p->pair_capacity & (p->pair_capacity - 1)) == 0 &&
+         p->list_buckets > 0 && (p->list_buckets & (p->list_buckets - 1)) == 0 &&
+         p->list_buckets <= int64_t(4096) * 4096 && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
 }
 int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
 int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
-bool brick_args_ok(const bm_params* p, const bm_state* s, int lds_floats_per_fw) {
-  if (!state_ok(p, s) || !s->perm || !s->cell_start || !s->p2g_partial || !s->remap_partial) return false;
+// LDS of a brick kernel: floats_per_fw floats per (field slot, window node) plus
+// floats_per_corner per (cell, corner) of the brick.
+size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
+  int64_t cells = p->brick[0] * p->brick[1] * p->brick[2];
+  return size_t(floats_per_fw) * size_t(p->fields) * size_t(window_nodes(p)) * sizeof(float) +
+         size_t(floats_per_corner) * size_t(8 * cells) * sizeof(float);
+}
+bool brick_args_ok(const bm_params* p, const bm_state* s) {
+  if (!state_ok(p, s) || !s->perm || !s->p2g_partial || !s->remap_partial || !s->field_idx || !s->node_nf) return false;
   if (p->brick[0] < 1 || p->brick[1] < 1 || p->brick[2] < 1 || brick_count(p) < 1) return false;
   if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
       p->bricks[2] * p->brick[2] < p->dims[2] - 1)
     return false;
-  return int64_t(lds_floats_per_fw) * p->fields * window_nodes(p) * 4 <= 65536;
+  return lds_bytes(p, 7, 7) <= 65536 && lds_bytes(p, 9, 3) <= 65536;
 }
-// Threads of a brick workgroup: one per cell, rounded up to whole waves, at most 256.
+// Threads of a brick workgroup: one per particle slot (8 per cell), whole waves, at most 1024
+// (larger bricks loop over their slots).
 int brick_threads(const bm_params* p) {
-  int64_t cells = p->brick[0] * p->brick[1] * p->brick[2];
-  int64_t t = ((cells + 63) / 64) * 64;
-  return int(t < 64 ? 64 : (t > 256 ? 256 : t));
-}
-size_t lds_bytes(const bm_params* p, int floats_per_fw) {
-  return size_t(floats_per_fw) * size_t(p->fields) * size_t(window_nodes(p)) * sizeof(float);
+  int64_t slots = 8 * p->brick[0] * p->brick[1] * p->brick[2];
+  int64_t t = ((slots + 63) / 64) * 64;
+  return int(t > 1024 ? 1024 : t);
 }
 void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
   int64_t total = p->fields * p->node_count;
   hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
   if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_topo_sort, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
-  if (p->n > 0) hipLaunchKernelGGL(k_topo_index, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
 }
-// P2G .. update_F with the brick kernels; update_flags as k_update_F.
-void launch_transfers(const bm_params* p, const bm_state* s, int clear_topology, int64_t digit, int update_flags,
+// P2G .. update_F with the brick kernels.  p2g_flags: reset topology_dirty (1) and
+// pair_dirty (2) once every reader of this step is done; g2p_flags: assign (1) and the
+// pair-list staleness check (2) on the moved particles.
+void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
                       hipStream_t st) {
   int nb = int(brick_count(p)), tb = brick_threads(p);
-  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7), st, *p, *s, clear_topology);
+  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags);
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
-  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9), st, *p, *s);
+  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3), st, *p, *s, digit, update_flags);
+  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3, 0), st, *p, *s);
 }
 }  // namespace
 
@@ -1768,10 +1724,11 @@ int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void
   int old_p = parity & 1, new_p = 1 - old_p;
   hipLaunchKernelGGL(k_check_pair_list, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_clear_pair_slots, dim3(256), dim3(kBlock), 0, st, *s, p->pair_capacity, new_p);
-  hipLaunchKernelGGL(k_zero_pair_count, dim3(1), dim3(1), 0, st, *s, new_p);
-  hipLaunchKernelGGL(k_list_reset, dim3(blocks_for(p->list_buckets + 1, 1024)), dim3(1024), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_list_count, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
-  launch_pair_list(p, s, old_p, st);
+  // Binning tags the bucket heads with the build count after this build's increment.
+  hipLaunchKernelGGL(k_list_bin, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_list_reach, dim3(1), dim3(1), 0, st, *p, *s, new_p);
+  hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
+  hipLaunchKernelGGL(k_pairs_done, dim3(1), dim3(1), 0, st, *s);
   return launch_status();
 }
 
@@ -1785,42 +1742,42 @@ int bm_register(const bm_params* p, const bm_state* s, void* stream) {
 }
 
 int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9)) return -1;
-  hipLaunchKernelGGL(k_p2g, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 7), (hipStream_t)stream,
+  if (!brick_args_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_p2g, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 7, 7), (hipStream_t)stream,
                      *p, *s, 0);
   return launch_status();
 }
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9)) return -1;
+  if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9)) return -1;
-  hipLaunchKernelGGL(k_g2p, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 9), (hipStream_t)stream,
-                     *p, *s);
+  if (!brick_args_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_g2p, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 9, 3), (hipStream_t)stream,
+                     *p, *s, int64_t(1), 0);
   return launch_status();
 }
 
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9)) return -1;
+  if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9)) return -1;
-  hipLaunchKernelGGL(k_update_F, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 3),
-                     (hipStream_t)stream, *p, *s, int64_t(1), 0);
+  if (!brick_args_ok(p, s)) return -1;
+  hipLaunchKernelGGL(k_update_F, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 0),
+                     (hipStream_t)stream, *p, *s);
   return launch_status();
 }
 
 int bm_material_step(const bm_params* p, const bm_state* s, void* stream) {
-  if (!brick_args_ok(p, s, 9) || !s->field_code || !s->node_nf || !s->field_idx || !s->transfer_codes) return -1;
+  if (!brick_args_ok(p, s) || !s->field_code || !s->node_nf || !s->field_idx || !s->transfer_codes) return -1;
   hipStream_t st = (hipStream_t)stream;
   launch_register(p, s, st);
   launch_transfers(p, s, 1, 1, 0, st);
@@ -1857,16 +1814,18 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
 int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
                int32_t parity, int32_t accumulate, const float* targets, const float* rotations, int32_t index,
                void* stream) {
-  if (!brick_args_ok(p, s, 9) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
+  if (!brick_args_ok(p, s) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
   if (p->rigid_blocks < blocks_for(p->n)) return -2;
   bool external = p->tool_model < 0;
   if (!external && (!targets || !rotations || index < 0)) return -1;
   if (p->n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int old_p = parity & 1, new_p = 1 - old_p;
-  // 1. contacts with the knife and the board; the last block applies the step sums and moves
-  //    the knife (its update reads only this step's rigid reaction, solver.py:824-864).
-  int mode = kOverwrite | kPairPrep;
+  // 1. contacts with the knife and the board, plus the step's preparation: empty the new
+  //    pair table's slots, bin particles for a stale pair list, clear a stale field table.
+  // 2. block 0 reduces the contact sums and moves the knife (its update reads only this
+  //    step's rigid reaction, solver.py:824-864); the other blocks re-register the fields.
+  int mode = kOverwrite | kPairPrep | kTopologyPrep;
   if (external) {
     // With a held blade the grid constraint adds to the reaction first (finish runs last).
     if (!p->blade_hold) mode |= kFinishExternal;
@@ -1874,14 +1833,14 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
     mode |= kToolStep | (accumulate ? kAccumulateInterval : 0);
   }
   int nb = blocks_for(p->n);
+  int insert_blocks = nb < 2048 ? nb : 2048;
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
-  hipLaunchKernelGGL(k_rigid_finish, dim3(1), dim3(kBlock), 0, st, *p, *s, nb, mode,
+  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks), dim3(kBlock), 0, st, *p, *s, nb, mode,
                      ToolTarget{targets, rotations, int(index)}, new_p);
-  // 2. particle pairs across cuts
-  launch_pair_list(p, s, old_p, st);
-  // 3. topology when labels changed, transfers, F update + side labels + list staleness
-  launch_register(p, s, st);
-  launch_transfers(p, s, 1, digit, (cutting ? 1 : 0) | 2, st);
+  // 3. particle pairs across cuts (partner lists rebuilt first when stale)
+  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p);
+  // 4. transfers; G2P labels the moved particles and checks the pair list's skin
+  launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st);
   if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
 }

@@ -19,7 +19,7 @@
  *   particle vec3 arrays  [n][3], matrices [n][3][3] row-major, codes int64[n]
  *   local   [n][3] cell fractions of the reference position, base_node int32[n]
  *   field_idx uint8[n][8]  field slot of each stencil corner
- *   field_code int64[F][N] (-1 = empty slot), node_nf int32[N]
+ *   field_code int64[F][N] (-1 = empty slot; used slots form a prefix), node_nf int32[N]
  *   grid scalars [F][N], grid vectors [F][N][3]
  *   histories [2][n][3]  (row 0 board, row 1 blade friction history)
  *   W = (brick[0] + 1) (brick[1] + 1) (brick[2] + 1) window nodes per brick
@@ -78,32 +78,27 @@ typedef struct bm_state {
   float *reaction, *torque, *board, *work, *speed, *table;
   float *interval_impulse, *interval_angular;
   int32_t *errors, *topology_dirty;
-  /* pair contacts: ping-pong history tables [2][P] and the cell list */
+  /* pair contacts: ping-pong history tables [2][P] and the partner lists */
   int64_t *pair_keys;      /* [2][P], -1 empty */
   float *pair_history;     /* [2][P][3] */
   int32_t *pair_used;      /* [2][P] */
   int32_t *pair_count;     /* [2] */
   int32_t *pair_dirty;     /* [1] rebuild request */
-  int32_t *pair_builds;    /* [1] */
+  int32_t *pair_builds;    /* [1] rebuilds so far; tags the bucket lists of the current build */
   float *list_reach;       /* [1] */
-  float *built_x;          /* [n][3] */
+  float *built_x;          /* [n][3] positions at the last rebuild */
   int64_t *built_codes;    /* [n] */
   int32_t *partners;       /* [n][lanes] */
-  int32_t *partner_count;  /* [n] */
-  int32_t *bucket_count;   /* [B + 1] */
-  int32_t *bucket_start;   /* [B + 1] */
-  int32_t *scan_partial;   /* [ceil(B / 4096) + 1] */
-  int32_t *particle_bucket;/* [n] */
-  int32_t *particle_rank;  /* [n] */
-  float *sorted_x;         /* [n][4]: position and particle index (bits) in bucket order */
-  /* brick bins: bricks x-major, cells of a brick z-fastest; the particles of the c-th cell
-   * in this order are perm[cell_start[c] .. cell_start[c+1]) */
-  int32_t *perm;           /* [n] */
-  int32_t *cell_start;     /* [bricks * cells per brick + 1] */
+  int32_t *partner_count;  /* [n], -1 = more than lanes partners */
+  int64_t *list_heads;     /* [B]: (build tag << 32) | first particle of the bucket */
+  int32_t *list_links;     /* [n]: next particle of the same bucket, -1 at the end */
+  /* brick bins: bricks x-major, cells of a brick z-fastest; cell c holds the particles
+   * perm[8c .. 8c+8) in reference order, -1 for unused slots (at most 8 per cell) */
+  int32_t *perm;           /* [8 * bricks * cells per brick] */
   float *p2g_partial;      /* [bricks][F][W][7] window sums of mass, momentum, impulse */
   float *remap_partial;    /* [bricks][F][W][3] window sums of the re-scattered momentum */
   float *rigid_partial;    /* [rigid_blocks][12] per-block contact sums */
-  uint32_t *counters;      /* [4] last-block tickets (zero between launches) */
+  uint32_t *counters;      /* [4] reserved */
   /* held-blade grid constraint (blade_constraint.py), used when blade_hold is set */
   float *bc_side;          /* [n] face side of the previous step (0, +1, -1) */
   float *bc_normal;        /* [n][3] */

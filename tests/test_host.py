@@ -247,16 +247,22 @@ def test_brick_bins_cover_every_particle_window(shape, size, spacing, fields):
 
     ref, origin, cells = points(size, spacing, shape)
     base, local, base_cell = compact_stencil(ref, origin, spacing, cells)
-    brick, bricks, perm, start = bin_bricks(cells, base_cell, fields)
-    np.testing.assert_array_equal(np.sort(perm), np.arange(len(ref)))
-    assert start[0] == 0 and start[-1] == len(ref) and np.all(np.diff(start) >= 0)
+    brick, bricks, slots = bin_bricks(cells, base_cell, fields)
+    used = slots[slots >= 0]
+    np.testing.assert_array_equal(np.sort(used), np.arange(len(ref)))
     assert np.all(bricks * brick >= cells)
     assert 9 * fields * np.prod(brick + 1) * 4 <= 65536
     per_brick = int(np.prod(brick))
-    assert len(start) == int(np.prod(bricks)) * per_brick + 1
-    # the particles of slot c are exactly those of cell c of the brick order
-    slot = np.repeat(np.arange(len(start) - 1), np.diff(start))
-    nb, cl = slot // per_brick, slot % per_brick
+    assert len(slots) == 8 * int(np.prod(bricks)) * per_brick
+    cell_slots = slots.reshape(-1, 8)
+    # used slots are a prefix of each cell, in reference order
+    filled = cell_slots >= 0
+    assert np.all(filled[:, 1:] <= filled[:, :-1])
+    both = filled[:, 1:] & filled[:, :-1]
+    assert np.all(cell_slots[:, 1:][both] > cell_slots[:, :-1][both])
+    # the particles of slot group c are exactly those of cell c of the brick order
+    c = np.repeat(np.arange(len(cell_slots)), filled.sum(1))
+    nb, cl = c // per_brick, c % per_brick
     bz = nb % bricks[2]
     by = (nb // bricks[2]) % bricks[1]
     bx = nb // (bricks[1] * bricks[2])
@@ -264,7 +270,4 @@ def test_brick_bins_cover_every_particle_window(shape, size, spacing, fields):
     ly = (cl // brick[2]) % brick[1]
     lx = cl // (brick[1] * brick[2])
     cell = np.stack([bx * brick[0] + lx, by * brick[1] + ly, bz * brick[2] + lz], 1)
-    np.testing.assert_array_equal(base_cell[perm], cell)
-    # reference order inside a cell
-    same = slot[1:] == slot[:-1]
-    assert np.all(perm[1:][same] > perm[:-1][same])
+    np.testing.assert_array_equal(base_cell[cell_slots[filled]], cell)
