@@ -89,17 +89,17 @@ __device__ __forceinline__ float mdet(const m3& m) {
          m.a[2] * (m.a[3] * m.a[7] - m.a[4] * m.a[6]);
 }
 __device__ __forceinline__ m3 minv(const m3& m) {
-  float d = mdet(m);
+  float inv = 1.0f / mdet(m);
   m3 r;
-  r.a[0] = (m.a[4] * m.a[8] - m.a[5] * m.a[7]) / d;
-  r.a[1] = (m.a[2] * m.a[7] - m.a[1] * m.a[8]) / d;
-  r.a[2] = (m.a[1] * m.a[5] - m.a[2] * m.a[4]) / d;
-  r.a[3] = (m.a[5] * m.a[6] - m.a[3] * m.a[8]) / d;
-  r.a[4] = (m.a[0] * m.a[8] - m.a[2] * m.a[6]) / d;
-  r.a[5] = (m.a[2] * m.a[3] - m.a[0] * m.a[5]) / d;
-  r.a[6] = (m.a[3] * m.a[7] - m.a[4] * m.a[6]) / d;
-  r.a[7] = (m.a[1] * m.a[6] - m.a[0] * m.a[7]) / d;
-  r.a[8] = (m.a[0] * m.a[4] - m.a[1] * m.a[3]) / d;
+  r.a[0] = (m.a[4] * m.a[8] - m.a[5] * m.a[7]) * inv;
+  r.a[1] = (m.a[2] * m.a[7] - m.a[1] * m.a[8]) * inv;
+  r.a[2] = (m.a[1] * m.a[5] - m.a[2] * m.a[4]) * inv;
+  r.a[3] = (m.a[5] * m.a[6] - m.a[3] * m.a[8]) * inv;
+  r.a[4] = (m.a[0] * m.a[8] - m.a[2] * m.a[6]) * inv;
+  r.a[5] = (m.a[2] * m.a[3] - m.a[0] * m.a[5]) * inv;
+  r.a[6] = (m.a[3] * m.a[7] - m.a[4] * m.a[6]) * inv;
+  r.a[7] = (m.a[1] * m.a[6] - m.a[0] * m.a[7]) * inv;
+  r.a[8] = (m.a[0] * m.a[4] - m.a[1] * m.a[3]) * inv;
   return r;
 }
 __device__ __forceinline__ m3 mtranspose(const m3& m) {
@@ -142,9 +142,14 @@ __device__ __forceinline__ quat axis_angle(f3 axis, float angle) {
 // ---------------------------------------------------------------------------------------
 // Float32 3x3 SVD, A = U diag(s) V^T, U and V proper rotations, |s0| >= |s1| >= |s2| and
 // only s2 may be negative.  Structure of McAdams et al. (2011): symmetric eigenanalysis of
-// A^T A by cyclic Jacobi (here with exact rotation angles and a fixed sweep count instead
-// of the paper's approximate quaternion rotations), column sort, then QR of A V by three
-// Givens rotations.
+// A^T A by cyclic Jacobi (here with exact rotation angles instead of the paper's
+// approximate quaternion rotations, and sweeps until the off-diagonal is below float
+// resolution, at most 6), column sort, then QR of A V by three Givens rotations.
+// Reciprocals and square roots use the hardware's single-instruction forms (1 ulp).
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
   // S = A^T A (symmetric)
   float s00 = A.a[0] * A.a[0] + A.a[3] * A.a[3] + A.a[6] * A.a[6];
@@ -155,10 +160,13 @@ __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
   float s22 = A.a[2] * A.a[2] + A.a[5] * A.a[5] + A.a[8] * A.a[8];
   float W[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
   auto jacobi = [&](float& app, float& aqq, float& apq, float& apr, float& aqr, int p, int q) {
-    if (apq == 0.0f) return;
-    float tau = (aqq - app) / (2.0f * apq);
-    float t = copysignf(1.0f, tau) / (fabsf(tau) + sqrtf(1.0f + tau * tau));
-    float c = 1.0f / sqrtf(1.0f + t * t), sn = t * c;
+    // A rotation smaller than this changes nothing at float resolution (or spins inside a
+    // degenerate eigenspace, which leaves U diag(s) V^T unchanged).
+    if (!(fabsf(apq) > 1.0e-30f)) return;
+    float tau = (aqq - app) * fast_rcp(2.0f * apq);
+    float t = copysignf(fast_rcp(fabsf(tau) + fast_sqrt(1.0f + tau * tau)), tau);
+    if (!(fabsf(tau) < 1.0e18f)) t = 0.5f * copysignf(fast_rcp(fabsf(tau)), tau);  // tau^2 overflow
+    float c = fast_rsqrt(1.0f + t * t), sn = t * c;
     app = app - t * apq;
     aqq = aqq + t * apq;
     apq = 0.0f;
@@ -174,6 +182,9 @@ __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
   };
 #pragma unroll 1
   for (int sweep = 0; sweep < 6; ++sweep) {
+    float off = s01 * s01 + s02 * s02 + s12 * s12;
+    float diag = s00 * s00 + s11 * s11 + s22 * s22;
+    if (off <= 1.0e-16f * diag) break;
     jacobi(s00, s11, s01, s02, s12, 0, 1);
     jacobi(s00, s22, s02, s01, s12, 0, 2);
     jacobi(s11, s22, s12, s01, s02, 1, 2);
@@ -209,11 +220,12 @@ __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
   float Q[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
   auto givens = [&](int i, int k, int col) {
     float a = B[3 * i + col], b = B[3 * k + col];
-    float r = sqrtf(a * a + b * b);
+    float r2 = a * a + b * b;
     float c = 1.0f, sn = 0.0f;
-    if (r > 0.0f) {
-      c = a / r;
-      sn = b / r;
+    if (r2 > 1.0e-36f) {
+      float inv = fast_rsqrt(r2);
+      c = a * inv;
+      sn = b * inv;
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -253,10 +265,12 @@ __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
 // First Piola stress of fixed corotated elasticity from the SVD of f = F Fp^-1,
 // P = U diag(2 mu (s - 1) + lam (J - 1) J / s) V^T Fp^-T  (material.py:78-91).
 __device__ __forceinline__ m3 corotated(const m3& U, f3 s, const m3& V, const m3& pinv, float mu, float lam) {
+  // J / s_i is the product of the other two singular values.
   float J = s.x * s.y * s.z;
-  float d0 = 2.0f * mu * (s.x - 1.0f) + lam * (J - 1.0f) * J / s.x;
-  float d1 = 2.0f * mu * (s.y - 1.0f) + lam * (J - 1.0f) * J / s.y;
-  float d2 = 2.0f * mu * (s.z - 1.0f) + lam * (J - 1.0f) * J / s.z;
+  float lj = lam * (J - 1.0f);
+  float d0 = 2.0f * mu * (s.x - 1.0f) + lj * (s.y * s.z);
+  float d1 = 2.0f * mu * (s.y - 1.0f) + lj * (s.x * s.z);
+  float d2 = 2.0f * mu * (s.z - 1.0f) + lj * (s.x * s.y);
   m3 UD;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -266,6 +280,10 @@ __device__ __forceinline__ m3 corotated(const m3& U, f3 s, const m3& V, const m3
   }
   m3 P = mmul_t(UD, V);     // U D V^T
   return mmul_t(P, pinv);   // (U D V^T) Fp^-T
+}
+__device__ __forceinline__ bool is_identity(const m3& m) {
+  return m.a[0] == 1.0f && m.a[4] == 1.0f && m.a[8] == 1.0f && m.a[1] == 0.0f && m.a[2] == 0.0f && m.a[3] == 0.0f &&
+         m.a[5] == 0.0f && m.a[6] == 0.0f && m.a[7] == 0.0f;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -1221,8 +1221,10 @@ __device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t
 
 // stress_at (material.py:94-112)
 __device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const m3& F, const m3& Fp) {
-  m3 pinv = minv(Fp);
-  m3 f = mmul(F, pinv);
+  // Fp = I (no plastic flow yet, and always without plasticity): f = F, Fp^-T = I exactly.
+  bool plain = is_identity(Fp);
+  m3 pinv = plain ? Fp : minv(Fp);
+  m3 f = plain ? F : mmul(F, pinv);
   float J = mdet(f);
   if (!(J > 0.0f) || !isfinite(J)) {
     atomicAdd(S.errors + 1, 1);
