@@ -71,7 +71,7 @@ def run_gpu(args):
     from blademaster_hip import _lib
 
     cfg, q, edge_z0, hz, E = workload()
-    s = Solver(cfg)
+    s = Solver(cfg, brick_z=args.brick_z)
     port = ExternalToolPort(s)
     interval = INTERVAL_SUBSTEPS * cfg.dt
     vel = np.array([0.0, 0.0, -0.1])
@@ -182,12 +182,25 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_1M_block.json"))
+    ap.add_argument("--brick-z", type=int, default=None, help="brick height of the transfer kernels (cells)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB three times (the calibration kernel of "
+                    "HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path is single-device, see DESIGN.md (sharding)", file=sys.stderr)
         sys.exit(2)
     r = run_gpu(args)
     s, cfg = r["solver"], r["cfg"]
+    if args.copy_mib > 0:
+        import torch
+
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not args.no_cpu:
         _, q, edge_z0, hz, _ = workload()
@@ -222,6 +235,7 @@ def main():
             "young_modulus": r["E"],
             "poisson_ratio": cfg.poisson_ratio,
             "interval_substeps": INTERVAL_SUBSTEPS,
+            "brick": [int(v) for v in s.brick],
         },
         "hbm_peak_frac": SUBSTEP_BYTES * r["rate"] / (peak * 1e9),
         "sim_per_wall": r["steps"] * cfg.dt / r["wall"],

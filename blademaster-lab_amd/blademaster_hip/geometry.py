@@ -122,13 +122,15 @@ def compact_stencil(reference, origin, spacing, cells):
     return base_node.astype(np.int32), local.astype(np.float32), base.astype(np.int32)
 
 
-# Cells per brick along x and y of the particle bins; z bricks are as tall as the LDS window
-# of the G2P kernel allows (at most 32 cells).
+# Cells per brick along x and y of the particle bins, and the default brick height along z
+# (bricks are at most as tall as the kernels' LDS windows allow, and at most 1024 particle
+# slots).
 BRICK_XY = 2
+BRICK_Z = 16
 LDS_BYTES = 65536
 
 
-def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY):
+def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
     """Particle bins of the brick kernels.
 
     Returns (brick cells per axis, bricks per axis, slots): bricks are numbered x-major, the
@@ -146,7 +148,9 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY):
         return max(7 * window + 7 * corners, 9 * window + 3 * corners) * 4 <= LDS_BYTES
 
     bx = by = brick_xy
-    bz = int(min(cells[2], 32))
+    bz = int(max(1, min(cells[2], brick_z, 1024 // (8 * bx * by))))
+    # Same-height bricks along z: ceil(cells / bz) bricks, each as low as that count allows.
+    bz = int(-(-cells[2] // -(-cells[2] // bz))) if cells[2] > 0 else bz
     while bz > 1 and not fits(bx, by, bz):
         bz -= 1
     while not fits(bx, by, bz) and (bx > 1 or by > 1):

@@ -56,8 +56,12 @@ class BladeConstraint:
 
 
 class Solver:
-    def __init__(self, config=Config(), device="cuda:0"):
+    """brick_z: height in cells of the particle bricks of the transfer kernels (a tuning knob;
+    results do not depend on it beyond float summation order)."""
+
+    def __init__(self, config=Config(), device="cuda:0", brick_z=None):
         self.config = cfg = config
+        self._brick_z = brick_z
         self.device = _Device(device)
         if not self.device.is_cuda or not torch.cuda.is_available():
             raise ValueError("CUDA device required")
@@ -302,7 +306,9 @@ class Solver:
         self._t["pair_dirty"].fill_(1)
 
     def _bin_bricks(self, base_cell):
-        self.brick, self.bricks, self.brick_slots = bin_bricks(self.cells, base_cell, int(self.config.fields_per_node))
+        kw = {} if self._brick_z is None else dict(brick_z=int(self._brick_z))
+        self.brick, self.bricks, self.brick_slots = bin_bricks(self.cells, base_cell, int(self.config.fields_per_node),
+                                                               **kw)
 
     def launch(self, name, *args):
         """Launch one C-ABI stage by name with the solver's params/state (cf. solver.py:331)."""

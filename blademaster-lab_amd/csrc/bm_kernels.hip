@@ -762,18 +762,17 @@ __device__ __forceinline__ int64_t window_node(const bm_params& P, const Window&
   if (nx >= P.dims[0] || ny >= P.dims[1] || nz >= P.dims[2]) return -1;
   return (int64_t(nx) * P.dims[1] + ny) * P.dims[2] + nz;
 }
-// Largest used field count over the window's nodes (block-wide).
-__device__ int window_fields(const bm_params& P, const bm_state& S, const Window& w, int* shared_max) {
-  if (threadIdx.x == 0) *shared_max = 0;
-  __syncthreads();
+// Largest used field count over the window's nodes: each wave stores the maximum of its
+// lanes' counts in wave_nf[wave]; after a barrier, window_max reads the block's maximum.
+__device__ __forceinline__ void wave_max_store(int v, int* wave_nf) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) wave_nf[threadIdx.x >> 6] = v;
+}
+__device__ __forceinline__ int window_max(const int* wave_nf) {
   int m = 0;
-  for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
-    int64_t node = window_node(P, w, i);
-    if (node >= 0) m = max(m, S.node_nf[node]);
-  }
-  atomicMax(shared_max, m);
-  __syncthreads();
-  return *shared_max;
+  for (int k = 0; k < int(blockDim.x >> 6); ++k) m = max(m, wave_nf[k]);
+  return m;
 }
 
 // Window index of corner 0 of the c-th cell of a brick (cells z-fastest).
@@ -822,15 +821,15 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 }
 
 // p2g (material.py:127-152) for one brick: mass, momentum and stress + contact impulse.
-// One lane per particle slot, 8 slots per cell.  Per stencil corner the cell's lanes sum the
-// 7 values of its particles on field slot 0 (DPP reduction) and lane 8c + a stores the sums of
-// corner a in a per-(cell, corner) LDS table with plain stores; a particle on another field
-// slot (a cut through the node) adds its values to an LDS node window with atomics.  Each
-// window node then gathers its cells' corner sums in fixed order, adds the window, and the
-// brick's partial goes out with plain stores.
+// One lane per particle slot, 8 slots per cell (at most 1024 slots per brick).  Per stencil
+// corner the cell's lanes sum the 7 values of its particles on field slot 0 (DPP reduction)
+// and lane 8c + a stores the sums of corner a in a per-(cell, corner) LDS table with plain
+// stores; a particle on another field slot (a cut through the node) adds its values to an LDS
+// node window with atomics.  Each window node then gathers its cells' corner sums in fixed
+// order, adds the window, and the brick's partial goes out with plain stores.
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags) {
   extern __shared__ float lds[];
-  __shared__ int fmax_shared;
+  __shared__ int wave_nf[16];
   int b = blockIdx.x;
   if (flags && b == 0 && threadIdx.x == 0) {
     // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
@@ -839,69 +838,69 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   }
   Window w = window_of(P, b);
   int F = int(P.fields);
-  int Fb = window_fields(P, S, w, &fmax_shared);
   int FW = F * w.W;
   int cpb = cells_per_brick(P);
   int CS = 8 * cpb;
   float* win = lds;          // [7][F][W]
   float* cc = lds + 7 * FW;  // [7][CS]
-  bool multi = Fb > 1;
-  if (multi) {
+  int slot = int(threadIdx.x);
+  int c = slot >> 3;
+  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+  bool live = p >= 0;
+  int64_t my_node = int(threadIdx.x) < w.W ? window_node(P, w, int(threadIdx.x)) : -1;
+  int my_nf = my_node >= 0 ? S.node_nf[my_node] : 0;
+  if (F > 1) {
     for (int k = 0; k < 7; ++k)
-      for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) win[k * FW + i] = 0.0f;
+      for (int i = threadIdx.x; i < FW; i += blockDim.x) win[k * FW + i] = 0.0f;
     __syncthreads();
   }
   float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
   int lane = int(threadIdx.x & 63);
-  for (int base_slot = 0; base_slot < CS; base_slot += blockDim.x) {
-    int slot = base_slot + int(threadIdx.x);
-    int c = slot >> 3;
-    int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
-    bool live = p >= 0;
-    float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (__ballot(live) != 0) {
-      f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
-      m3 Pk = mzero();
-      uint2 fidx = make_uint2(0u, 0u);
-      if (live) {
-        vp = ld3(S.v, p);
-        imp = ld3(S.impulses, p);
-        Pk = ld9(S.stress, p);
+  float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (__ballot(live) != 0) {
+    f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
+    m3 Pk = mzero();
+    uint2 fidx = make_uint2(0u, 0u);
+    if (live) {
+      vp = ld3(S.v, p);
+      imp = ld3(S.impulses, p);
+      Pk = ld9(S.stress, p);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
-        lp = ld3(S.local, p);
-        fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      }
-      int wb = cell_window_base(P, w, c < cpb ? c : 0);
-#pragma unroll 2
-      for (int a = 0; a < 8; ++a) {
-        int f = field_of(fidx, a);
-        float wt;
-        f3 ga;
-        stencil_corner(lp, inv_h, a, wt, ga);
-        if (!live) wt = 0.0f;
-        float mw = mass * wt;
-        f3 jm = wt * imp - mvec(Pk, ga);
-        float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
-        bool own = live && f != 0;
-        if (own) {
-          int idx = f * w.W + wb + window_corner(w, a);
-#pragma unroll
-          for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-          float t = cell_sum(own ? 0.0f : cv[k]);
-          if ((lane & 7) == a) mine[k] = t;
-        }
-      }
+      for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
+      lp = ld3(S.local, p);
+      fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
     }
-    if (slot < CS) {
+    int wb = cell_window_base(P, w, c < cpb ? c : 0);
+#pragma unroll 2
+    for (int a = 0; a < 8; ++a) {
+      int f = field_of(fidx, a);
+      float wt;
+      f3 ga;
+      stencil_corner(lp, inv_h, a, wt, ga);
+      if (!live) wt = 0.0f;
+      float mw = mass * wt;
+      f3 jm = wt * imp - mvec(Pk, ga);
+      float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
+      bool own = live && f != 0;
+      if (own) {
+        int idx = f * w.W + wb + window_corner(w, a);
 #pragma unroll
-      for (int k = 0; k < 7; ++k) cc[k * CS + slot] = mine[k];
+        for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        float t = cell_sum(own ? 0.0f : cv[k]);
+        if ((lane & 7) == a) mine[k] = t;
+      }
     }
   }
+  if (slot < CS) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) cc[k * CS + slot] = mine[k];
+  }
+  wave_max_store(my_nf, wave_nf);
   __syncthreads();
+  int Fb = window_max(wave_nf);
   float* out = S.p2g_partial + int64_t(b) * FW * 7;
   for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
     float sum[7];
@@ -911,7 +910,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
 #pragma unroll
       for (int k = 0; k < 7; ++k) sum[k] = 0.0f;
     }
-    if (multi) {
+    if (F > 1) {
 #pragma unroll
       for (int k = 0; k < 7; ++k) sum[k] += win[k * FW + i];
     }
@@ -1024,118 +1023,121 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 // LDS, each particle lane gathers its 8 corners, updates v and x, and re-scatters m w v
 // through the same cell-corner table as k_p2g into remap_partial.  In a substep the updated
 // particle is also labelled (assign, contact.py:996-1039) and checked against the pair
-// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.
+// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.  The
+// particle loads are issued before the window staging so the two latencies overlap.
 __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags) {
   extern __shared__ float lds[];
-  __shared__ int fmax_shared;
+  __shared__ int wave_nf[16];
   int b = blockIdx.x;
   Window w = window_of(P, b);
   int F = int(P.fields);
-  int Fb = window_fields(P, S, w, &fmax_shared);
   int FW = F * w.W;
   int cpb = cells_per_brick(P);
   int CS = 8 * cpb;
   float* vel = lds;             // [6][F][W]: old_v xyz, new_v xyz
   float* win = lds + 6 * FW;    // [3][F][W]
   float* cc = lds + 9 * FW;     // [3][CS]
-  bool multi = Fb > 1;
   int64_t N = P.node_count;
-  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
-    int f = i / w.W, wi = i - f * w.W;
-    int64_t node = window_node(P, w, wi);
-    f3 o = f3{0.f, 0.f, 0.f}, nv = o;
-    if (node >= 0 && f < S.node_nf[node]) {
-      o = ld3(S.old_v, int64_t(f) * N + node);
-      nv = ld3(S.new_v, int64_t(f) * N + node);
-    }
-    vel[i] = o.x;
-    vel[FW + i] = o.y;
-    vel[2 * FW + i] = o.z;
-    vel[3 * FW + i] = nv.x;
-    vel[4 * FW + i] = nv.y;
-    vel[5 * FW + i] = nv.z;
-    if (multi) {
-      win[i] = 0.0f;
-      win[FW + i] = 0.0f;
-      win[2 * FW + i] = 0.0f;
+  int slot = int(threadIdx.x);
+  int c = slot >> 3;
+  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+  bool live = p >= 0;
+  f3 lp = f3{0.f, 0.f, 0.f}, xp = lp, vold = lp;
+  uint2 fidx = make_uint2(0u, 0u);
+  if (live) {
+    lp = ld3(S.local, p);
+    fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+    xp = ld3(S.x, p);
+    vold = ld3(S.v, p);
+  }
+  int my_nf = 0;
+  for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
+    int64_t node = window_node(P, w, i);
+    int nf = node >= 0 ? S.node_nf[node] : 0;
+    my_nf = max(my_nf, nf);
+    for (int f = 0; f < F; ++f) {
+      f3 o = f3{0.f, 0.f, 0.f}, nv = o;
+      if (f < nf) {
+        o = ld3(S.old_v, int64_t(f) * N + node);
+        nv = ld3(S.new_v, int64_t(f) * N + node);
+      }
+      int j = f * w.W + i;
+      vel[j] = o.x;
+      vel[FW + j] = o.y;
+      vel[2 * FW + j] = o.z;
+      vel[3 * FW + j] = nv.x;
+      vel[4 * FW + j] = nv.y;
+      vel[5 * FW + j] = nv.z;
+      win[j] = 0.0f;
+      win[FW + j] = 0.0f;
+      win[2 * FW + j] = 0.0f;
     }
   }
+  wave_max_store(my_nf, wave_nf);
   __syncthreads();
   float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
-  float half_skin = 0.5f * float(P.pair_skin);
-  bool stale = false;
   int lane = int(threadIdx.x & 63);
-  for (int base_slot = 0; base_slot < CS; base_slot += blockDim.x) {
-    int slot = base_slot + int(threadIdx.x);
-    int c = slot >> 3;
-    int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
-    bool live = p >= 0;
-    float mine[3] = {0.f, 0.f, 0.f};
-    if (__ballot(live) != 0) {
+  float mine[3] = {0.f, 0.f, 0.f};
+  if (__ballot(live) != 0) {
+    int wb = cell_window_base(P, w, c < cpb ? c : 0);
+    f3 vp = f3{0.f, 0.f, 0.f};
+    if (live) {
       float wt[8];
-      uint2 fidx = make_uint2(0u, 0u);
-      if (live) {
-        stencil_weights(ld3(S.local, p), wt);
-        fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      } else {
-#pragma unroll
-        for (int a = 0; a < 8; ++a) wt[a] = 0.0f;
-      }
-      int wb = cell_window_base(P, w, c < cpb ? c : 0);
-      f3 vp = f3{0.f, 0.f, 0.f};
-      if (live) {
-        f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
-          f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
-          f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
-          pic = pic + wt[a] * nv;
-          dv = dv + wt[a] * (nv - o);
-        }
-        vp = (1.0f - flip) * pic + flip * (ld3(S.v, p) + dv);
-        f3 xp = ld3(S.x, p) + dt * pic;
-        st3(S.v, p, vp);
-        st3(S.x, p, xp);
-        if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
-        if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > half_skin) stale = true;
-      }
+      stencil_weights(lp, wt);
+      f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
-        int f = field_of(fidx, a);
-        f3 r = (mass * wt[a]) * vp;
-        bool own = live && f != 0;
-        if (own) {
-          int idx = f * w.W + wb + window_corner(w, a);
-          atomicAdd(win + idx, r.x);
-          atomicAdd(win + FW + idx, r.y);
-          atomicAdd(win + 2 * FW + idx, r.z);
-        }
-        float tx = cell_sum(own ? 0.0f : r.x), ty = cell_sum(own ? 0.0f : r.y), tz = cell_sum(own ? 0.0f : r.z);
-        if ((lane & 7) == a) {
-          mine[0] = tx;
-          mine[1] = ty;
-          mine[2] = tz;
-        }
+        int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+        f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
+        f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
+        pic = pic + wt[a] * nv;
+        dv = dv + wt[a] * (nv - o);
+      }
+      vp = (1.0f - flip) * pic + flip * (vold + dv);
+      xp = xp + dt * pic;
+      st3(S.v, p, vp);
+      st3(S.x, p, xp);
+      if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
+      if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin))
+        S.pair_dirty[0] = 1;
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int f = field_of(fidx, a);
+      float wt;
+      f3 ga;
+      stencil_corner(lp, 0.0f, a, wt, ga);
+      if (!live) wt = 0.0f;
+      f3 r = (mass * wt) * vp;
+      bool own = live && f != 0;
+      if (own) {
+        int idx = f * w.W + wb + window_corner(w, a);
+        atomicAdd(win + idx, r.x);
+        atomicAdd(win + FW + idx, r.y);
+        atomicAdd(win + 2 * FW + idx, r.z);
+      }
+      float tx = cell_sum(own ? 0.0f : r.x), ty = cell_sum(own ? 0.0f : r.y), tz = cell_sum(own ? 0.0f : r.z);
+      if ((lane & 7) == a) {
+        mine[0] = tx;
+        mine[1] = ty;
+        mine[2] = tz;
       }
     }
-    if (slot < CS) {
-      cc[slot] = mine[0];
-      cc[CS + slot] = mine[1];
-      cc[2 * CS + slot] = mine[2];
-    }
   }
-  if (stale) S.pair_dirty[0] = 1;
+  if (slot < CS) {
+    cc[slot] = mine[0];
+    cc[CS + slot] = mine[1];
+    cc[2 * CS + slot] = mine[2];
+  }
   __syncthreads();
+  int Fb = window_max(wave_nf);
   float* out = S.remap_partial + int64_t(b) * FW * 3;
   for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
     float sum[3] = {0.f, 0.f, 0.f};
     if (i < w.W) gather_cells<3>(P, w, cc, CS, i, sum);
-    if (multi) {
-      sum[0] += win[i];
-      sum[1] += win[FW + i];
-      sum[2] += win[2 * FW + i];
-    }
+    sum[0] += win[i];
+    sum[1] += win[FW + i];
+    sum[2] += win[2 * FW + i];
     out[int64_t(i) * 3] = sum[0];
     out[int64_t(i) * 3 + 1] = sum[1];
     out[int64_t(i) * 3 + 2] = sum[2];
@@ -1236,8 +1238,7 @@ __device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const
   return corotated(U, sg, V, pinv, float(P.mu), float(P.lam));
 }
 
-__device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p, const m3& F) {
-  m3 Fp = ld9(S.Fp, p);
+__device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p, const m3& F, m3 Fp) {
   if (P.plastic) {
     m3 U, V;
     f3 sg;
@@ -1248,51 +1249,59 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 }
 
 // update_F_plastic (material.py:445-475) for one brick, one lane per particle slot, with the
-// field velocities of the window staged in LDS.
+// field velocities of the window staged in LDS; the particle loads are issued first so they
+// overlap the staging.
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S) {
   extern __shared__ float lds[];
-  __shared__ int fmax_shared;
   int b = blockIdx.x;
   Window w = window_of(P, b);
   int F = int(P.fields);
-  int Fb = window_fields(P, S, w, &fmax_shared);
   int FW = F * w.W;
   int64_t N = P.node_count;
-  for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
-    int f = i / w.W, wi = i - f * w.W;
-    int64_t node = window_node(P, w, wi);
-    f3 v = f3{0.f, 0.f, 0.f};
-    if (node >= 0 && f < S.node_nf[node]) v = ld3(S.grid_vel, int64_t(f) * N + node);
-    lds[i] = v.x;
-    lds[FW + i] = v.y;
-    lds[2 * FW + i] = v.z;
+  int CS = 8 * cells_per_brick(P);
+  int slot = int(threadIdx.x);
+  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+  f3 lp = f3{0.f, 0.f, 0.f};
+  uint2 fidx = make_uint2(0u, 0u);
+  m3 Fm = mzero(), Fp = mzero();
+  if (p >= 0) {
+    lp = ld3(S.local, p);
+    fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+    Fm = ld9(S.F, p);
+    Fp = ld9(S.Fp, p);
+  }
+  for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
+    int64_t node = window_node(P, w, i);
+    int nf = node >= 0 ? S.node_nf[node] : 0;
+    for (int f = 0; f < F; ++f) {
+      f3 v = f3{0.f, 0.f, 0.f};
+      if (f < nf) v = ld3(S.grid_vel, int64_t(f) * N + node);
+      int j = f * w.W + i;
+      lds[j] = v.x;
+      lds[FW + j] = v.y;
+      lds[2 * FW + j] = v.z;
+    }
   }
   __syncthreads();
+  if (p < 0) return;
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  int cpb = cells_per_brick(P);
-  for (int slot = threadIdx.x; slot < 8 * cpb; slot += blockDim.x) {
-    int64_t p = S.perm[(int64_t(b) * cpb) * 8 + slot];
-    if (p < 0) continue;
-    int wb = cell_window_base(P, w, slot >> 3);
-    Stencil st;
-    stencil_full(ld3(S.local, p), inv_h, st);
-    uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-    m3 rate = mzero();
+  int wb = cell_window_base(P, w, slot >> 3);
+  m3 rate = mzero();
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
-      f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
-      f3 g = st.g[a];
-      rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
-      rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
-      rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
-    }
-    m3 Fm = ld9(S.F, p);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
-    st9(S.F, p, Fm);
-    st9(S.stress, p, stress_of(P, S, p, Fm));
+  for (int a = 0; a < 8; ++a) {
+    float wt;
+    f3 g;
+    stencil_corner(lp, inv_h, a, wt, g);
+    int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+    f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
+    rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
+    rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
+    rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
   }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
+  st9(S.F, p, Fm);
+  st9(S.stress, p, stress_of(P, S, p, Fm, Fp));
 }
 
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
@@ -1686,16 +1695,19 @@ size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
 bool brick_args_ok(const bm_params* p, const bm_state* s) {
   if (!state_ok(p, s) || !s->perm || !s->p2g_partial || !s->remap_partial || !s->field_idx || !s->node_nf) return false;
   if (p->brick[0] < 1 || p->brick[1] < 1 || p->brick[2] < 1 || brick_count(p) < 1) return false;
+  // One lane per particle slot, and one per window node in k_p2g's field count.
+  if (8 * p->brick[0] * p->brick[1] * p->brick[2] > 1024 || window_nodes(p) > 1024) return false;
   if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
       p->bricks[2] * p->brick[2] < p->dims[2] - 1)
     return false;
   return lds_bytes(p, 7, 7) <= 65536 && lds_bytes(p, 9, 3) <= 65536;
 }
-// Threads of a brick workgroup: one per particle slot (8 per cell), whole waves, at most 1024
-// (larger bricks loop over their slots).
+// Threads of a brick workgroup: one per particle slot (8 per cell) and at least one per window
+// node, in whole waves (at most 1024 by brick_args_ok and the LDS bound on the window).
 int brick_threads(const bm_params* p) {
   int64_t slots = 8 * p->brick[0] * p->brick[1] * p->brick[2];
-  int64_t t = ((slots + 63) / 64) * 64;
+  int64_t t = slots > window_nodes(p) ? slots : window_nodes(p);
+  t = ((t + 63) / 64) * 64;
   return int(t > 1024 ? 1024 : t);
 }
 void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
