@@ -82,6 +82,12 @@ class ExternalToolPort:
         self.solver = solver
         dev = solver.device.torch
         self.packed = torch.zeros(49, dtype=torch.float32, device=dev)
+        # Pinned staging ring for the per-interval tool state: the upload is asynchronous, so
+        # the host can enqueue the next interval while the device runs this one; a slot is
+        # reused only after its copy has executed.
+        self._staging = [torch.zeros(49, dtype=torch.float32, pin_memory=True) for _ in range(4)]
+        self._staged = [None] * len(self._staging)
+        self._stage_next = 0
         self.result = torch.zeros(10, dtype=torch.float32, device=dev)
         self.record_trajectory = record_trajectory
         self.trajectory = None
@@ -124,7 +130,15 @@ class ExternalToolPort:
             [state.position, state.orientation_xyzw, state.linear_velocity, state.angular_velocity,
              state.mobility.ravel()]
         ).astype(np.float32)
-        self.packed.copy_(torch.from_numpy(host).to(dev))
+        k = self._stage_next
+        self._stage_next = (k + 1) % len(self._staging)
+        if self._staged[k] is not None:
+            self._staged[k].synchronize()
+        self._staging[k].copy_(torch.from_numpy(host))
+        self.packed.copy_(self._staging[k], non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(torch.cuda.current_stream(dev))
+        self._staged[k] = done
         result = self.result
         packed = self.packed
 

@@ -149,8 +149,6 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
 
     bx = by = brick_xy
     bz = int(max(1, min(cells[2], brick_z, 1024 // (8 * bx * by))))
-    # Same-height bricks along z: ceil(cells / bz) bricks, each as low as that count allows.
-    bz = int(-(-cells[2] // -(-cells[2] // bz))) if cells[2] > 0 else bz
     while bz > 1 and not fits(bx, by, bz):
         bz -= 1
     while not fits(bx, by, bz) and (bx > 1 or by > 1):

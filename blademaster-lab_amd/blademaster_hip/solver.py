@@ -166,6 +166,11 @@ class Solver:
         # reads as an empty bucket, so the lists need no clearing pass.
         T["list_heads"] = t((self.list_buckets,), i64)
         T["list_links"] = t((max(n, 1),), i32, -1)
+        self.changed_capacity = 65536
+        T["changed"] = t((self.changed_capacity,), i32)
+        T["changed_count"] = t((1,), i32)
+        T["pair_mark"] = t((max(n, 1),), i32)
+        T["pair_epoch"] = t((1,), i32)
         T["perm"] = torch.tensor(self.brick_slots, dtype=i32, device=dev)
         window = int(np.prod(np.asarray(self.brick) + 1))
         nb = int(np.prod(self.bricks))
@@ -272,6 +277,7 @@ class Solver:
             p.bricks[i] = int(self.bricks[i])
         p.rigid_blocks = self.rigid_blocks
         p.blade_hold = int(self.blade_hold)
+        p.changed_capacity = self.changed_capacity
         return p
 
     def _make_state(self):

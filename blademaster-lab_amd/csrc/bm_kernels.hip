@@ -31,31 +31,32 @@ inline int blocks_for(int64_t n, int block = kBlock) {
 }
 
 // ------------------------------------------------------------------------ wave helpers
+// DPP lane move within rows of 16 lanes (all lanes of the wavefront active).
+template <int Ctrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+// Wavefront sum / max in a fixed order, result in every lane: DPP butterflies inside each row
+// of 16 lanes (swap neighbours, swap pairs, mirror halves, mirror rows), then the four row
+// results.  All lanes of the wavefront must be active.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
+  v = fmaxf(v, dpp_move<0xB1>(v));
+  v = fmaxf(v, dpp_move<0x4E>(v));
+  v = fmaxf(v, dpp_move<0x141>(v));
+  v = fmaxf(v, dpp_move<0x140>(v));
+  return fmaxf(fmaxf(lane_value(v, 0), lane_value(v, 16)), fmaxf(lane_value(v, 32), lane_value(v, 48)));
 }
 __device__ __forceinline__ bool lane0() { return (threadIdx.x & 63) == 0; }
-
-// atomicMax on non-negative floats through their integer bit patterns.
-__device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {
-  atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
-}
-// Wave-reduced add of a vec3 (all lanes of the wave must call it).
-__device__ __forceinline__ void wave_add3(float* addr, f3 v) {
-  float a = wave_sum(v.x), b = wave_sum(v.y), c = wave_sum(v.z);
-  if (lane0() && (a != 0.0f || b != 0.0f || c != 0.0f)) {
-    atomicAdd(addr, a);
-    atomicAdd(addr + 1, b);
-    atomicAdd(addr + 2, c);
-  }
-}
 
 // ------------------------------------------------------------------------ contact laws
 struct Contact {
@@ -369,39 +370,51 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
 }
 
 __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
+__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d);
 
 // Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
 // the single-thread work of the step: apply the sums, list reach, knife update.  Other
 // blocks: registration of every particle's field slots when the topology changed.
 __global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
-                                                 int parity_new) {
-  if (blockIdx.x > 0) {
+                                                 int parity_new, int insert_blocks) {
+  if (blockIdx.x > 0 && int(blockIdx.x) <= insert_blocks) {
     if ((mode & kTopologyPrep) == 0 || S.topology_dirty[0] == 0) return;
-    int64_t stride = int64_t(gridDim.x - 1) * blockDim.x;
+    int64_t stride = int64_t(insert_blocks) * blockDim.x;
     for (int64_t p = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; p < P.n; p += stride) insert_particle(P, S, p);
     return;
   }
-  __shared__ float part[kBlock][kRigidSums];
+  if (blockIdx.x > 0) {
+    // Relabelled particles with a current pair list: mark the lists to rebuild this step.
+    if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0) return;
+    int count = min(S.changed_count[0], int(P.changed_capacity));
+    int first = int(blockIdx.x) - 1 - insert_blocks, blocks = int(gridDim.x) - 1 - insert_blocks;
+    for (int i = first * int(blockDim.x) + int(threadIdx.x); i < count; i += blocks * int(blockDim.x))
+      mark_pair_neighbours(P, S, S.changed[i]);
+    return;
+  }
+  __shared__ float part[kBlock / 64][kRigidSums];
   float acc[kRigidSums];
 #pragma unroll
   for (int k = 0; k < kRigidSums; ++k) acc[k] = 0.0f;
+#pragma unroll 4
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    const float* src = S.rigid_partial + int64_t(b) * kRigidSums;
+    const float4* src = reinterpret_cast<const float4*>(S.rigid_partial + int64_t(b) * kRigidSums);
+    float4 q0 = src[0], q1 = src[1], q2 = src[2];
+    float v[kRigidSums] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
-    for (int k = 0; k < kRigidSums; ++k) acc[k] = (k == 10) ? fmaxf(acc[k], src[k]) : acc[k] + src[k];
+    for (int k = 0; k < kRigidSums; ++k) acc[k] = (k == 10) ? fmaxf(acc[k], v[k]) : acc[k] + v[k];
   }
 #pragma unroll
-  for (int k = 0; k < kRigidSums; ++k) part[threadIdx.x][k] = acc[k];
-  __syncthreads();
-  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
-    if (threadIdx.x < stride) {
+  for (int k = 0; k < kRigidSums; ++k) acc[k] = (k == 10) ? wave_max(acc[k]) : wave_sum(acc[k]);
+  if (lane0()) {
 #pragma unroll
-      for (int k = 0; k < kRigidSums; ++k) {
-        float o = part[threadIdx.x + stride][k];
-        part[threadIdx.x][k] = (k == 10) ? fmaxf(part[threadIdx.x][k], o) : part[threadIdx.x][k] + o;
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < kRigidSums; ++k) part[threadIdx.x >> 6][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w)
+#pragma unroll
+      for (int k = 0; k < kRigidSums; ++k) part[0][k] = (k == 10) ? fmaxf(part[0][k], part[w][k]) : part[0][k] + part[w][k];
   }
   if (threadIdx.x != 0) return;
   const float* t = part[0];
@@ -424,12 +437,14 @@ __global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, 
   S.work[0] += t[9];
   if (mode & kPairPrep) {
     // set_pair_reach (contact.py:749-761) for a rebuild in this step; reset the new table count.
-    float r = float(P.radius), skin = float(P.pair_skin), sp = S.speed[0];
-    float value = 2.0f * r + skin;
-    if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
-    S.list_reach[0] = value;
     S.pair_count[parity_new] = 0;
-    if (S.pair_dirty[0] != 0) S.pair_builds[0] += 1;
+    if (S.pair_dirty[0] != 0) {
+      float r = float(P.radius), skin = float(P.pair_skin), sp = S.speed[0];
+      float value = 2.0f * r + skin;
+      if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
+      S.list_reach[0] = value;
+      S.pair_builds[0] += 1;
+    }
   }
   if (mode & kAccumulateInterval) accumulate_reaction_dev(S);
   if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
@@ -593,24 +608,48 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   int lanes = int(P.list_lanes);
   int count;
   bool dirty = S.pair_dirty[0] != 0;
-  if (dirty) {
-    f3 xp = ld3(S.x, p);
-    st3(S.built_x, p, xp);
+  // A full rebuild uses the current positions; a rebuild after relabelling only (this
+  // particle or a neighbour got a new code) keeps the positions of the last full build, so
+  // every list stays valid for the same skin.
+  bool rebuild = dirty || S.pair_mark[p] == S.pair_epoch[0] + 1;
+  const float* bx = dirty ? S.x : S.built_x;
+  if (rebuild) {
+    f3 xp = ld3(bx, p);
+    if (dirty) st3(S.built_x, p, xp);
     int64_t cp = S.codes[p];
     S.built_codes[p] = cp;
     f3 rp = ld3(S.ref, p);
     float two_h = 2.0f * float(P.spacing);
     Cell home = cell_of(xp, cell);
     count = 0;
-    for (int dx = -rings; dx <= rings; ++dx)
-      for (int dy = -rings; dy <= rings; ++dy)
-        for (int dz = -rings; dz <= rings; ++dz) {
-          Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-          for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
+    // The bucket chains of up to 9 cells are walked in lockstep, so each step of the walk has
+    // up to 9 independent loads in flight instead of one.
+    constexpr int kChains = 9;
+    int side = 2 * rings + 1, nyz = side * side;
+    for (int dx = -rings; dx <= rings; ++dx) {
+      for (int g0 = 0; g0 < nyz; g0 += kChains) {
+        int32_t cur[kChains];
+        Cell cc[kChains];
+#pragma unroll
+        for (int j = 0; j < kChains; ++j) {
+          int k = g0 + j;
+          cc[j] = Cell{home.x + dx, home.y + k / side - rings, home.z + k % side - rings};
+          cur[j] = k < nyz ? bucket_head(S, bucket_of(cc[j], mask), tag) : -1;
+        }
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int j = 0; j < kChains; ++j) {
+            int32_t q = cur[j];
+            if (q < 0) continue;
+            int32_t next = S.list_links[q];
+            cur[j] = next;
+            more = more || next >= 0;
             if (q <= p || count < 0) continue;
-            f3 xq = ld3(S.x, q);
+            f3 xq = ld3(bx, q);
             Cell cq = cell_of(xq, cell);
-            if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+            if (cq.x != cc[j].x || cq.y != cc[j].y || cq.z != cc[j].z) continue;
             if (!(len(xq - xp) < reach)) continue;
             if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
             if (count == lanes) {
@@ -621,6 +660,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
             }
           }
         }
+      }
+    }
     S.partner_count[p] = count;
   } else {
     count = S.partner_count[p];
@@ -635,8 +676,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
     return;
   }
   // Overflowing particle: search the buckets of the last build around its build position.
-  // In a build step built_x is being written by this kernel; the build positions are x.
-  const float* bx = dirty ? S.x : S.built_x;
+  // In a full build step built_x is being written by this kernel; the build positions are x.
   Cell home = cell_of(ld3(bx, p), cell);
   for (int dx = -rings; dx <= rings; ++dx)
     for (int dy = -rings; dy <= rings; ++dy)
@@ -650,7 +690,32 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
         }
       }
 }
-__global__ void k_pairs_done(bm_state S) { S.pair_dirty[0] = 0; }
+__global__ void k_pairs_done(bm_state S) {
+  S.pair_dirty[0] = 0;
+  S.changed_count[0] = 0;
+  S.pair_epoch[0] += 1;
+}
+
+// Marks relabelled particle d and every particle within the list reach of d at the build
+// positions: the pairs whose candidate status can change with d's code.
+__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d) {
+  int mark = S.pair_epoch[0] + 1;
+  S.pair_mark[d] = mark;
+  int tag = S.pair_builds[0];
+  float reach = S.list_reach[0];
+  float cell = list_cell_size(P);
+  int rings = list_rings(P, reach);
+  int mask = int(P.list_buckets - 1);
+  f3 xd = ld3(S.built_x, d);
+  Cell home = cell_of(xd, cell);
+  for (int dx = -rings; dx <= rings; ++dx)
+    for (int dy = -rings; dy <= rings; ++dy)
+      for (int dz = -rings; dz <= rings; ++dz) {
+        Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+        for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q])
+          if (len(ld3(S.built_x, q) - xd) < reach) S.pair_mark[q] = mark;
+      }
+}
 
 // check_pair_list (contact.py:716-735) as its own pass for a standalone pair-contact call.
 __global__ void k_check_pair_list(const bm_params P, bm_state S) {
@@ -786,10 +851,6 @@ __device__ __forceinline__ int cells_per_brick(const bm_params& P) { return int(
 // Sum over the 8 lanes of a cell (lanes 8c .. 8c+7 of the wavefront) with DPP lane moves:
 // swap neighbours, swap pairs, then mirror the two quads; every lane of the group ends with
 // the total.  Call with the whole wavefront active.
-template <int Ctrl>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xf, 0xf, false));
-}
 __device__ __forceinline__ float cell_sum(float v) {
   v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
@@ -834,7 +895,11 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   if (flags && b == 0 && threadIdx.x == 0) {
     // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
     if (flags & 1) S.topology_dirty[0] = 0;
-    if (flags & 2) S.pair_dirty[0] = 0;
+    if (flags & 2) {
+      S.pair_dirty[0] = 0;
+      S.changed_count[0] = 0;
+      S.pair_epoch[0] += 1;
+    }
   }
   Window w = window_of(P, b);
   int F = int(P.fields);
@@ -972,7 +1037,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   int64_t N = P.node_count;
   int nx, ny, nz;
   node_coords(P, node, nx, ny, nz);
-  constexpr int kMaxLocal = 8;
+  constexpr int kMaxLocal = 4;
   if (nf == 1) {
     float g[7];
     gather_node<7>(P, S.p2g_partial, nx, ny, nz, 0, g);
@@ -984,7 +1049,41 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
     }
     return;
   }
-  // Several fields: gather all, then exchange (fields beyond kMaxLocal re-read global sums).
+  // Several fields: gather all, then exchange.  Up to kMaxLocal fields stay in registers;
+  // more go through the grid arrays.
+  if (nf <= kMaxLocal) {
+    float g[kMaxLocal][7];
+    int64_t code[kMaxLocal];
+    float total = 0.0f;
+#pragma unroll
+    for (int f = 0; f < kMaxLocal; ++f) {
+      if (f < nf) {
+        gather_node<7>(P, S.p2g_partial, nx, ny, nz, f, g[f]);
+        code[f] = S.field_code[int64_t(f) * N + node];
+        S.grid_mass[int64_t(f) * N + node] = g[f][0];
+        total += g[f][0];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < kMaxLocal; ++a) {
+      if (a >= nf || !(g[a][0] > 0.0f)) continue;
+      float ma = g[a][0];
+      f3 pa = f3{g[a][1], g[a][2], g[a][3]}, ja = f3{g[a][4], g[a][5], g[a][6]};
+      f3 dm = f3{0.f, 0.f, 0.f}, di = dm;
+#pragma unroll
+      for (int b = 0; b < kMaxLocal; ++b) {
+        if (b == a || b >= nf || !compatible(code[a], code[b])) continue;
+        float mb = g[b][0];
+        dm = dm + (ma * f3{g[b][1], g[b][2], g[b][3]} - mb * pa);
+        di = di + (ma * f3{g[b][4], g[b][5], g[b][6]} - mb * ja);
+      }
+      f3 pm = pa + dm / total, jm = ja + di / total;
+      int64_t sa = int64_t(a) * N + node;
+      st3(S.old_v, sa, pm / ma);
+      st3(S.new_v, sa, (pm + jm) / ma);
+    }
+    return;
+  }
   float total = 0.0f;
   for (int f = 0; f < nf; ++f) {
     float g[7];
@@ -995,7 +1094,6 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
     st3(S.grid_impulse, s, f3{g[4], g[5], g[6]});
     total += g[0];
   }
-  (void)kMaxLocal;
   for (int a = 0; a < nf; ++a) {
     int64_t sa = int64_t(a) * N + node;
     float ma = S.grid_mass[sa];
@@ -1161,6 +1259,40 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
     f3 vel = f3{0.f, 0.f, 0.f};
     if (m > 0.0f) vel = f3{g[0], g[1], g[2]} / m;
     st3(S.grid_vel, node, vel);
+    return;
+  }
+  constexpr int kMaxLocal = 4;
+  if (nf <= kMaxLocal) {
+    float g[kMaxLocal][3], m[kMaxLocal];
+    int64_t code[kMaxLocal];
+    float total = 0.0f;
+#pragma unroll
+    for (int f = 0; f < kMaxLocal; ++f) {
+      if (f < nf) {
+        gather_node<3>(P, S.remap_partial, nx, ny, nz, f, g[f]);
+        m[f] = S.grid_mass[int64_t(f) * N + node];
+        code[f] = S.field_code[int64_t(f) * N + node];
+        total += m[f];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < kMaxLocal; ++a) {
+      if (a >= nf) continue;
+      int64_t sa = int64_t(a) * N + node;
+      float ma = m[a];
+      if (ma <= 0.0f) {
+        st3(S.grid_vel, sa, f3{0.f, 0.f, 0.f});
+        continue;
+      }
+      f3 xa = f3{g[a][0], g[a][1], g[a][2]};
+      f3 ex = f3{0.f, 0.f, 0.f};
+#pragma unroll
+      for (int b = 0; b < kMaxLocal; ++b) {
+        if (b == a || b >= nf || !compatible(code[a], code[b])) continue;
+        ex = ex + (ma * f3{g[b][0], g[b][1], g[b][2]} - m[b] * xa);
+      }
+      st3(S.grid_vel, sa, (xa + ex / total) / ma);
+    }
     return;
   }
   float total = 0.0f;
@@ -1502,9 +1634,15 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   if (fabsf(local.x) <= 6.0f * spacing && fabsf(local.y) <= hf.y && (behind || pinched)) {
     int64_t side = local.x < 0.0f ? 1 : 2;
     S.codes[p] = code + digit * side;
-    // A new side code changes the grid fields and the pair candidates.
+    // A new side code changes the grid fields and the pair candidates: the lists around the
+    // relabelled particle are rebuilt next step (all lists if the record overflows).
     S.topology_dirty[0] = 1;
-    S.pair_dirty[0] = 1;
+    int k = atomicAdd(S.changed_count, 1);
+    if (k < P.changed_capacity) {
+      S.changed[k] = int32_t(p);
+    } else {
+      S.pair_dirty[0] = 1;
+    }
   }
 }
 
@@ -1671,7 +1809,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
                      int(kAccumulate), 0);
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
-                     ToolTarget{nullptr, nullptr, 0}, 0);
+                     ToolTarget{nullptr, nullptr, 0}, 0, 0);
   return launch_status();
 }
 
@@ -1679,7 +1817,8 @@ namespace {
 bool pair_args_ok(const bm_params* p, const bm_state* s) {
   return state_ok(p, s) && s->pair_keys && s->pair_history && s->pair_used && s->pair_count && s->pair_dirty &&
          s->pair_builds && s->list_reach && s->built_x && s->built_codes && s->partners && s->partner_count &&
-         s->list_heads && s->list_links && p->pair_capacity > 0 && (p->pair_capacity & (p->pair_capacity - 1)) == 0 &&
+         s->list_heads && s->list_links && s->changed && s->changed_count && s->pair_mark && s->pair_epoch &&
+         p->changed_capacity >= 0 && p->pair_capacity > 0 && (p->pair_capacity & (p->pair_capacity - 1)) == 0 &&
          p->list_buckets > 0 && (p->list_buckets & (p->list_buckets - 1)) == 0 &&
          p->list_buckets <= int64_t(4096) * 4096 && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
 }
@@ -1818,7 +1957,9 @@ int bm_accumulate_reaction(const bm_params* p, const bm_state* s, void* stream) 
 }
 
 int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutting, void* stream) {
-  if (!state_ok(p, s) || digit < 1) return -1;
+  if (!state_ok(p, s) || digit < 1 || !s->codes || !s->topology_dirty || !s->pair_dirty || !s->changed ||
+      !s->changed_count)
+    return -1;
   if (p->n == 0 || !cutting) return 0;
   hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, digit,
                      int(cutting));
@@ -1848,9 +1989,11 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
   }
   int nb = blocks_for(p->n);
   int insert_blocks = nb < 2048 ? nb : 2048;
+  int mark_blocks = int((p->changed_capacity + kBlock - 1) / kBlock);
+  mark_blocks = mark_blocks < 64 ? mark_blocks : 64;
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
-  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks), dim3(kBlock), 0, st, *p, *s, nb, mode,
-                     ToolTarget{targets, rotations, int(index)}, new_p);
+  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kBlock), 0, st, *p, *s, nb, mode,
+                     ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);
   // 3. particle pairs across cuts (partner lists rebuilt first when stale)
   hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p);
   // 4. transfers; G2P labels the moved particles and checks the pair list's skin

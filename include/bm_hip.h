@@ -55,6 +55,7 @@ typedef struct bm_params {
   int64_t bricks[3];      /* bricks along x, y, z */
   int64_t rigid_blocks;   /* blocks of the contact kernel (partial sums per block) */
   int64_t blade_hold;     /* external tool held over the interval: grid constraint active */
+  int64_t changed_capacity; /* entries of the changed-code list (more: full pair-list rebuild) */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
@@ -92,6 +93,12 @@ typedef struct bm_state {
   int32_t *partner_count;  /* [n], -1 = more than lanes partners */
   int64_t *list_heads;     /* [B]: (build tag << 32) | first particle of the bucket */
   int32_t *list_links;     /* [n]: next particle of the same bucket, -1 at the end */
+  /* pair-list updates after label changes only: the particles relabelled since the last
+   * list update, and the particles whose lists are rebuilt at the build positions */
+  int32_t *changed;        /* [changed_capacity] */
+  int32_t *changed_count;  /* [1] */
+  int32_t *pair_mark;      /* [n]: == pair_epoch + 1 for a list to rebuild this step */
+  int32_t *pair_epoch;     /* [1] */
   /* brick bins: bricks x-major, cells of a brick z-fastest; cell c holds the particles
    * perm[8c .. 8c+8) in reference order, -1 for unused slots (at most 8 per cell) */
   int32_t *perm;           /* [8 * bricks * cells per brick] */

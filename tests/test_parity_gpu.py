@@ -188,3 +188,44 @@ def test_graph_replay_matches_eager():
         np.testing.assert_allclose(a["velocity"], b["velocity"], atol=2e-5)
         np.testing.assert_allclose(a["deformation"], b["deformation"], atol=2e-5)
         np.testing.assert_array_equal(a["sides"], b["sides"])
+
+
+def test_relabelled_pair_lists_match_a_full_rebuild():
+    """After labels change on the device (as assign does), the pair lists of the relabelled
+    particles and their neighbours are rebuilt at the last build positions; the contacts of
+    the next substep equal those of a full rebuild from the current positions."""
+    cfg = small_cfg(dt=0.00025)
+
+    def make():
+        s = Solver(cfg)
+        x = s.x.numpy()
+        codes = np.where(s.reference[:, 0] < 0, 1, 2)
+        x[:, 0] += np.where(codes == 1, 0.0006, -0.0006)
+        s.x.assign(x)
+        s.codes.assign(codes)
+        s.v.assign(np.random.default_rng(4).normal(0, 0.01, (s.n, 3)).astype(np.float32))
+        s.step(cutting=False)
+        return s
+
+    a, b, control = make(), make(), make()
+    assert int(a.pair_count[a.step_count % 2].numpy()[0]) > 0
+    # Relabel part of the right half to the left half's code.
+    ref = a.reference
+    moved = np.nonzero((ref[:, 0] > 0) & (ref[:, 1] > 0))[0].astype(np.int32)
+    for s in (a, b):
+        T = s._t
+        T["codes"][torch.from_numpy(moved).long().to(T["codes"].device)] = 1
+        T["topology_dirty"].fill_(1)
+    a._t["changed"][: len(moved)] = torch.from_numpy(moved).to(a._t["changed"].device)
+    a._t["changed_count"].fill_(len(moved))
+    b._t["pair_dirty"].fill_(1)
+    builds = int(a.pair_list_builds.numpy()[0])
+    for s in (a, b, control):
+        s.step(cutting=False)
+    assert int(a.pair_list_builds.numpy()[0]) == builds  # no full rebuild in a
+    count_a, count_b, unchanged = (int(s.pair_count[s.step_count % 2].numpy()[0]) for s in (a, b, control))
+    # The relabelled particles lose their cross-cut pairs and gain pairs with their former
+    # same-side neighbours: the contact set differs from the unrelabelled run.
+    assert count_a == count_b and count_a > 0 and count_a != unchanged
+    for name in ("x", "v"):
+        assert rel(getattr(a, name).numpy().ravel(), getattr(b, name).numpy().ravel()) < 1e-6, name
