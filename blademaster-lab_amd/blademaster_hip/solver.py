@@ -148,6 +148,7 @@ class Solver:
         T["interval_angular"] = t((1, 3))
         T["errors"] = t((3,), i32)
         T["topology_dirty"] = t((1,), i32, 1)
+        T["fp_identity"] = t((1,), i32, 1)
         # pair contacts
         P_ = self.pair_capacity
         self.list_buckets = _pow2_at_least(2 * n)
@@ -202,7 +203,9 @@ class Solver:
         mark_pairs = self._mark_pairs
         self.x, self.v = arr(T["x"], mark_pairs), arr(T["v"], mark_pairs)
         self.ref = arr(T["ref"])
-        self.F, self.Fp, self.stress = arr(T["F"]), arr(T["Fp"]), arr(T["stress"])
+        self.F, self.stress = arr(T["F"]), arr(T["stress"])
+        # Host writes to Fp end the device's identity shortcut for Fp.
+        self.Fp = arr(T["Fp"], lambda: T["fp_identity"].fill_(0))
         self.codes = arr(T["codes"], mark_topology)
         self.single_field_codes = arr(T["single_codes"], mark_topology)
         self.damage = arr(T["damage"])
@@ -515,6 +518,19 @@ class Solver:
         count = torch.zeros(1, dtype=torch.int32, device=self.device.torch)
         self._call("bm_count_embedded", ctypes.c_void_p(count.data_ptr()))
         return int(count.item())
+
+    def embedded_particles_previous(self):
+        """`embedded_particles` of the previous call; this call's count is copied back
+        asynchronously into pinned memory (solver.py:902-918).  The next synchronising call
+        (an `advance`, a readback) completes the copy before it is read."""
+        if not hasattr(self, "_embedded"):
+            self._embedded = torch.zeros(1, dtype=torch.int32, device=self.device.torch)
+            self._embedded_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        previous = int(self._embedded_host[0])
+        self._embedded.zero_()
+        self._call("bm_count_embedded", ctypes.c_void_p(self._embedded.data_ptr()))
+        self._embedded_host.copy_(self._embedded, non_blocking=True)
+        return previous
 
     # ------------------------------------------------------------------ field-table views
     @property

@@ -182,9 +182,11 @@ __device__ inline void svd3(const m3& A, m3& U, f3& s, m3& V) {
   };
 #pragma unroll 1
   for (int sweep = 0; sweep < 6; ++sweep) {
+    // Converged when the off-diagonal is at float rounding level of the diagonal (relative
+    // 3e-7): further sweeps only exchange rounding errors.
     float off = s01 * s01 + s02 * s02 + s12 * s12;
     float diag = s00 * s00 + s11 * s11 + s22 * s22;
-    if (off <= 1.0e-16f * diag) break;
+    if (off <= 1.0e-13f * diag) break;
     jacobi(s00, s11, s01, s02, s12, 0, 1);
     jacobi(s00, s22, s02, s01, s12, 0, 2);
     jacobi(s11, s22, s12, s01, s02, 1, 2);

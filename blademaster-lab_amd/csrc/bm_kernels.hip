@@ -840,6 +840,15 @@ __device__ __forceinline__ int window_max(const int* wave_nf) {
   return m;
 }
 
+// Brick of workgroup blk.  Workgroups go to the 8 XCDs round-robin (blk % 8), each XCD with
+// its own L2: give every XCD a contiguous run of bricks, so bricks that share window nodes and
+// particle cache lines are processed under the same L2.
+__device__ __forceinline__ int brick_of_block(int blk, int nb) {
+  int q = nb >> 3, r = nb & 7;
+  int xcd = blk & 7, k = blk >> 3;
+  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
 // Window index of corner 0 of the c-th cell of a brick (cells z-fastest).
 __device__ __forceinline__ int cell_window_base(const bm_params& P, const Window& w, int c) {
   int bz = int(P.brick[2]), by = int(P.brick[1]);
@@ -891,7 +900,7 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
-  int b = blockIdx.x;
+  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
   if (flags && b == 0 && threadIdx.x == 0) {
     // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
     if (flags & 1) S.topology_dirty[0] = 0;
@@ -993,7 +1002,8 @@ __device__ __forceinline__ int bricks_at(int c, int B, int nb, int* out) {
   return k;
 }
 
-// Sum of the window entries of node (nx, ny, nz), field f, components [0, C) over its bricks.
+// Sum of the window entries of node (nx, ny, nz), field f, components [0, C) over its (at
+// most 2 x 2 x 2) bricks, in brick order.  All loads are issued before the first add.
 template <int C>
 __device__ __forceinline__ void gather_node(const bm_params& P, const float* partial, int nx, int ny, int nz, int f,
                                             float* out) {
@@ -1004,17 +1014,31 @@ __device__ __forceinline__ void gather_node(const bm_params& P, const float* par
   int kz = bricks_at(nz, BZ, int(P.bricks[2]), bz);
   int wy = BY + 1, wz = BZ + 1, W = (BX + 1) * wy * wz;
   int F = int(P.fields);
+  float v[8][C];
+  bool ok[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    int i = t >> 2, j = (t >> 1) & 1, l = t & 1;
+    ok[t] = i < kx && j < ky && l < kz;
+    int64_t b = ok[t] ? (int64_t(bx[i]) * P.bricks[1] + by[j]) * P.bricks[2] + bz[l] : 0;
+    int widx = ok[t] ? ((nx - bx[i] * BX) * wy + (ny - by[j] * BY)) * wz + (nz - bz[l] * BZ) : 0;
+    const float* src = partial + ((b * F + f) * W + widx) * C;
+    if (ok[t]) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[t][c] = src[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[t][c] = 0.0f;
+    }
+  }
 #pragma unroll
   for (int c = 0; c < C; ++c) out[c] = 0.0f;
-  for (int i = 0; i < kx; ++i)
-    for (int j = 0; j < ky; ++j)
-      for (int l = 0; l < kz; ++l) {
-        int64_t b = (int64_t(bx[i]) * P.bricks[1] + by[j]) * P.bricks[2] + bz[l];
-        int widx = ((nx - bx[i] * BX) * wy + (ny - by[j] * BY)) * wz + (nz - bz[l] * BZ);
-        const float* src = partial + ((b * F + f) * W + widx) * C;
 #pragma unroll
-        for (int c = 0; c < C; ++c) out[c] += src[c];
-      }
+  for (int t = 0; t < 8; ++t)
+    if (ok[t]) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) out[c] += v[t][c];
+    }
 }
 
 __device__ __forceinline__ void node_coords(const bm_params& P, int64_t node, int& nx, int& ny, int& nz) {
@@ -1126,7 +1150,7 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
-  int b = blockIdx.x;
+  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
   Window w = window_of(P, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -1349,6 +1373,7 @@ __device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t
     m3 next_f = mmul_t(US, V);
     Fp = mmul(minv(next_f), F);
     st9(S.Fp, p, Fp);
+    if (S.fp_identity != nullptr && !is_identity(Fp)) S.fp_identity[0] = 0;
   }
   return true;
 }
@@ -1385,7 +1410,7 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 // overlap the staging.
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S) {
   extern __shared__ float lds[];
-  int b = blockIdx.x;
+  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
   Window w = window_of(P, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -1396,11 +1421,18 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
   f3 lp = f3{0.f, 0.f, 0.f};
   uint2 fidx = make_uint2(0u, 0u);
   m3 Fm = mzero(), Fp = mzero();
+  // While no plastic flow has happened and the host has not written Fp, every Fp is I.
+  bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
   if (p >= 0) {
     lp = ld3(S.local, p);
     fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
     Fm = ld9(S.F, p);
-    Fp = ld9(S.Fp, p);
+    if (fp_identity) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fp.a[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    } else {
+      Fp = ld9(S.Fp, p);
+    }
   }
   for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
     int64_t node = window_node(P, w, i);

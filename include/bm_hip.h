@@ -79,6 +79,7 @@ typedef struct bm_state {
   float *reaction, *torque, *board, *work, *speed, *table;
   float *interval_impulse, *interval_angular;
   int32_t *errors, *topology_dirty;
+  int32_t *fp_identity;    /* [1] 1 while every Fp is exactly I (no plastic flow, no host write) */
   /* pair contacts: ping-pong history tables [2][P] and the partner lists */
   int64_t *pair_keys;      /* [2][P], -1 empty */
   float *pair_history;     /* [2][P][3] */
