@@ -23,6 +23,9 @@ wrong sign or index fails:
   tool.step        free-fall closed form (tests/test_multi_tool.py:68-79)
   predict mode     x = 0.01 t, q_z = sin(0.1 t) (tests/test_coupling.py:322-339)
   compatible       tests/test_topology.py:10-12
+  MLS transfers    APIC carries an affine velocity field exactly (v = A x + b + g dt, C = A)
+  blade constraint face particles leave a held-blade substep without velocity into the
+                   faces beyond their gap (tests/test_blade_constraint.py:57-101)
 
 Parity unpinned: the exact float32 bits of the reference's wp.svd3 and of Warp's quaternion
 code generation (no Warp in this environment); the oracle's SVD is evaluated in float64.
@@ -369,3 +372,64 @@ def test_max_cut_index_capacity():
 
     assert max_cut_index(225) == 33
     assert 225 * 3**34 - 1 <= np.iinfo(np.int64).max < 225 * 3**35 - 1
+
+
+def test_mls_apic_reproduces_an_affine_velocity_field():
+    """APIC transfers with quadratic B-splines carry an affine velocity field v = A x + b
+    exactly (Jiang et al. 2015): after one substep from F = I (no stress) every particle has
+    v = A x + b + g dt and C = A.  A wrong weight, D^-1 scale, dt or gravity sign fails."""
+    from blademaster_hip.mlsmpm import TopoCutParams
+
+    cfg = Config(size=(0.048, 0.024, 0.018), grid_spacing=0.006, dt=1.0e-4)
+    prm = TopoCutParams()
+    ref, _, _ = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    ref = ref + np.array([0.0, 0.0, 0.03])  # clear of the board
+    dx = cfg.grid_spacing
+    low = ref.min(0) - prm.margin
+    high = ref.max(0) + prm.margin
+    low[2] = -2 * dx
+    high[2] = ref[:, 2].max() + prm.headroom
+    dims = np.ceil((high - low) / dx).astype(int) + 1
+    o = oracle.OracleMls(cfg, prm, ref, low, dims, precision=64)
+    A = np.array([[0.0, -0.2, 0.05], [0.2, 0.0, 0.0], [0.0, 0.03, 0.1]])
+    b = np.array([0.01, -0.02, 0.005])
+    v = ref @ A.T + b
+    o.set("v", v)
+    o.set("C", np.tile(A, (len(ref), 1, 1)))
+    o.set("tool_pos", [0.0, 0.0, 1.0])  # knife far away
+    o.substep(1)
+    g = np.array([0.0, 0.0, -9.81]) * cfg.dt
+    np.testing.assert_allclose(o.get("v").reshape(-1, 3), v + g, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(o.get("C").reshape(-1, 3, 3), np.tile(A, (len(ref), 1, 1)), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(o.get("x").reshape(-1, 3), ref + cfg.dt * (v + g), rtol=0, atol=1e-12)
+
+
+def test_held_blade_face_particles_do_not_advance_into_the_faces():
+    """With the knife held (external tool, hold mode) and pure PIC transfer, particles squeezed
+    against both blade faces leave the substep with no velocity into a face beyond what closes
+    their present gap (blade_constraint.py; reference tests/test_blade_constraint.py:57-101),
+    while without the constraint they would move in at the squeeze speed."""
+    cfg = Config(size=(0.036, 0.018, 0.018), grid_spacing=0.006, dt=0.0005, external_tool=True,
+                 cache_topology=True, blade_half_size=(0.0005, 0.025, 0.02), flip_fraction=0.0)
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, w, gr = stencil(ref, origin, cfg.grid_spacing, cells)
+    o = OracleSim(cfg, ref, nodes, w, gr, int(np.prod(cells + 1)), 64, 0)
+    hx, hz = cfg.blade_half_size[0], cfg.blade_half_size[2]
+    x = ref.copy()
+    near = np.abs(x[:, 0]) < 0.002
+    x[near, 0] = np.sign(x[near, 0]) * (hx + 5.0e-5)
+    squeeze = 0.2
+    v = np.zeros_like(x)
+    v[:, 0] = -np.sign(ref[:, 0]) * squeeze
+    o.set("x", x)
+    o.set("v", v)
+    o.set("tool_pos", [0.0, 0.0, cfg.size[2] + hz - 0.009])
+    o.substep(cutting=False)
+    side = o.get("bc_side")
+    active = np.flatnonzero(side != 0)
+    assert len(active) > 0
+    vx = o.get("v").reshape(-1, 3)[active, 0]
+    gap = np.abs(x[active, 0]) - hx
+    assert np.all(side[active] * vx >= -np.maximum(gap, 0) / cfg.dt - 1e-9)
+    assert np.max(np.abs(vx)) < 0.5 * squeeze
+    assert o.get("bc_max_violation")[0] <= 1e-6
