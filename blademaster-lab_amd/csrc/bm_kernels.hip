@@ -238,6 +238,25 @@ __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float
 __device__ void finish_external_dev(const bm_params& P, const bm_state& S);
 __device__ void accumulate_reaction_dev(const bm_state& S);
 
+// Tool and interval state read by finish_external_step, loaded in one batch.
+struct ExternalTool {
+  f3 impulse, angular, pos, vel, omega;
+  quat rot;
+  float mobility[36];
+};
+__device__ __forceinline__ ExternalTool load_external_tool(const bm_params& P, const bm_state& S) {
+  ExternalTool e;
+  e.impulse = ld3(S.interval_impulse, 0);
+  e.angular = ld3(S.interval_angular, 0);
+  e.pos = ld3(S.tool_pos, 0);
+  e.vel = ld3(S.tool_vel, 0);
+  e.omega = ld3(S.tool_omega, 0);
+  e.rot = ldq(S.tool_rot);
+  for (int k = 0; k < 36; ++k) e.mobility[k] = P.predict ? S.mobility[k] : 0.0f;
+  return e;
+}
+__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau);
+
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag);
 
 constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
@@ -375,7 +394,9 @@ __device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int3
 // Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
 // the single-thread work of the step: apply the sums, list reach, knife update.  Other
 // blocks: registration of every particle's field slots when the topology changed.
-__global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
+constexpr int kPrepBlock = 1024;  // k_prep: block 0 reduces the partials with 16 waves
+
+__global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
                                                  int parity_new, int insert_blocks) {
   if (blockIdx.x > 0 && int(blockIdx.x) <= insert_blocks) {
     if ((mode & kTopologyPrep) == 0 || S.topology_dirty[0] == 0) return;
@@ -392,7 +413,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, 
       mark_pair_neighbours(P, S, S.changed[i]);
     return;
   }
-  __shared__ float part[kBlock / 64][kRigidSums];
+  __shared__ float part[kPrepBlock / 64][kRigidSums];
   float acc[kRigidSums];
 #pragma unroll
   for (int k = 0; k < kRigidSums; ++k) acc[k] = 0.0f;
@@ -412,43 +433,53 @@ __global__ void __launch_bounds__(kBlock) k_prep(const bm_params P, bm_state S, 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < kBlock / 64; ++w)
+    for (int w = 1; w < int(blockDim.x >> 6); ++w)
 #pragma unroll
       for (int k = 0; k < kRigidSums; ++k) part[0][k] = (k == 10) ? fmaxf(part[0][k], part[w][k]) : part[0][k] + part[w][k];
   }
   if (threadIdx.x != 0) return;
   const float* t = part[0];
   const float dt = float(P.dt);
-  if (mode & kOverwrite) {
-    for (int i = 0; i < 3; ++i) {
-      S.board[i] = t[i];
-      S.reaction[i] = t[3 + i];
-      S.torque[i] = t[6 + i];
-    }
-    S.speed[0] = t[10];
-  } else {
-    for (int i = 0; i < 3; ++i) {
-      S.board[i] += t[i];
-      S.reaction[i] += t[3 + i];
-      S.torque[i] += t[6 + i];
-    }
-    S.speed[0] = fmaxf(S.speed[0], t[10]);
+  // A load's wait also waits for this thread's earlier stores, so everything the serial tail
+  // reads is loaded before anything is written.
+  f3 board = f3{t[0], t[1], t[2]}, react = f3{t[3], t[4], t[5]}, torq = f3{t[6], t[7], t[8]};
+  float speed = t[10];
+  if (!(mode & kOverwrite)) {
+    board = board + ld3(S.board, 0);
+    react = react + ld3(S.reaction, 0);
+    torq = torq + ld3(S.torque, 0);
+    speed = fmaxf(speed, S.speed[0]);
   }
-  S.work[0] += t[9];
+  float work = S.work[0] + t[9];
+  int dirty = (mode & kPairPrep) ? S.pair_dirty[0] : 0;
+  int builds = (mode & kPairPrep) ? S.pair_builds[0] : 0;
+  bool finish = (mode & kFinishExternal) != 0;
+  ExternalTool ext;
+  if (finish) ext = load_external_tool(P, S);
+  // stores
+  S.work[0] = work;
   if (mode & kPairPrep) {
     // set_pair_reach (contact.py:749-761) for a rebuild in this step; reset the new table count.
     S.pair_count[parity_new] = 0;
-    if (S.pair_dirty[0] != 0) {
-      float r = float(P.radius), skin = float(P.pair_skin), sp = S.speed[0];
+    if (dirty) {
+      float r = float(P.radius), skin = float(P.pair_skin);
       float value = 2.0f * r + skin;
-      if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
+      if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
       S.list_reach[0] = value;
-      S.pair_builds[0] += 1;
+      S.pair_builds[0] = builds + 1;
     }
   }
+  if (finish) {
+    // finish_external_step: the step sums are consumed and reset here.
+    finish_external_regs(P, S, ext, react, torq);
+    return;
+  }
+  st3(S.board, 0, board);
+  st3(S.reaction, 0, react);
+  st3(S.torque, 0, torq);
+  S.speed[0] = speed;
   if (mode & kAccumulateInterval) accumulate_reaction_dev(S);
   if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
-  if (mode & kFinishExternal) finish_external_dev(P, S);
 }
 
 __global__ void k_clear_step(bm_state S) {
@@ -992,7 +1023,6 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
     for (int k = 0; k < 7; ++k) out[int64_t(i) * 7 + k] = sum[k];
   }
 }
-
 // Bricks along one axis whose window holds node coordinate c: c / B, and c / B - 1 when c
 // is on a brick boundary.
 __device__ __forceinline__ int bricks_at(int c, int B, int nb, int* out) {
@@ -1265,7 +1295,6 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     out[int64_t(i) * 3 + 2] = sum[2];
   }
 }
-
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S) {
@@ -1595,6 +1624,57 @@ __global__ void k_tool_step(const bm_params P, bm_state S, const float* targets,
 }
 
 // finish_external_step (tool.py:70-115)
+// finish_external_step (tool.py:70-115) on values already in registers: interval impulse and
+// angular impulse about the world origin; with predict, twist += Mobility [j; tau] and the
+// pose is integrated; the step sums are reset.
+__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau) {
+  f3 impulse = e.impulse + j;
+  f3 angular = e.angular + (tau + cross(e.pos, j));
+  f3 vel = e.vel, w = e.omega, pos = e.pos;
+  quat rot = e.rot;
+  bool turn = false;
+  if (P.predict) {
+    float u[6] = {j.x, j.y, j.z, tau.x, tau.y, tau.z};
+    float dv[3], dw[3];
+    for (int r = 0; r < 3; ++r) {
+      float a = 0.0f, b = 0.0f;
+      for (int k = 0; k < 6; ++k) {
+        a += e.mobility[6 * r + k] * u[k];
+        b += e.mobility[6 * (r + 3) + k] * u[k];
+      }
+      dv[r] = a;
+      dw[r] = b;
+    }
+    vel = vel + f3{dv[0], dv[1], dv[2]};
+    w = w + f3{dw[0], dw[1], dw[2]};
+    pos = pos + float(P.dt) * vel;
+    float rate = len(w);
+    if (rate > 1.0e-12f) {
+      rot = qnormalize(qmul(axis_angle(w / rate, float(P.dt) * rate), rot));
+      turn = true;
+    }
+  }
+  st3(S.interval_impulse, 0, impulse);
+  st3(S.interval_angular, 0, angular);
+  if (P.predict) {
+    st3(S.tool_vel, 0, vel);
+    st3(S.tool_omega, 0, w);
+    st3(S.tool_pos, 0, pos);
+    if (turn) {
+      S.tool_rot[0] = rot.x;
+      S.tool_rot[1] = rot.y;
+      S.tool_rot[2] = rot.z;
+      S.tool_rot[3] = rot.w;
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    S.reaction[i] = 0.0f;
+    S.torque[i] = 0.0f;
+    S.board[i] = 0.0f;
+  }
+  S.speed[0] = 0.0f;
+}
+
 __device__ void finish_external_dev(const bm_params& P, const bm_state& S) {
   f3 j = ld3(S.reaction, 0), tau = ld3(S.torque, 0);
   f3 pos = ld3(S.tool_pos, 0);
@@ -1840,7 +1920,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   int nb = blocks_for(p->n);
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
                      int(kAccumulate), 0);
-  hipLaunchKernelGGL(k_prep, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
+  hipLaunchKernelGGL(k_prep, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
                      ToolTarget{nullptr, nullptr, 0}, 0, 0);
   return launch_status();
 }
@@ -2020,11 +2100,13 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
     mode |= kToolStep | (accumulate ? kAccumulateInterval : 0);
   }
   int nb = blocks_for(p->n);
-  int insert_blocks = nb < 2048 ? nb : 2048;
-  int mark_blocks = int((p->changed_capacity + kBlock - 1) / kBlock);
-  mark_blocks = mark_blocks < 64 ? mark_blocks : 64;
+  // Registration and marking blocks exit at once on most substeps; keep their count modest.
+  int insert_blocks = int((p->n + kPrepBlock - 1) / kPrepBlock);
+  insert_blocks = insert_blocks < 128 ? insert_blocks : 128;
+  int mark_blocks = int((p->changed_capacity + kPrepBlock - 1) / kPrepBlock);
+  mark_blocks = mark_blocks < 4 ? mark_blocks : 4;
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
-  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kBlock), 0, st, *p, *s, nb, mode,
+  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
                      ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);
   // 3. particle pairs across cuts (partner lists rebuilt first when stale)
   hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p);
