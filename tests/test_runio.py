@@ -1,0 +1,68 @@
+"""On-disk formats of the run outputs (paper_freefall.py:108-155, run.py:37-38,
+paper_impact.py:65-72): states.npz keys, JSON writer, knife edge height."""
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+from blademaster_hip.runio import StateLog, edge_height, load_states, write
+
+STATE_KEYS = ["position", "velocity", "deformation", "plastic_deformation", "sides", "damage", "blade_position",
+              "blade_velocity", "blade_orientation", "blade_omega", "time"]
+
+
+def fake_state(n, t):
+    rng = np.random.default_rng(int(t * 1000))
+    return {
+        "position": rng.normal(size=(n, 3)).astype(np.float32),
+        "velocity": rng.normal(size=(n, 3)).astype(np.float32),
+        "deformation": np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
+        "plastic_deformation": np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
+        "sides": rng.integers(0, 3, n).astype(np.int64),
+        "damage": rng.random(n).astype(np.float32),
+        "blade_position": rng.normal(size=(1, 3)).astype(np.float32),
+        "blade_velocity": rng.normal(size=(1, 3)).astype(np.float32),
+        "blade_orientation": np.array([[0, 0, 0, 1]], np.float32),
+        "blade_omega": np.zeros((1, 3), np.float32),
+        "time": t,
+    }
+
+
+def test_states_npz_stacks_every_state_key_and_the_reference(tmp_path):
+    n = 7
+    log = StateLog(fake_state(n, 0.0))
+    states = [fake_state(n, 0.01 * k) for k in range(1, 4)]
+    for st in states:
+        log.add(st)
+    ref = np.arange(3 * n, dtype=np.float64).reshape(n, 3)
+    log.save(tmp_path / "states.npz", ref)
+    z = load_states(tmp_path / "states.npz")
+    assert sorted(z) == sorted(STATE_KEYS + ["reference"])
+    assert z["position"].shape == (4, n, 3) and z["sides"].dtype == np.int64
+    np.testing.assert_array_equal(z["time"], [0.0, 0.01, 0.02, 0.03])
+    np.testing.assert_array_equal(z["position"][2], states[1]["position"])
+    np.testing.assert_array_equal(z["reference"], ref)
+    with pytest.raises(ValueError):
+        log.add({"position": 0})
+
+
+def test_json_writer_matches_the_reference_format(tmp_path):
+    write(tmp_path / "a.json", {"b": [1, 2.5], "a": "x"})
+    text = (tmp_path / "a.json").read_text()
+    assert text == json.dumps({"b": [1, 2.5], "a": "x"}, indent=2) + "\n"
+    with pytest.raises(ValueError):
+        write(tmp_path / "b.json", {"x": float("nan")})
+
+
+def test_edge_height_closed_forms():
+    half = (0.0005, 0.06, 0.02)
+    low, tilt = edge_height([[[0.0, 0.0, 0.1]]], [[[0.0, 0.0, 0.0, 1.0]]], half)
+    assert low.shape == (1, 1) and abs(low[0, 0] - 0.08) < 1e-15 and abs(tilt[0, 0]) < 1e-12
+    a = math.radians(25.0)
+    q = [math.sin(a / 2), 0.0, 0.0, math.cos(a / 2)]  # about x: the edge line tilts in y-z
+    low, tilt = edge_height([[[0.0, 0.0, 0.1]]], [[q]], half)
+    expected = 0.1 - half[2] * math.cos(a) - half[1] * math.sin(a)
+    assert abs(low[0, 0] - expected) < 1e-12
+    assert abs(tilt[0, 0] - 25.0) < 1e-9
