@@ -106,6 +106,17 @@ class MlsState(ctypes.Structure):
     _fields_ = MLS_STATE_FIELDS
 
 
+LABELS_FIELDS = [
+    ("fields", ctypes.c_int64), ("corner_mask", c_u8p), ("field_code", c_i64p), ("node_nf", c_i32p), ("field_idx", c_u8p),
+    ("parent", c_i32p), ("root", c_i32p), ("block_sum", c_i32p), ("labels", c_i32p), ("count", c_i32p),
+    ("errors", c_i32p),
+]
+
+
+class Labels(ctypes.Structure):
+    _fields_ = LABELS_FIELDS
+
+
 MP = ctypes.POINTER(MlsParams)
 MS = ctypes.POINTER(MlsState)
 P = ctypes.POINTER(Params)
@@ -150,6 +161,7 @@ SIGNATURES = {
     "bm_mls_grid_update": [MP, MS, VP],
     "bm_mls_g2p": [MP, MS, VP],
     "bm_mls_substep": [P, S, MP, MS, VP],
+    "bm_piece_labels": [P, S, ctypes.POINTER(Labels), VP],
 }
 
 _lib = None

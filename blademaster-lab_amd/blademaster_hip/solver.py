@@ -532,6 +532,48 @@ class Solver:
         self._embedded_host.copy_(self._embedded, non_blocking=True)
         return previous
 
+    def piece_labels(self):
+        """Piece labels of the current codes on the device, equal bit for bit to
+        `components(node_numpy, weight_numpy, codes)` (topology.py:17-57): labels (n,)
+        ranked by the lowest particle index of each piece, and the piece sizes."""
+        dev = self.device.torch
+        n, N = self.n, self.node_count
+        if n == 0:
+            return np.zeros(0, dtype=np.int64), np.zeros(0, dtype=np.int64)
+        # The reference has no cap on codes per node: the scratch table grows until it fits.
+        F = getattr(self, "_label_fields", 16)
+        while True:
+            B = self._label_scratch(F)
+            B["errors"].zero_()
+            L = _lib.Labels()
+            L.fields = F
+            for field, ctype in _lib.LABELS_FIELDS[1:]:
+                setattr(L, field, ctypes.cast(ctypes.c_void_p(B[field].data_ptr()), ctype))
+            self._call("bm_piece_labels", ctypes.byref(L))
+            if int(B["errors"].cpu()[0]) == 0:
+                break
+            if F >= 255:
+                raise RuntimeError("Piece labels: a grid node holds more than 255 cut codes")
+            F = min(255, 2 * F)
+        self._label_fields = F
+        labels = B["labels"].cpu().numpy().astype(np.int64)
+        return labels, np.bincount(labels, minlength=int(B["count"].cpu()[0]))
+
+    def _label_scratch(self, F):
+        B = getattr(self, "_label_buffers", None)
+        if B is not None and B["field_code"].shape[0] == F:
+            return B
+        dev = self.device.torch
+        n, N = self.n, self.node_count
+        bits = (np.asarray(self.weight_numpy) > 0).astype(np.uint8) << np.arange(8, dtype=np.uint8)
+        z = lambda *shape, dtype=torch.int32: torch.zeros(shape, dtype=dtype, device=dev)  # noqa: E731
+        self._label_buffers = dict(
+            corner_mask=torch.tensor(bits.sum(axis=1).astype(np.uint8), device=dev),
+            field_code=z(F, N, dtype=torch.int64), node_nf=z(N), field_idx=z(n, 8, dtype=torch.uint8),
+            parent=z(n + F * N), root=z(n), block_sum=z(-(-n // 1024)), labels=z(n), count=z(1), errors=z(1),
+        )
+        return self._label_buffers
+
     # ------------------------------------------------------------------ field-table views
     @property
     def weights(self):

@@ -222,6 +222,28 @@ int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk,
 /* 3x3 SVD of A[9*count] -> U[9], s[3], V[9] per lane. */
 int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream);
 
+/* ---------------------------------------------------------------- piece labels
+ * topology.components (topology.py:17-57): connected components of the particles through
+ * the (grid node, cut code) fields of their positive-weight stencil corners, compatible
+ * fields of one node joined; labels ranked by the lowest particle index of each piece.
+ * The caller provides the scratch buffers. */
+typedef struct bm_labels {
+  int64_t fields;             /* L: slots per node of the scratch field table (1..255) */
+  const uint8_t *corner_mask; /* [n] bit a set: the stencil weight of corner a is > 0 */
+  int64_t *field_code;        /* [L][N] scratch field table */
+  int32_t *node_nf;           /* [N] */
+  uint8_t *field_idx;         /* [n][8] */
+  int32_t *parent;            /* [n + L N] union-find forest */
+  int32_t *root;              /* [n] */
+  int32_t *block_sum;         /* [ceil(n / 1024)] */
+  int32_t *labels;            /* [n] out */
+  int32_t *count;             /* [1] out: number of pieces */
+  int32_t *errors;            /* [1] out: nonzero when a node needed more than L fields */
+} bm_labels;
+
+/* Replaces topology.components (topology.py:17-57) for a solver's current codes. */
+int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream);
+
 /* ---------------------------------------------------------------------------------
  * MLS-MPM cutting baseline (blademaster_warp/mlsmpm.py:42-229, MlsSolver._substep
  * mlsmpm.py:293-393).  Dense grid of dims nodes with planes [3][nodes] for momentum and

@@ -216,7 +216,8 @@ def test_struct_layouts_match_the_header():
     from blademaster_hip import _lib
 
     text = open(os.path.join(ROOT, "include", "bm_hip.h")).read()
-    for struct, mirror in (("bm_params", _lib.Params), ("bm_state", _lib.State)):
+    for struct, mirror in (("bm_params", _lib.Params), ("bm_state", _lib.State), ("bm_labels", _lib.Labels),
+                           ("bm_mls_params", _lib.MlsParams), ("bm_mls_state", _lib.MlsState)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, flags=re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = []
@@ -224,8 +225,8 @@ def test_struct_layouts_match_the_header():
             decl = decl.strip()
             if not decl:
                 continue
-            parts = decl.split(None, 1)[1]
-            for item in parts.split(","):
+            declarators = re.sub(r"^(const\s+)?\w+\s*", "", decl)
+            for item in declarators.split(","):
                 item = item.strip().lstrip("*").strip()
                 names.append(re.sub(r"\[.*\]", "", item))
         assert names == [f[0] for f in mirror._fields_], struct
