@@ -36,6 +36,7 @@ class Params(ctypes.Structure):
         ("rigid_blocks", ctypes.c_int64),
         ("blade_hold", ctypes.c_int64),
         ("changed_capacity", ctypes.c_int64),
+        ("instances", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)
         for name in (
@@ -67,7 +68,8 @@ STATE_FIELDS = [
     ("tool_pos", c_f32p), ("tool_vel", c_f32p), ("tool_rot", c_f32p), ("tool_omega", c_f32p), ("mobility", c_f32p),
     ("reaction", c_f32p), ("torque", c_f32p), ("board", c_f32p), ("work", c_f32p), ("speed", c_f32p),
     ("table", c_f32p), ("interval_impulse", c_f32p), ("interval_angular", c_f32p),
-    ("errors", c_i32p), ("topology_dirty", c_i32p), ("fp_identity", c_i32p),
+    ("errors", c_i32p), ("topology_dirty", c_i32p), ("fp_identity", c_i32p), ("instance", c_i32p),
+    ("inst_mu", c_f32p), ("inst_lam", c_f32p),
     ("pair_keys", c_i64p), ("pair_history", c_f32p), ("pair_used", c_i32p), ("pair_count", c_i32p),
     ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
     ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("list_heads", c_i64p),
@@ -162,6 +164,8 @@ SIGNATURES = {
     "bm_mls_g2p": [MP, MS, VP],
     "bm_mls_substep": [P, S, MP, MS, VP],
     "bm_piece_labels": [P, S, ctypes.POINTER(Labels), VP],
+    "bm_load_external_batch": [P, S, VP, VP],
+    "bm_store_interval_batch": [P, S, VP, VP],
 }
 
 _lib = None

@@ -1,0 +1,227 @@
+"""Many independent cutting instances in one solver: the batched driver of force_convergence.
+
+`scripts/force_convergence.cut_trace` (`:92-129`) runs one prescribed cut per Solver, one
+after the other.  Here the instances are packed side by side on one grid, with a gap wider than
+a knife between neighbours, so no particle, grid node or knife of one instance reaches
+another.  Every substep kernel then runs once for all instances. The particle arrays carry an
+instance id; the knife state, the Lame constants, the step sums and the interval sums are kept
+per instance.  Instances share the grid spacing, time step, density, knife geometry and contact
+parameters.  Size, shape, sampling seed, Young's modulus and Poisson's ratio vary per instance,
+and so does every knife's pose and twist (submitted per interval, in the instance's own frame).
+
+The brick height of the transfer kernels is chosen by timing (`autotune_brick_z`).
+"""
+
+from dataclasses import asdict, replace
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import derived
+from .coupling import Reaction, ToolState
+from .geometry import grid_cells, points
+from .solver import Solver
+
+# Fields that may differ between packed instances.
+PER_INSTANCE = {"size", "young_modulus", "poisson_ratio", "sampling_seed", "shape"}
+
+
+def _reach(cfg):
+    """Radius of a sphere around the knife origin that holds the knife."""
+    return float(np.linalg.norm(cfg.blade_half_size)) + cfg.grid_spacing
+
+
+def pack(configs, gap=None):
+    """Side-by-side placement of the instances' reference particles on one grid.
+
+    Shelf packing in x and y with `gap` metres between neighbours (default: a knife diameter
+    plus two cells).  Each instance keeps its own lattice: its grid corner lands on a node of
+    the packed grid.  Returns the layout for `Solver` and, per instance, the offset added to
+    its own coordinates and its particle range.
+    """
+    base = configs[0]
+    h = base.grid_spacing
+    if gap is None:
+        gap = 2.0 * max(_reach(c) for c in configs) + 2 * h
+    gap_cells = int(np.ceil(gap / h))
+    parts = []
+    for cfg in configs:
+        ref, origin, cells = points(cfg.size, h, cfg.shape, cfg.sampling_seed)
+        parts.append((ref, origin, np.asarray(cells)))
+    widths = np.array([p[2][0] for p in parts]) + gap_cells
+    depths = np.array([p[2][1] for p in parts]) + gap_cells
+    row_width = max(int(np.ceil(np.sqrt(float(np.sum(widths * depths))))), int(widths.max()))
+    corner = []
+    x = y = row_depth = 0
+    for w, d in zip(widths, depths):
+        if x and x + w > row_width:
+            x, y, row_depth = 0, y + row_depth, 0
+        corner.append((x, y))
+        x += w
+        row_depth = max(row_depth, d)
+    corner = np.array(corner, dtype=np.int64)
+    cells = np.array([max(c[0] + p[2][0] for c, p in zip(corner, parts)),
+                      max(c[1] + p[2][1] for c, p in zip(corner, parts)),
+                      max(p[2][2] for p in parts)], dtype=np.int64)
+    origin = np.zeros(3)
+    reference, instance, offsets, slices = [], [], [], []
+    start = 0
+    mu, lam = [], []
+    for i, ((ref, own_origin, own_cells), (cx, cy)) in enumerate(zip(parts, corner)):
+        offset = origin + np.array([cx * h, cy * h, 0.0]) - own_origin
+        reference.append(ref + offset)
+        instance.append(np.full(len(ref), i, dtype=np.int32))
+        offsets.append(offset)
+        slices.append(slice(start, start + len(ref)))
+        start += len(ref)
+        d = derived(configs[i])
+        mu.append(d["mu"])
+        lam.append(d["lam"])
+    layout = dict(reference=np.concatenate(reference), origin=origin, cells=cells,
+                  instance=np.concatenate(instance), mu=np.array(mu), lam=np.array(lam))
+    return layout, np.array(offsets), slices
+
+
+class BatchSolver(Solver):
+    """Independent cutting instances advanced together.
+
+    configs: one Config per instance; they must agree on every field but size, shape,
+    sampling seed, Young's modulus and Poisson's ratio, and use an external knife with the
+    mobility predictor (prescribed motion: a zero mobility matrix).
+    """
+
+    def __init__(self, configs, device="cuda:0", brick_z=None, gap=None):
+        configs = list(configs)
+        if not configs:
+            raise ValueError("No instances")
+        base = configs[0]
+        shared = {k: v for k, v in asdict(base).items() if k not in PER_INSTANCE}
+        for cfg in configs[1:]:
+            other = {k: v for k, v in asdict(cfg).items() if k not in PER_INSTANCE}
+            if other != shared:
+                diff = sorted(k for k in shared if shared[k] != other[k])
+                raise ValueError(f"Batched instances must share {diff}")
+        if not base.external_tool or base.tool_substep != "predict":
+            raise ValueError("Batched instances need external_tool=True with tool_substep='predict'")
+        layout, self.offsets, self.slices = pack(configs, gap)
+        self.instance_configs = configs
+        size = tuple(float(c) * base.grid_spacing for c in layout["cells"])
+        super().__init__(replace(base, size=size), device, brick_z, _layout=layout)
+
+    # ------------------------------------------------------------------ per-instance views
+    def instance_state(self, i, name):
+        """Host copy of a per-particle array of instance i, positions in its own frame."""
+        values = getattr(self, name).numpy()[self.slices[i]]
+        if name in ("x", "ref"):
+            values = values - self.offsets[i].astype(values.dtype)
+        return values
+
+    def instance_labels(self, i):
+        """Piece labels of instance i alone (`topology.components` of its particles)."""
+        labels, _ = self.piece_labels()
+        _, local = np.unique(labels[self.slices[i]], return_inverse=True)
+        local = local.ravel().astype(np.int64)
+        return local, np.bincount(local)
+
+
+class BatchPort:
+    """Per-instance coupling intervals for a BatchSolver (the batched ExternalToolPort).
+
+    submit(states, duration) enqueues one interval for all instances: each ToolState is in
+    its instance's own frame; collect() returns one Reaction per instance, with the angular
+    impulse about that frame's origin.
+    """
+
+    def __init__(self, solver):
+        self.solver = s = solver
+        dev = s.device.torch
+        I = s.instances
+        self.packed = torch.zeros(I, 49, dtype=torch.float32, device=dev)
+        self.result = torch.zeros(I, 6, dtype=torch.float32, device=dev)
+        self._staging = [torch.zeros(I, 49, dtype=torch.float32, pin_memory=True) for _ in range(4)]
+        self._staged = [None] * len(self._staging)
+        self._stage_next = 0
+
+    def _ptr(self, tensor):
+        return ctypes.c_void_p(tensor.data_ptr())
+
+    def submit(self, states, duration_s, *, cutting=True, cut_index=0):
+        s = self.solver
+        if len(states) != s.instances:
+            raise ValueError(f"{len(states)} tool states for {s.instances} instances")
+        states = [st.checked() for st in states]
+        steps = int(round(duration_s / s.config.dt))
+        if steps < 1 or abs(steps * s.config.dt - duration_s) > 1.0e-12 + 1.0e-10 * abs(duration_s):
+            raise ValueError("Coupling duration must be an integer number of material substeps")
+        cut_index = s._set_cut(cut_index)
+        host = np.zeros((s.instances, 49), dtype=np.float32)
+        for i, st in enumerate(states):
+            host[i] = np.concatenate([np.asarray(st.position) + s.offsets[i], st.orientation_xyzw, st.linear_velocity,
+                                      st.angular_velocity, np.asarray(st.mobility).ravel()])
+        k = self._stage_next
+        self._stage_next = (k + 1) % len(self._staging)
+        if self._staged[k] is not None:
+            self._staged[k].synchronize()
+        self._staging[k].copy_(torch.from_numpy(host))
+        self.packed.copy_(self._staging[k], non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(torch.cuda.current_stream(s.device.torch))
+        self._staged[k] = done
+
+        def load():
+            s.launch("bm_load_external_batch", self._ptr(self.packed))
+
+        def finish():
+            s.launch("bm_validate_interval")
+            s.launch("bm_store_interval_batch", self._ptr(self.result))
+
+        unused_targets = np.zeros((steps, 3), dtype=np.float32)
+        _, _, graph = s.prepare(unused_targets, cutting, cut_index, before=[load], after=[finish],
+                                tag=("batch", id(self)))
+        graph.replay()
+        s.step_count += steps
+        return states, steps
+
+    def collect(self, states, duration_s):
+        s = self.solver
+        errors = s.errors.numpy()
+        if errors[1]:
+            raise RuntimeError("Inverted or nonfinite material; run is invalid")
+        out = self.result.cpu().numpy().astype(np.float64)
+        reactions = []
+        for i, st in enumerate(states):
+            impulse = out[i, :3]
+            angular = out[i, 3:] - np.cross(s.offsets[i], impulse)
+            reactions.append(Reaction(duration_s=duration_s, impulse_ns=impulse, angular_impulse_nms=angular,
+                                      origin_world_m=np.asarray(st.position, dtype=float)))
+        return reactions
+
+    def advance(self, states, duration_s, *, cutting=True, cut_index=0):
+        states, _ = self.submit(states, duration_s, cutting=cutting, cut_index=cut_index)
+        return self.collect(states, duration_s)
+
+
+def autotune_brick_z(configs, candidates=(8, 12, 16, 24, 32), substeps=20, device="cuda:0"):
+    """Brick height with the shortest substep for this batch, by timing a prescribed
+    plunge of every knife over `substeps` substeps per candidate."""
+    best, timings = None, {}
+    for bz in candidates:
+        solver = BatchSolver(configs, device=device, brick_z=bz)
+        port = BatchPort(solver)
+        states = [ToolState([0.0, 0.0, c.size[2] + 0.01 + c.blade_half_size[2]], [0, 0, 0, 1], [0, 0, -0.05],
+                            [0, 0, 0]) for c in configs]
+        dt = configs[0].dt
+        port.submit(states, substeps * dt)  # capture
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        port.submit(states, substeps * dt)
+        torch.cuda.synchronize()
+        timings[bz] = (time.perf_counter() - start) / substeps
+        if best is None or timings[bz] < timings[best]:
+            best = bz
+        del port, solver
+        torch.cuda.empty_cache()
+    return best, timings

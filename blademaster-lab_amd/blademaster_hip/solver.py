@@ -59,9 +59,12 @@ class Solver:
     """brick_z: height in cells of the particle bricks of the transfer kernels (a tuning knob;
     results do not depend on it beyond float summation order)."""
 
-    def __init__(self, config=Config(), device="cuda:0", brick_z=None):
+    def __init__(self, config=Config(), device="cuda:0", brick_z=None, _layout=None):
         self.config = cfg = config
         self._brick_z = brick_z
+        # _layout (batched instances): reference, origin, cells of the packed domain, the
+        # instance of each particle and per-instance Lame constants.
+        self.instances = 1 if _layout is None else len(_layout["mu"])
         self.device = _Device(device)
         if not self.device.is_cuda or not torch.cuda.is_available():
             raise ValueError("CUDA device required")
@@ -69,7 +72,10 @@ class Solver:
             raise ValueError("tool_count > 1 (several free knives) is not provided by this build")
         self._lib = _lib.load()
         dev = self.device.torch
-        self.reference, self.origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+        if _layout is None:
+            self.reference, self.origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+        else:
+            self.reference, self.origin, cells = _layout["reference"], _layout["origin"], _layout["cells"]
         self.cells = np.asarray(cells)
         self.n = n = len(self.reference)
         self.node_numpy, self.weight_numpy, self._gradient_numpy = stencil(
@@ -133,19 +139,24 @@ class Solver:
         for name in ("momentum", "grid_impulse", "old_v", "new_v", "remapped", "grid_vel"):
             T[name] = t((F_, N_, 3))
         hz = cfg.blade_half_size[2]
-        T["tool_pos"] = torch.tensor([[0.0, 0.0, cfg.size[2] + 0.012 + hz]], dtype=f32, device=dev)
-        T["tool_vel"] = t((1, 3))
-        T["tool_rot"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]], dtype=f32, device=dev)
-        T["tool_omega"] = t((1, 3))
-        T["tool_mobility"] = t((6, 6))
-        T["reaction"] = t((1, 3))
-        T["torque"] = t((1, 3))
-        T["board"] = t((1, 3))
-        T["work"] = t((1,))
+        I = self.instances
+        T["tool_pos"] = torch.tensor([[0.0, 0.0, cfg.size[2] + 0.012 + hz]] * I, dtype=f32, device=dev)
+        T["tool_vel"] = t((I, 3))
+        T["tool_rot"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]] * I, dtype=f32, device=dev)
+        T["tool_omega"] = t((I, 3))
+        T["tool_mobility"] = t((6, 6)) if I == 1 else t((I, 6, 6))
+        T["reaction"] = t((I, 3))
+        T["torque"] = t((I, 3))
+        T["board"] = t((I, 3))
+        T["work"] = t((I,))
         T["speed"] = t((1,))
         T["table"] = t((1, 3))
-        T["interval_impulse"] = t((1, 3))
-        T["interval_angular"] = t((1, 3))
+        T["interval_impulse"] = t((I, 3))
+        T["interval_angular"] = t((I, 3))
+        if _layout is not None:
+            T["instance"] = torch.tensor(np.asarray(_layout["instance"], dtype=np.int32), device=dev)
+            T["inst_mu"] = torch.tensor(np.asarray(_layout["mu"], dtype=np.float32), device=dev)
+            T["inst_lam"] = torch.tensor(np.asarray(_layout["lam"], dtype=np.float32), device=dev)
         T["errors"] = t((3,), i32)
         T["topology_dirty"] = t((1,), i32, 1)
         T["fp_identity"] = t((1,), i32, 1)
@@ -281,6 +292,7 @@ class Solver:
         p.rigid_blocks = self.rigid_blocks
         p.blade_hold = int(self.blade_hold)
         p.changed_capacity = self.changed_capacity
+        p.instances = getattr(self, "instances", 1)
         return p
 
     def _make_state(self):
@@ -292,7 +304,9 @@ class Solver:
                 key = "tool_mobility"
             elif field == "transfer_codes":
                 key = "single_codes" if self._single_field_flag() else "codes"
-            tensor = T[key]
+            tensor = T.get(key)
+            if tensor is None:
+                continue  # absent buffers stay NULL (e.g. the batched-instance arrays)
             setattr(s, field, ctypes.cast(ctypes.c_void_p(tensor.data_ptr()), ctype))
         return s
 

@@ -57,6 +57,8 @@ __device__ __forceinline__ float wave_max(float v) {
   return fmaxf(fmaxf(lane_value(v, 0), lane_value(v, 16)), fmaxf(lane_value(v, 32), lane_value(v, 48)));
 }
 __device__ __forceinline__ bool lane0() { return (threadIdx.x & 63) == 0; }
+// Instance of particle p (0 for a single solver).
+__device__ __forceinline__ int instance_of(const bm_state& S, int64_t p) { return S.instance ? S.instance[p] : 0; }
 
 // ------------------------------------------------------------------------ contact laws
 struct Contact {
@@ -244,18 +246,19 @@ struct ExternalTool {
   quat rot;
   float mobility[36];
 };
-__device__ __forceinline__ ExternalTool load_external_tool(const bm_params& P, const bm_state& S) {
+__device__ __forceinline__ ExternalTool load_external_tool(const bm_params& P, const bm_state& S, int i = 0) {
   ExternalTool e;
-  e.impulse = ld3(S.interval_impulse, 0);
-  e.angular = ld3(S.interval_angular, 0);
-  e.pos = ld3(S.tool_pos, 0);
-  e.vel = ld3(S.tool_vel, 0);
-  e.omega = ld3(S.tool_omega, 0);
-  e.rot = ldq(S.tool_rot);
-  for (int k = 0; k < 36; ++k) e.mobility[k] = P.predict ? S.mobility[k] : 0.0f;
+  e.impulse = ld3(S.interval_impulse, i);
+  e.angular = ld3(S.interval_angular, i);
+  e.pos = ld3(S.tool_pos, i);
+  e.vel = ld3(S.tool_vel, i);
+  e.omega = ld3(S.tool_omega, i);
+  e.rot = ldq(S.tool_rot + 4 * i);
+  for (int k = 0; k < 36; ++k) e.mobility[k] = P.predict ? S.mobility[36 * i + k] : 0.0f;
   return e;
 }
-__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau);
+__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau,
+                                     int i = 0);
 
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag);
 
@@ -272,8 +275,10 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   const float dt = float(P.dt);
   Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
-  quat q = ldq(S.tool_rot);
-  f3 origin = ld3(S.tool_pos, 0), tvel = ld3(S.tool_vel, 0), tom = ld3(S.tool_omega, 0);
+  // The knife of this particle's instance (instance 0 for a single solver).
+  int inst = live ? instance_of(S, p) : 0;
+  quat q = ldq(S.tool_rot + 4 * inst);
+  f3 origin = ld3(S.tool_pos, inst), tvel = ld3(S.tool_vel, inst), tom = ld3(S.tool_omega, inst);
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
   int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -332,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
       if (distance - offset + dt * fminf(dot(relative, n), 0.0f) >= 0.0f) {
         st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
       } else {
-        float me = effective_mass(P, S.mobility, lever, n, q);
+        float me = effective_mass(P, S.mobility + 36 * inst, lever, n, q);
         f3 j, h;
         penalty(c, distance - offset, relative, n, me, ld3(S.histories, hrow), float(P.sn), j, h);
         st3(S.histories, hrow, h);
@@ -366,6 +371,28 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
       }
     }
     st3(S.impulses, p, total);
+  }
+  if (S.instance != nullptr) {
+    // Batched instances: the step sums go to each instance's own scalars (a wave reduction
+    // when the whole wave belongs to one instance, else per lane).
+    float v[10] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z, torq.x, torq.y, torq.z, work};
+    float* dst[10] = {S.board + 3 * inst, S.board + 3 * inst + 1, S.board + 3 * inst + 2,
+                      S.reaction + 3 * inst, S.reaction + 3 * inst + 1, S.reaction + 3 * inst + 2,
+                      S.torque + 3 * inst, S.torque + 3 * inst + 1, S.torque + 3 * inst + 2, S.work + inst};
+    int first = __builtin_amdgcn_readfirstlane(inst);
+    if (__ballot(inst != first) == 0) {
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        float t = wave_sum(v[k]);
+        if (lane0() && t != 0.0f) atomicAdd(dst[k], t);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        if (v[k] != 0.0f) atomicAdd(dst[k], v[k]);
+    }
+    board_j = react = torq = f3{0.f, 0.f, 0.f};
+    work = 0.0f;
   }
   // Block sums: wave reduction, then the waves of the block in order.
   __shared__ float wsum[kBlock / 64][kRigidSums];
@@ -436,6 +463,33 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     for (int w = 1; w < int(blockDim.x >> 6); ++w)
 #pragma unroll
       for (int k = 0; k < kRigidSums; ++k) part[0][k] = (k == 10) ? fmaxf(part[0][k], part[w][k]) : part[0][k] + part[w][k];
+  }
+  if (S.instance != nullptr) {
+    // Batched instances: each thread finishes the step of some instances (their sums were
+    // accumulated by k_rigid); thread 0 also keeps the shared pair-list scalars.
+    if (mode & kFinishExternal) {
+      for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
+        ExternalTool e = load_external_tool(P, S, i);
+        f3 j = ld3(S.reaction, i), tau = ld3(S.torque, i);
+        finish_external_regs(P, S, e, j, tau, i);
+      }
+    }
+    if (threadIdx.x != 0) return;
+    const float dt = float(P.dt);
+    float speed = part[0][10];
+    S.speed[0] = speed;
+    if (mode & kPairPrep) {
+      int dirty = S.pair_dirty[0];
+      S.pair_count[parity_new] = 0;
+      if (dirty) {
+        float r = float(P.radius), skin = float(P.pair_skin);
+        float value = 2.0f * r + skin;
+        if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
+        S.list_reach[0] = value;
+        S.pair_builds[0] += 1;
+      }
+    }
+    return;
   }
   if (threadIdx.x != 0) return;
   const float* t = part[0];
@@ -1376,9 +1430,21 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
   }
 }
 
+// Lame constants of particle p: its instance's for batched instances, else the solver's.
+__device__ __forceinline__ void lame_of(const bm_params& P, const bm_state& S, int64_t p, float& mu, float& lam) {
+  if (S.inst_mu != nullptr) {
+    int i = instance_of(S, p);
+    mu = S.inst_mu[i];
+    lam = S.inst_lam[i];
+  } else {
+    mu = float(P.mu);
+    lam = float(P.lam);
+  }
+}
+
 // Hencky von Mises return map (material.py:404-431).  Returns false when inverted.
 __device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t p, const m3& F, m3& Fp, m3& U,
-                                  f3& sg, m3& V) {
+                                  f3& sg, m3& V, float mu) {
   svd3(mmul(F, minv(Fp)), U, sg, V);
   if (fminf(fminf(sg.x, sg.y), sg.z) <= 0.0f) {
     atomicAdd(S.errors + 1, 1);
@@ -1388,7 +1454,7 @@ __device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t
   float mean = (e.x + e.y + e.z) / 3.0f;
   f3 dev = e - f3{mean, mean, mean};
   float norm = len(dev);
-  float limit = float(P.yield_stress) / (2.0f * float(P.mu)) * sqrtf(2.0f / 3.0f);
+  float limit = float(P.yield_stress) / (2.0f * mu) * sqrtf(2.0f / 3.0f);
   if (norm > limit) {
     f3 ep = e - (1.0f - limit / norm) * dev;
     sg = f3{expf(ep.x), expf(ep.y), expf(ep.z)};
@@ -1408,7 +1474,8 @@ __device__ inline bool plastic_at(const bm_params& P, const bm_state& S, int64_t
 }
 
 // stress_at (material.py:94-112)
-__device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const m3& F, const m3& Fp) {
+__device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const m3& F, const m3& Fp, float mu,
+                                   float lam) {
   // Fp = I (no plastic flow yet, and always without plasticity): f = F, Fp^-T = I exactly.
   bool plain = is_identity(Fp);
   m3 pinv = plain ? Fp : minv(Fp);
@@ -1421,17 +1488,19 @@ __device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const
   m3 U, V;
   f3 sg;
   svd3(f, U, sg, V);
-  return corotated(U, sg, V, pinv, float(P.mu), float(P.lam));
+  return corotated(U, sg, V, pinv, mu, lam);
 }
 
 __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p, const m3& F, m3 Fp) {
+  float mu, lam;
+  lame_of(P, S, p, mu, lam);
   if (P.plastic) {
     m3 U, V;
     f3 sg;
-    if (!plastic_at(P, S, p, F, Fp, U, sg, V)) return mzero();
-    return corotated(U, sg, V, minv(Fp), float(P.mu), float(P.lam));
+    if (!plastic_at(P, S, p, F, Fp, U, sg, V, mu)) return mzero();
+    return corotated(U, sg, V, minv(Fp), mu, lam);
   }
-  return elastic_stress(P, S, F, Fp);
+  return elastic_stress(P, S, F, Fp, mu, lam);
 }
 
 // update_F_plastic (material.py:445-475) for one brick, one lane per particle slot, with the
@@ -1500,7 +1569,9 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  st9(S.stress, p, elastic_stress(P, S, ld9(S.F, p), ld9(S.Fp, p)));
+  float mu, lam;
+  lame_of(P, S, p, mu, lam);
+  st9(S.stress, p, elastic_stress(P, S, ld9(S.F, p), ld9(S.Fp, p), mu, lam));
 }
 
 __global__ void k_plastic_project(const bm_params P, bm_state S) {
@@ -1508,7 +1579,9 @@ __global__ void k_plastic_project(const bm_params P, bm_state S) {
   if (p >= P.n) return;
   m3 Fp = ld9(S.Fp, p), U, V;
   f3 sg;
-  plastic_at(P, S, p, ld9(S.F, p), Fp, U, sg, V);
+  float mu, lam;
+  lame_of(P, S, p, mu, lam);
+  plastic_at(P, S, p, ld9(S.F, p), Fp, U, sg, V, mu);
 }
 
 // ------------------------------------------------------------------------ tool
@@ -1627,7 +1700,8 @@ __global__ void k_tool_step(const bm_params P, bm_state S, const float* targets,
 // finish_external_step (tool.py:70-115) on values already in registers: interval impulse and
 // angular impulse about the world origin; with predict, twist += Mobility [j; tau] and the
 // pose is integrated; the step sums are reset.
-__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau) {
+__device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau,
+                                     int i) {
   f3 impulse = e.impulse + j;
   f3 angular = e.angular + (tau + cross(e.pos, j));
   f3 vel = e.vel, w = e.omega, pos = e.pos;
@@ -1654,25 +1728,23 @@ __device__ void finish_external_regs(const bm_params& P, const bm_state& S, cons
       turn = true;
     }
   }
-  st3(S.interval_impulse, 0, impulse);
-  st3(S.interval_angular, 0, angular);
+  st3(S.interval_impulse, i, impulse);
+  st3(S.interval_angular, i, angular);
   if (P.predict) {
-    st3(S.tool_vel, 0, vel);
-    st3(S.tool_omega, 0, w);
-    st3(S.tool_pos, 0, pos);
+    st3(S.tool_vel, i, vel);
+    st3(S.tool_omega, i, w);
+    st3(S.tool_pos, i, pos);
     if (turn) {
-      S.tool_rot[0] = rot.x;
-      S.tool_rot[1] = rot.y;
-      S.tool_rot[2] = rot.z;
-      S.tool_rot[3] = rot.w;
+      S.tool_rot[4 * i] = rot.x;
+      S.tool_rot[4 * i + 1] = rot.y;
+      S.tool_rot[4 * i + 2] = rot.z;
+      S.tool_rot[4 * i + 3] = rot.w;
     }
   }
-  for (int i = 0; i < 3; ++i) {
-    S.reaction[i] = 0.0f;
-    S.torque[i] = 0.0f;
-    S.board[i] = 0.0f;
-  }
-  S.speed[0] = 0.0f;
+  st3(S.reaction, i, f3{0.f, 0.f, 0.f});
+  st3(S.torque, i, f3{0.f, 0.f, 0.f});
+  st3(S.board, i, f3{0.f, 0.f, 0.f});
+  if (S.instance == nullptr) S.speed[0] = 0.0f;
 }
 
 __device__ void finish_external_dev(const bm_params& P, const bm_state& S) {
@@ -1727,13 +1799,14 @@ __global__ void k_accumulate_reaction(bm_state S) { accumulate_reaction_dev(S); 
 __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp) {
   int64_t code = S.codes[p];
   if ((code / digit) % 3 != 0) return;
+  int inst = instance_of(S, p);
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
-  quat q = ldq(S.tool_rot);
-  f3 pos = ld3(S.tool_pos, 0);
+  quat q = ldq(S.tool_rot + 4 * inst);
+  f3 pos = ld3(S.tool_pos, inst);
   f3 edge = pos + rotate(q, f3{0.f, 0.f, -hf.z});
   f3 local = rotate_inv(q, xp - edge);
   float front = -local.z;
-  f3 edge_v = ld3(S.tool_vel, 0) + cross(ld3(S.tool_omega, 0), edge - pos);
+  f3 edge_v = ld3(S.tool_vel, inst) + cross(ld3(S.tool_omega, inst), edge - pos);
   f3 d = rotate(q, f3{0.f, 0.f, -1.f});
   float advance = dot(edge_v - vp, d);
   float spacing = float(P.spacing), r = float(P.radius), dt = float(P.dt);
@@ -1788,6 +1861,26 @@ __global__ void k_store_interval(bm_state S, float* out) {
     out[6 + i] = float(S.errors[i]);
   }
 }
+// Batched instances: one thread per instance.
+__global__ void k_load_external_batch(const bm_params P, bm_state S, const float* packed) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.instances) return;
+  const float* r = packed + 49 * i;
+  st3(S.tool_pos, i, f3{r[0], r[1], r[2]});
+  for (int k = 0; k < 4; ++k) S.tool_rot[4 * i + k] = r[3 + k];
+  st3(S.tool_vel, i, f3{r[7], r[8], r[9]});
+  st3(S.tool_omega, i, f3{r[10], r[11], r[12]});
+  for (int k = 0; k < 36; ++k) S.mobility[36 * i + k] = r[13 + k];
+  st3(S.interval_impulse, i, f3{0.f, 0.f, 0.f});
+  st3(S.interval_angular, i, f3{0.f, 0.f, 0.f});
+}
+__global__ void k_store_interval_batch(const bm_params P, bm_state S, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.instances) return;
+  st3(out, 2 * i, ld3(S.interval_impulse, i));
+  st3(out, 2 * i + 1, ld3(S.interval_angular, i));
+}
+
 // validate_interval (material.py:6-20)
 __global__ void k_validate(const bm_params P, bm_state S) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1820,7 +1913,8 @@ __global__ void __launch_bounds__(kBlock) k_metrics(const bm_params P, bm_state 
     svd3(fe, U, sg, V);
     m3 R = mmul_t(U, V);
     float e = 0.f;
-    float mu = float(P.mu), lam = float(P.lam);
+    float mu, lam;
+    lame_of(P, S, p, mu, lam);
     for (int k = 0; k < 9; ++k) {
       float d = fe.a[k] - R.a[k];
       e += mu * d * d;
@@ -2119,6 +2213,20 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
 int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream) {
   if (!state_ok(p, s) || !packed49) return -1;
   hipLaunchKernelGGL(k_load_external, dim3(1), dim3(1), 0, (hipStream_t)stream, *s, packed49);
+  return launch_status();
+}
+
+int bm_load_external_batch(const bm_params* p, const bm_state* s, const float* packed, void* stream) {
+  if (!state_ok(p, s) || !packed || p->instances < 1 || !s->tool_pos || !s->mobility) return -1;
+  hipLaunchKernelGGL(k_load_external_batch, dim3(blocks_for(p->instances)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+                     *s, packed);
+  return launch_status();
+}
+
+int bm_store_interval_batch(const bm_params* p, const bm_state* s, float* out, void* stream) {
+  if (!state_ok(p, s) || !out || p->instances < 1 || !s->interval_impulse || !s->interval_angular) return -1;
+  hipLaunchKernelGGL(k_store_interval_batch, dim3(blocks_for(p->instances)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+                     *s, out);
   return launch_status();
 }
 

@@ -56,6 +56,7 @@ typedef struct bm_params {
   int64_t rigid_blocks;   /* blocks of the contact kernel (partial sums per block) */
   int64_t blade_hold;     /* external tool held over the interval: grid constraint active */
   int64_t changed_capacity; /* entries of the changed-code list (more: full pair-list rebuild) */
+  int64_t instances;      /* independent cutting instances packed in this state (1: a single solver) */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
@@ -80,6 +81,11 @@ typedef struct bm_state {
   float *interval_impulse, *interval_angular;
   int32_t *errors, *topology_dirty;
   int32_t *fp_identity;    /* [1] 1 while every Fp is exactly I (no plastic flow, no host write) */
+  /* batched instances (null for a single solver): the instance of each particle and its Lame
+   * constants; tool_pos/vel/rot/omega, mobility, reaction, torque, board, work and the
+   * interval sums are then per instance ([instances][...]) */
+  int32_t *instance;       /* [n] */
+  float *inst_mu, *inst_lam; /* [instances] */
   /* pair contacts: ping-pong history tables [2][P] and the partner lists */
   int64_t *pair_keys;      /* [2][P], -1 empty */
   float *pair_history;     /* [2][P][3] */
@@ -240,6 +246,12 @@ typedef struct bm_labels {
   int32_t *count;             /* [1] out: number of pieces */
   int32_t *errors;            /* [1] out: nonzero when a node needed more than L fields */
 } bm_labels;
+
+/* Batched instances: per-instance interval load (instances x 49 floats: position, quaternion
+ * xyzw, velocity, angular velocity, mobility) and store (instances x 6 floats: interval impulse
+ * and angular impulse).  No reference counterpart: the batched driver of force_convergence. */
+int bm_load_external_batch(const bm_params* p, const bm_state* s, const float* packed, void* stream);
+int bm_store_interval_batch(const bm_params* p, const bm_state* s, float* out, void* stream);
 
 /* Replaces topology.components (topology.py:17-57) for a solver's current codes. */
 int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream);

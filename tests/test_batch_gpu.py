@@ -1,0 +1,73 @@
+"""Batched instances against separate solvers: each instance of a BatchSolver follows the run of
+its own Solver with the same config and knife motion.
+
+Bar: cut codes bit-exact; positions within 1e-5 of the instance's extent and velocities within
+1e-3 relative (the packed instances sit at other absolute coordinates, so float32 rounding of
+positions and contact gaps differs at the 1e-8 m level); per-interval knife impulse within 1 %
+of its magnitude over the run (north_star's force bar)."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from blademaster_hip import Config, ExternalToolPort, Solver, ToolState  # noqa: E402
+from blademaster_hip.batch import BatchPort, BatchSolver, autotune_brick_z  # noqa: E402
+
+
+def configs():
+    base = dict(grid_spacing=0.006, dt=0.0002, external_tool=True, tool_substep="predict",
+                blade_half_size=(0.0005, 0.02, 0.015), cache_topology=True, cutting_energy=0.02)
+    return [Config(size=(0.048, 0.024, 0.018), young_modulus=1e5, **base),
+            Config(size=(0.036, 0.036, 0.024), young_modulus=3e5, poisson_ratio=0.35, sampling_seed=2, **base),
+            Config(size=(0.06, 0.018, 0.012), young_modulus=2e4, **base)]
+
+
+def plan(cfgs, intervals, steps):
+    """Per instance: a plunging knife with its own tilt, lateral offset and speed."""
+    rng = np.random.default_rng(3)
+    out = []
+    for cfg in cfgs:
+        tilt = np.deg2rad(rng.uniform(-15, 15))
+        q = [np.sin(tilt / 2), 0.0, 0.0, np.cos(tilt / 2)]
+        speed = rng.uniform(0.05, 0.2)
+        x0 = rng.uniform(0, cfg.grid_spacing / 2)
+        z0 = cfg.size[2] + cfg.blade_half_size[2] - 0.002
+        dt = cfg.dt * steps
+        out.append([ToolState([x0, 0.0, z0 - speed * k * dt], q, [0, 0, -speed], [0, 0, 0]) for k in range(intervals)])
+    return out
+
+
+def test_batched_instances_follow_their_own_solvers():
+    cfgs = configs()
+    intervals, steps = 6, 5
+    knives = plan(cfgs, intervals, steps)
+    batch = BatchSolver(cfgs)
+    bport = BatchPort(batch)
+    singles = [Solver(c) for c in cfgs]
+    ports = [ExternalToolPort(s) for s in singles]
+    dt = cfgs[0].dt * steps
+    for k in range(intervals):
+        rb = bport.advance([kn[k] for kn in knives], dt, cutting=True)
+        for i, (s, port) in enumerate(zip(singles, ports)):
+            r = port.advance(knives[i][k], dt, cutting=True)
+            scale = max(np.abs(r.impulse_ns).max(), 1e-12)
+            assert np.abs(rb[i].impulse_ns - r.impulse_ns).max() <= 0.01 * scale + 1e-12, (k, i)
+    for i, s in enumerate(singles):
+        np.testing.assert_array_equal(batch.instance_state(i, "codes"), s.codes.numpy())
+        x = s.x.numpy()
+        extent = np.abs(x).max()
+        assert np.abs(batch.instance_state(i, "x") - x).max() <= 1e-5 * extent, i
+        v = s.v.numpy()
+        assert np.abs(batch.instance_state(i, "v") - v).max() <= 1e-3 * max(np.abs(v).max(), 1e-6), i
+        labels, sizes = batch.instance_labels(i)
+        ref_labels, ref_sizes = s.piece_labels()
+        np.testing.assert_array_equal(labels, ref_labels)
+    assert sum(int((batch.instance_state(i, "codes") != 0).sum()) for i in range(len(cfgs))) > 0
+
+
+def test_autotuner_picks_a_candidate():
+    best, timings = autotune_brick_z(configs(), candidates=(4, 8), substeps=4)
+    assert best in (4, 8) and all(t > 0 for t in timings.values())
