@@ -187,6 +187,65 @@ def run_cpu_baseline(cfg, q, edge_z0, hz, substeps):
     return len(ref) * substeps / dt, dt
 
 
+def run_batch(args):
+    """Batched sweep of uneven instances (256 by default) in one solver: particle-substeps/s
+    over all instances, brick height chosen by the autotuner on a 16-instance subset."""
+    import torch
+    from blademaster_hip.batch import BatchPort, BatchSolver, autotune_brick_z, sweep_instances
+
+    configs, knives, gap = sweep_instances(args.instances, seed=0)
+    brick_z, timings = autotune_brick_z(configs[:16], substeps=20) if args.brick_z is None else (args.brick_z, {})
+    s = BatchSolver(configs, brick_z=brick_z, gap=gap)
+    port = BatchPort(s)
+    interval = INTERVAL_SUBSTEPS * configs[0].dt
+    total = args.steps + args.warmup
+    intervals = -(-total // INTERVAL_SUBSTEPS)
+    warm = -(-args.warmup // INTERVAL_SUBSTEPS)
+    for k in range(warm):
+        port.submit([kn(k * interval) for kn in knives], interval)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    last = None
+    for k in range(warm, intervals):
+        last = port.submit([kn(k * interval) for kn in knives], interval)
+    b.record(stream)
+    torch.cuda.synchronize()
+    steps = (intervals - warm) * INTERVAL_SUBSTEPS
+    gpu_ms = a.elapsed_time(b)
+    reactions = port.collect(last[0], interval)
+    rate = s.n * steps / (gpu_ms * 1e-3)
+    cut = int(np.count_nonzero(s.codes.numpy()))
+    return {
+        "metric": "particle-substeps/s",
+        "value": rate,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu_ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched sweep: {args.instances} uneven blocks (10-60 x 10-40 x 8-40 particles on a "
+            "1.5 mm lattice, 3 mm grid), E log-uniform 1e4-1e6 Pa, nu 0.25-0.45, knives tilted +-15 deg, "
+            "offset 0-h/2, plunging 0.02-0.2 m/s, dt 25 us, one solver",
+            "instances": args.instances,
+            "particles": s.n,
+            "grid_nodes": s.node_count,
+            "active_nodes": s.active_nodes or s.node_count,
+            "brick": [int(v) for v in s.brick],
+            "autotune_us_per_substep": {str(k): round(v * 1e6, 1) for k, v in timings.items()},
+        },
+        "cut_particles": cut,
+        "mean_abs_force_n": float(np.mean([np.abs(r.force_n).max() for r in reactions])),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +256,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_1M_block.json"))
     ap.add_argument("--brick-z", type=int, default=None, help="brick height of the transfer kernels (cells)")
+    ap.add_argument("--batch", action="store_true",
+                    help="measure the batched sweep of uneven instances instead of the 1M-particle block")
+    ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times (the calibration kernel of "
                     "HBM byte counters)")
@@ -205,6 +267,9 @@ def main():
         print("bench.py measures one GPU only: this build runs the substep on a single device; "
               "--gpus N with N > 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     r = run_gpu(args)
     s, cfg = r["solver"], r["cfg"]
     if args.copy_mib > 0:

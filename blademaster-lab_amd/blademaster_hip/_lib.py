@@ -37,6 +37,8 @@ class Params(ctypes.Structure):
         ("blade_hold", ctypes.c_int64),
         ("changed_capacity", ctypes.c_int64),
         ("instances", ctypes.c_int64),
+        ("active_bricks", ctypes.c_int64),
+        ("active_nodes", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)
         for name in (
@@ -74,7 +76,7 @@ STATE_FIELDS = [
     ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
     ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("list_heads", c_i64p),
     ("list_links", c_i32p), ("changed", c_i32p), ("changed_count", c_i32p), ("pair_mark", c_i32p),
-    ("pair_epoch", c_i32p), ("perm", c_i32p), ("p2g_partial", c_f32p), ("remap_partial", c_f32p),
+    ("pair_epoch", c_i32p), ("perm", c_i32p), ("brick_list", c_i32p), ("node_list", c_i32p), ("p2g_partial", c_f32p), ("remap_partial", c_f32p),
     ("rigid_partial", c_f32p), ("counters", ctypes.POINTER(ctypes.c_uint32)),
     ("bc_side", c_f32p), ("bc_normal", c_f32p), ("bc_lever", c_f32p), ("bc_rhs", c_f32p), ("bc_bound", c_f32p),
     ("bc_impulse", c_f32p), ("bc_extrapolated", c_f32p), ("bc_active", c_i32p), ("bc_nodes", c_i32p),

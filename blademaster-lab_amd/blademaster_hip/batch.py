@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .config import derived
+from .config import Config, derived
 from .coupling import Reaction, ToolState
 from .geometry import grid_cells, points
 from .solver import Solver
@@ -37,22 +37,26 @@ def _reach(cfg):
 def pack(configs, gap=None):
     """Side-by-side placement of the instances' reference particles on one grid.
 
-    Shelf packing in x and y with `gap` metres between neighbours (default: a knife diameter
-    plus two cells).  Each instance keeps its own lattice: its grid corner lands on a node of
-    the packed grid.  Returns the layout for `Solver` and, per instance, the offset added to
-    its own coordinates and its particle range.
+    Shelf packing in x and y with `gap` metres between neighbours: a number, or (x, y) gaps.
+    The default is a knife diameter plus two cells, safe for any knife orientation.  Each
+    instance keeps its own lattice: its grid corner lands on a node of the packed grid.
+    Returns the layout for `Solver` and, per instance, the offset added to its own coordinates
+    and its particle range.
     """
     base = configs[0]
     h = base.grid_spacing
     if gap is None:
         gap = 2.0 * max(_reach(c) for c in configs) + 2 * h
-    gap_cells = int(np.ceil(gap / h))
+    gx, gy = (gap, gap) if np.isscalar(gap) else gap
+    if min(gx, gy) < 2 * h:
+        raise ValueError("Instances need at least two empty cells between them")
+    gap_cells = np.array([int(np.ceil(gx / h)), int(np.ceil(gy / h))])
     parts = []
     for cfg in configs:
         ref, origin, cells = points(cfg.size, h, cfg.shape, cfg.sampling_seed)
         parts.append((ref, origin, np.asarray(cells)))
-    widths = np.array([p[2][0] for p in parts]) + gap_cells
-    depths = np.array([p[2][1] for p in parts]) + gap_cells
+    widths = np.array([p[2][0] for p in parts]) + gap_cells[0]
+    depths = np.array([p[2][1] for p in parts]) + gap_cells[1]
     row_width = max(int(np.ceil(np.sqrt(float(np.sum(widths * depths))))), int(widths.max()))
     corner = []
     x = y = row_depth = 0
@@ -94,6 +98,7 @@ class BatchSolver(Solver):
     """
 
     def __init__(self, configs, device="cuda:0", brick_z=None, gap=None):
+        """gap: spacing between packed instances, metres (a number or (x, y); see `pack`)."""
         configs = list(configs)
         if not configs:
             raise ValueError("No instances")
@@ -225,3 +230,39 @@ def autotune_brick_z(configs, candidates=(8, 12, 16, 24, 32), substeps=20, devic
         del port, solver
         torch.cuda.empty_cache()
     return best, timings
+
+
+def sweep_instances(count=256, seed=0, spacing=0.003, dt=25.0e-6, blade_half_size=(0.0005, 0.035, 0.02)):
+    """A batched sweep of uneven cutting instances: per instance a block of nx x ny x nz
+    particles (nx in [10, 60], ny in [10, 40], nz in [8, 40]) on the h/2 lattice, Young's
+    modulus log-uniform in [1e4, 1e6] Pa, Poisson's ratio uniform in [0.25, 0.45], and a knife
+    tilted about x by up to +-15 degrees, offset laterally by [0, h/2] and plunging at
+    [0.02, 0.2] m/s from 2 mm above the block.
+
+    Returns (configs, knife(t) per instance, gap for `pack`): the knives stay in their
+    instances' x range, so x gaps are three cells; y gaps hold the knives' overhang."""
+    rng = np.random.default_rng(seed)
+    lattice = spacing / 2
+    configs, knives, overhang = [], [], 0.0
+    for _ in range(count):
+        nx, ny, nz = int(rng.integers(10, 61)), int(rng.integers(10, 41)), int(rng.integers(8, 41))
+        size = (nx * lattice, ny * lattice, nz * lattice)
+        cfg = Config(size=size, grid_spacing=spacing, dt=dt, young_modulus=float(10 ** rng.uniform(4, 6)),
+                     poisson_ratio=float(rng.uniform(0.25, 0.45)), external_tool=True, tool_substep="predict",
+                     cache_topology=True, grid_storage="compact", blade_half_size=blade_half_size)
+        tilt = np.deg2rad(rng.uniform(-15.0, 15.0))
+        offset = rng.uniform(0.0, spacing / 2)
+        speed = rng.uniform(0.02, 0.2)
+        q = np.array([np.sin(tilt / 2), 0.0, 0.0, np.cos(tilt / 2)])
+        hy, hz = blade_half_size[1], blade_half_size[2]
+        reach_z = hz * np.cos(tilt) + hy * abs(np.sin(tilt))
+        z0 = size[2] + 0.002 + reach_z
+
+        def knife(t, q=q, z0=z0, offset=offset, speed=speed):
+            return ToolState([offset, 0.0, z0 - speed * t], q, [0.0, 0.0, -speed], [0.0, 0.0, 0.0])
+
+        configs.append(cfg)
+        knives.append(knife)
+        overhang = max(overhang, hy * abs(np.cos(tilt)) + hz * abs(np.sin(tilt)) - size[1] / 2)
+    gap = (3 * spacing, 2 * max(overhang, 0.0) + 3 * spacing)
+    return configs, knives, gap

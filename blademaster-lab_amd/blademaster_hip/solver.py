@@ -184,6 +184,17 @@ class Solver:
         T["pair_mark"] = t((max(n, 1),), i32)
         T["pair_epoch"] = t((1,), i32)
         T["perm"] = torch.tensor(self.brick_slots, dtype=i32, device=dev)
+        # Bricks with particles and nodes in some particle's stencil never change in TLMPM; the
+        # kernels walk only these when they are a strict subset (sparse shapes, packed batches).
+        nb_all = int(np.prod(self.bricks))
+        occupied = np.flatnonzero((self.brick_slots.reshape(nb_all, -1) >= 0).any(axis=1)).astype(np.int32)
+        self.active_bricks = len(occupied) if len(occupied) < nb_all else 0
+        if self.active_bricks:
+            T["brick_list"] = torch.tensor(occupied, device=dev)
+        used_nodes = np.unique(self.node_numpy).astype(np.int32) if n else np.zeros(0, np.int32)
+        self.active_nodes = len(used_nodes) if 0 < len(used_nodes) < self.node_count else 0
+        if self.active_nodes:
+            T["node_list"] = torch.tensor(used_nodes, device=dev)
         window = int(np.prod(np.asarray(self.brick) + 1))
         nb = int(np.prod(self.bricks))
         T["p2g_partial"] = t((max(nb * F_ * window * 7, 1),))
@@ -293,6 +304,8 @@ class Solver:
         p.blade_hold = int(self.blade_hold)
         p.changed_capacity = self.changed_capacity
         p.instances = getattr(self, "instances", 1)
+        p.active_bricks = self.active_bricks
+        p.active_nodes = self.active_nodes
         return p
 
     def _make_state(self):

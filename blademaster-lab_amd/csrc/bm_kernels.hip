@@ -59,6 +59,15 @@ __device__ __forceinline__ float wave_max(float v) {
 __device__ __forceinline__ bool lane0() { return (threadIdx.x & 63) == 0; }
 // Instance of particle p (0 for a single solver).
 __device__ __forceinline__ int instance_of(const bm_state& S, int64_t p) { return S.instance ? S.instance[p] : 0; }
+// Bricks holding particles and nodes touched by particles (static in TLMPM): the transfer and
+// grid kernels run over these lists; without lists, over every brick and node.
+__device__ __forceinline__ int active_brick(const bm_state& S, int k) { return S.brick_list ? S.brick_list[k] : k; }
+__device__ __forceinline__ int64_t active_node_count(const bm_params& P) {
+  return P.active_nodes > 0 ? P.active_nodes : P.node_count;
+}
+__device__ __forceinline__ int64_t active_node(const bm_state& S, int64_t i) {
+  return S.node_list ? int64_t(S.node_list[i]) : i;
+}
 
 // ------------------------------------------------------------------------ contact laws
 struct Contact {
@@ -291,9 +300,12 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   }
   if ((mode & kTopologyPrep) != 0 && S.topology_dirty[0] != 0) {
     // register, first half (solver.py:392-393): empty field table before the inserts.
-    int64_t total = P.fields * P.node_count;
-    for (int64_t i = p; i < total; i += stride) S.field_code[i] = -1;
-    for (int64_t i = p; i < P.node_count; i += stride) S.node_nf[i] = 0;
+    int64_t nodes = active_node_count(P);
+    for (int64_t i = p; i < nodes; i += stride) {
+      int64_t node = active_node(S, i);
+      for (int f = 0; f < int(P.fields); ++f) S.field_code[int64_t(f) * P.node_count + node] = -1;
+      S.node_nf[node] = 0;
+    }
   }
   if (live) {
     f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
@@ -854,9 +866,11 @@ __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p
 
 __global__ void k_topo_clear(const bm_params P, bm_state S) {
   int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  for (int64_t i = t; i < P.fields * P.node_count; i += stride) S.field_code[i] = -1;
-  for (int64_t i = t; i < P.node_count; i += stride) S.node_nf[i] = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < active_node_count(P); i += stride) {
+    int64_t node = active_node(S, i);
+    for (int f = 0; f < int(P.fields); ++f) S.field_code[int64_t(f) * P.node_count + node] = -1;
+    S.node_nf[node] = 0;
+  }
 }
 __global__ void k_topo_insert(const bm_params P, bm_state S) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -985,8 +999,8 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
-  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
-  if (flags && b == 0 && threadIdx.x == 0) {
+  int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
+  if (flags && blockIdx.x == 0 && threadIdx.x == 0) {
     // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
     if (flags & 1) S.topology_dirty[0] = 0;
     if (flags & 2) {
@@ -1138,8 +1152,9 @@ __device__ __forceinline__ void node_coords(const bm_params& P, int64_t node, in
 // old_v = p / m, new_v = (p + j) / m.
 
 __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S) {
-  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= P.node_count) return;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= active_node_count(P)) return;
+  int64_t node = active_node(S, i);
   int nf = S.node_nf[node];
   if (nf == 0) return;
   int64_t N = P.node_count;
@@ -1234,7 +1249,7 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
-  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
+  int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
   Window w = window_of(P, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -1352,8 +1367,9 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S) {
-  int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= P.node_count) return;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= active_node_count(P)) return;
+  int64_t node = active_node(S, i);
   int nf = S.node_nf[node];
   if (nf == 0) return;
   int64_t N = P.node_count;
@@ -1508,7 +1524,7 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 // overlap the staging.
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S) {
   extern __shared__ float lds[];
-  int b = brick_of_block(int(blockIdx.x), int(gridDim.x));
+  int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
   Window w = window_of(P, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -2029,6 +2045,9 @@ bool pair_args_ok(const bm_params* p, const bm_state* s) {
          p->list_buckets <= int64_t(4096) * 4096 && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
 }
 int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
+// Workgroups of a brick kernel: one per brick with particles.
+int64_t brick_launch(const bm_params* p) { return p->active_bricks > 0 ? p->active_bricks : brick_count(p); }
+int64_t node_launch(const bm_params* p) { return p->active_nodes > 0 ? p->active_nodes : p->node_count; }
 int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
 // LDS of a brick kernel: floats_per_fw floats per (field slot, window node) plus
 // floats_per_corner per (cell, corner) of the brick.
@@ -2040,6 +2059,9 @@ size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
 bool brick_args_ok(const bm_params* p, const bm_state* s) {
   if (!state_ok(p, s) || !s->perm || !s->p2g_partial || !s->remap_partial || !s->field_idx || !s->node_nf) return false;
   if (p->brick[0] < 1 || p->brick[1] < 1 || p->brick[2] < 1 || brick_count(p) < 1) return false;
+  if ((p->active_bricks > 0 && (!s->brick_list || p->active_bricks > brick_count(p))) ||
+      (p->active_nodes > 0 && (!s->node_list || p->active_nodes > p->node_count)))
+    return false;
   // One lane per particle slot, and one per window node in k_p2g's field count.
   if (8 * p->brick[0] * p->brick[1] * p->brick[2] > 1024 || window_nodes(p) > 1024) return false;
   if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
@@ -2056,7 +2078,7 @@ int brick_threads(const bm_params* p) {
   return int(t > 1024 ? 1024 : t);
 }
 void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
-  int64_t total = p->fields * p->node_count;
+  int64_t total = node_launch(p);
   hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
   if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
 }
@@ -2065,13 +2087,13 @@ void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
 // pair-list staleness check (2) on the moved particles.
 void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
                       hipStream_t st) {
-  int nb = int(brick_count(p)), tb = brick_threads(p);
+  int nb = int(brick_launch(p)), tb = brick_threads(p);
   hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags);
-  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s);
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags);
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, st, *p, *s);
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3, 0), st, *p, *s);
 }
 }  // namespace
@@ -2102,35 +2124,35 @@ int bm_register(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_p2g, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 7, 7), (hipStream_t)stream,
+  hipLaunchKernelGGL(k_p2g, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 7, 7), (hipStream_t)stream,
                      *p, *s, 0);
   return launch_status();
 }
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_g2p, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 9, 3), (hipStream_t)stream,
+  hipLaunchKernelGGL(k_g2p, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 9, 3), (hipStream_t)stream,
                      *p, *s, int64_t(1), 0);
   return launch_status();
 }
 
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(p->node_count)), dim3(kBlock), 0, (hipStream_t)stream, *p,
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
                      *s);
   return launch_status();
 }
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_update_F, dim3(int(brick_count(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 0),
+  hipLaunchKernelGGL(k_update_F, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 0),
                      (hipStream_t)stream, *p, *s);
   return launch_status();
 }

@@ -57,6 +57,8 @@ typedef struct bm_params {
   int64_t blade_hold;     /* external tool held over the interval: grid constraint active */
   int64_t changed_capacity; /* entries of the changed-code list (more: full pair-list rebuild) */
   int64_t instances;      /* independent cutting instances packed in this state (1: a single solver) */
+  int64_t active_bricks;  /* bricks holding particles (entries of brick_list; 0: every brick) */
+  int64_t active_nodes;   /* nodes some particle's stencil touches (entries of node_list; 0: all) */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
@@ -109,6 +111,8 @@ typedef struct bm_state {
   /* brick bins: bricks x-major, cells of a brick z-fastest; cell c holds the particles
    * perm[8c .. 8c+8) in reference order, -1 for unused slots (at most 8 per cell) */
   int32_t *perm;           /* [8 * bricks * cells per brick] */
+  int32_t *brick_list;     /* [active_bricks] bricks with particles, in brick order (null: all) */
+  int32_t *node_list;      /* [active_nodes] nodes of the particles' stencils, ascending (null: all) */
   float *p2g_partial;      /* [bricks][F][W][7] window sums of mass, momentum, impulse */
   float *remap_partial;    /* [bricks][F][W][3] window sums of the re-scattered momentum */
   float *rigid_partial;    /* [rigid_blocks][12] per-block contact sums */
