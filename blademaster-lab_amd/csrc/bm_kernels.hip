@@ -410,8 +410,18 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   __shared__ float wsum[kBlock / 64][kRigidSums];
   float v[kRigidSums] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z,
                          torq.x,    torq.y,    torq.z,    work,    spd,     0.0f};
+  // Most waves have no particle at the blade (and many none on the board): their reaction,
+  // torque and work sums are exact zeros and skip the reduction.
+  bool board_any = __ballot(board_j.x != 0.0f || board_j.y != 0.0f || board_j.z != 0.0f) != 0;
+  bool blade_any = __ballot(react.x != 0.0f || react.y != 0.0f || react.z != 0.0f || work != 0.0f) != 0;
+  if (board_any) {
 #pragma unroll
-  for (int k = 0; k < kRigidSums - 2; ++k) v[k] = wave_sum(v[k]);
+    for (int k = 0; k < 3; ++k) v[k] = wave_sum(v[k]);
+  }
+  if (blade_any) {
+#pragma unroll
+    for (int k = 3; k < 10; ++k) v[k] = wave_sum(v[k]);
+  }
   v[10] = wave_max(v[10]);
   int wave = threadIdx.x >> 6;
   if (lane0()) {
