@@ -121,6 +121,22 @@ class Labels(ctypes.Structure):
     _fields_ = LABELS_FIELDS
 
 
+MLP_FIELDS = [
+    ("inputs", ctypes.c_int32),
+    ("hidden", ctypes.c_int32),
+    ("outputs", ctypes.c_int32),
+] + [(name, c_f32p) for name in ("params", "grads", "adam_m", "adam_v", "h1", "h2", "out", "d_out", "d_h2", "d_h1",
+                                 "loss")] + [
+    ("step", c_i32p),
+    ("lr", c_f32p),
+    ("packed", ctypes.POINTER(ctypes.c_uint16)),
+] + [(name, c_f32p) for name in ("x_shift", "x_scale", "y_shift", "y_scale")]
+
+
+class Mlp(ctypes.Structure):
+    _fields_ = MLP_FIELDS
+
+
 MP = ctypes.POINTER(MlsParams)
 MS = ctypes.POINTER(MlsState)
 P = ctypes.POINTER(Params)
@@ -168,6 +184,14 @@ SIGNATURES = {
     "bm_piece_labels": [P, S, ctypes.POINTER(Labels), VP],
     "bm_load_external_batch": [P, S, VP, VP],
     "bm_store_interval_batch": [P, S, VP, VP],
+    "bm_mlp_sizes": [I32, I32, I32, c_i64p, c_i64p],
+    "bm_mlp_forward": [ctypes.POINTER(Mlp), VP, I32, VP],
+    "bm_mlp_backward": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
+    "bm_mlp_train_step": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
+    "bm_mlp_pack": [ctypes.POINTER(Mlp), VP],
+    "bm_mlp_infer": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
+    "bm_mlp_gemm": [VP, VP, VP, I32, I32, I32, I32, I32, I32, I32, I32, VP],
+    "bm_blade_patch": [P, S, VP, VP],
 }
 
 _lib = None

@@ -559,6 +559,28 @@ class Solver:
         self._embedded_host.copy_(self._embedded, non_blocking=True)
         return previous
 
+    def blade_patch(self, out=None):
+        """Particle count and summed damage in 8 x 8 x 8 cells around each knife edge (the
+        surrogate's patch): a [instances][2][8][8][8] device tensor, filled asynchronously."""
+        if out is None:
+            out = torch.empty(self.instances, 2, 8, 8, 8, dtype=torch.float32, device=self.device.torch)
+        if out.numel() != self.instances * 1024 or out.dtype != torch.float32 or not out.is_contiguous():
+            raise ValueError("Patch buffer must be a contiguous float32 tensor of instances x 1024 values")
+        self._call("bm_blade_patch", ctypes.c_void_p(out.data_ptr()))
+        return out
+
+    def snapshot(self):
+        """Device copy of the whole simulation state (every buffer and the host counters);
+        `restore` rewinds to it.  Buffers are copied in place, so captured graphs stay valid."""
+        return dict(tensors={k: t.clone() for k, t in self._t.items()}, step_count=self.step_count,
+                    cut_index=self.cut_index)
+
+    def restore(self, snap):
+        for k, t in self._t.items():
+            t.copy_(snap["tensors"][k])
+        self.step_count = snap["step_count"]
+        self.cut_index = snap["cut_index"]
+
     def piece_labels(self):
         """Piece labels of the current codes on the device, equal bit for bit to
         `components(node_numpy, weight_numpy, codes)` (topology.py:17-57): labels (n,)

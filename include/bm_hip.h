@@ -291,6 +291,56 @@ int bm_mls_g2p(const bm_mls_params* m, const bm_mls_state* s, void* stream);
 int bm_mls_substep(const bm_params* p, const bm_state* t, const bm_mls_params* m, const bm_mls_state* s,
                    void* stream);
 
+/* ---------------------------------------------------------------------------------
+ * Interval-wrench surrogate.  No reference counterpart: the reference's only intra-interval
+ * predictor is the mobility matrix of tool_substep "predict" (tool.py:97-111); this network
+ * predicts the knife's interval wrench from the knife pose and twist, the previous interval's
+ * wrench and a pooled occupancy/damage patch around the edge, for couplings that exchange
+ * less often than once per robot step.
+ *
+ * Network: inputs -> hidden (ReLU) -> hidden (ReLU) -> outputs.  Layer l is an augmented
+ * matrix Wa_l [rows_l + 1][cols_l], row-major, whose last row is the bias, so a layer is
+ * [x, 1] Wa.  params = Wa_1 | Wa_2 | Wa_3 (float32); grads, adam_m, adam_v have its layout.
+ * Training is float32 (MFMA 16x16x4 f32); inference is bfloat16 weights and activations with
+ * float32 accumulation (MFMA 16x16x32 bf16) from a packed copy of params. */
+typedef struct bm_mlp {
+  int32_t inputs, hidden, outputs;
+  float *params, *grads, *adam_m, *adam_v;
+  float *h1, *h2, *out;          /* activations [batch][hidden], [batch][hidden], [batch][outputs] */
+  float *d_out, *d_h2, *d_h1;    /* their gradients */
+  float *loss;                   /* [1] mean squared error of the last forward/backward */
+  int32_t *step;                 /* [1] Adam steps taken; incremented by every training step */
+  const float *lr;               /* [1] Adam learning rate */
+  uint16_t *packed;              /* bf16 inference weights; bm_mlp_sizes gives the length */
+  /* Standardisation applied by bm_mlp_infer only (each null: identity), in float32:
+   * x' = (x - x_shift) x_scale per input, y = y' y_scale + y_shift per output. */
+  const float *x_shift, *x_scale, *y_shift, *y_scale;
+} bm_mlp;
+
+/* Lengths of params and of packed for a network shape. */
+int bm_mlp_sizes(int32_t inputs, int32_t hidden, int32_t outputs, int64_t* params, int64_t* packed);
+/* out = net(x), float32; x [batch][inputs]. */
+int bm_mlp_forward(const bm_mlp* m, const float* x, int32_t batch, void* stream);
+/* Forward, mean squared error against target [batch][outputs], and grads; params unchanged. */
+int bm_mlp_backward(const bm_mlp* m, const float* x, const float* target, int32_t batch, void* stream);
+/* bm_mlp_backward followed by one Adam step (beta1 0.9, beta2 0.999, eps 1e-8). */
+int bm_mlp_train_step(const bm_mlp* m, const float* x, const float* target, int32_t batch, void* stream);
+/* packed = bf16 copy of params (transposed, zero-padded); needs hidden 256, inputs < 128,
+ * outputs <= 16. */
+int bm_mlp_pack(const bm_mlp* m, void* stream);
+/* y [batch][outputs] = net(x) from packed, one fused launch. */
+int bm_mlp_infer(const bm_mlp* m, const float* x, float* y, int32_t batch, void* stream);
+/* Raw GEMM of the training path, for tests: C [M][N] = op(A) op(B), op(A) [M][K], op(B) [K][N];
+ * trans_a: A is stored [K][M]; trans_b: B is stored [N][K]. */
+int bm_mlp_gemm(const float* A, const float* B, float* C, int32_t M, int32_t N, int32_t K, int32_t lda,
+                int32_t ldb, int32_t ldc, int32_t trans_a, int32_t trans_b, void* stream);
+
+/* Occupancy/damage patch around each knife edge (input of the surrogate).  The box spans
+ * [-4h, 4h] across the blade (local x), the blade's half length along the edge (local y) and
+ * [-4h, 4h] about the edge (local z), in 8 x 8 x 8 cells; per instance and cell, out holds the
+ * particle count and the summed damage: out [instances][2][8][8][8].  Cleared first. */
+int bm_blade_patch(const bm_params* p, const bm_state* s, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -217,7 +217,8 @@ def test_struct_layouts_match_the_header():
 
     text = open(os.path.join(ROOT, "include", "bm_hip.h")).read()
     for struct, mirror in (("bm_params", _lib.Params), ("bm_state", _lib.State), ("bm_labels", _lib.Labels),
-                           ("bm_mls_params", _lib.MlsParams), ("bm_mls_state", _lib.MlsState)):
+                           ("bm_mls_params", _lib.MlsParams), ("bm_mls_state", _lib.MlsState),
+                           ("bm_mlp", _lib.Mlp)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, flags=re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = []
@@ -238,6 +239,13 @@ def test_library_argument_errors_need_no_device():
     lib = _lib.load()
     assert lib.bm_refresh_stress(None, None, None) < 0
     assert lib.bm_substep(None, None, 0, 0, 1, 0, 0, None, None, 0, None) < 0
+    assert lib.bm_mlp_forward(None, None, 1, None) < 0
+    m = _lib.Mlp(inputs=96, hidden=128, outputs=6)
+    assert lib.bm_mlp_infer(m, None, None, 1, None) < 0
+    params, packed = ctypes.c_int64(), ctypes.c_int64()
+    assert lib.bm_mlp_sizes(96, 256, 6, ctypes.byref(params), ctypes.byref(packed)) == 0
+    assert params.value == 97 * 256 + 257 * 256 + 257 * 6
+    assert packed.value == 256 * 128 + 256 * 288 + 16 * 288
 
 
 @pytest.mark.parametrize("shape,size,spacing,fields", [("block", (0.3, 0.15, 0.075), 0.003, 4),
