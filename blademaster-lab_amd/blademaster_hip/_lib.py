@@ -130,6 +130,8 @@ MLP_FIELDS = [
     ("step", c_i32p),
     ("lr", c_f32p),
     ("packed", ctypes.POINTER(ctypes.c_uint16)),
+    ("work", c_f32p),
+    ("work_len", ctypes.c_int64),
 ] + [(name, c_f32p) for name in ("x_shift", "x_scale", "y_shift", "y_scale")]
 
 
@@ -184,13 +186,14 @@ SIGNATURES = {
     "bm_piece_labels": [P, S, ctypes.POINTER(Labels), VP],
     "bm_load_external_batch": [P, S, VP, VP],
     "bm_store_interval_batch": [P, S, VP, VP],
-    "bm_mlp_sizes": [I32, I32, I32, c_i64p, c_i64p],
+    "bm_mlp_sizes": [I32, I32, I32, c_i64p, c_i64p, c_i64p],
     "bm_mlp_forward": [ctypes.POINTER(Mlp), VP, I32, VP],
     "bm_mlp_backward": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
     "bm_mlp_train_step": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
     "bm_mlp_pack": [ctypes.POINTER(Mlp), VP],
     "bm_mlp_infer": [ctypes.POINTER(Mlp), VP, VP, I32, VP],
     "bm_mlp_gemm": [VP, VP, VP, I32, I32, I32, I32, I32, I32, I32, I32, VP],
+    "bm_mlp_gemm_split": [VP, VP, VP, I32, I32, I32, I32, I32, I32, I32, VP, I64, VP],
     "bm_blade_patch": [P, S, VP, VP],
 }
 

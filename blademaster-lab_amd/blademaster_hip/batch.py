@@ -137,72 +137,135 @@ class BatchPort:
 
     submit(states, duration) enqueues one interval for all instances: each ToolState is in
     its instance's own frame; collect() returns one Reaction per instance, with the angular
-    impulse about that frame's origin.
+    impulse about the tool origin, as `ExternalToolPort` returns it (coupling.py:181-182).
+
+    `submit_records` takes the 49-float records of all instances as one array and writes the
+    interval's result into one of `slots` result buffers, so several intervals can be in
+    flight with one readback for all of them (`collect_wrenches`).  With `patch=True` every
+    interval also leaves the occupancy/damage patch of its end state in its slot
+    (`Solver.blade_patch`).
     """
 
-    def __init__(self, solver):
+    def __init__(self, solver, slots=1, patch=False):
         self.solver = s = solver
         dev = s.device.torch
         I = s.instances
-        self.packed = torch.zeros(I, 49, dtype=torch.float32, device=dev)
-        self.result = torch.zeros(I, 6, dtype=torch.float32, device=dev)
-        self._staging = [torch.zeros(I, 49, dtype=torch.float32, pin_memory=True) for _ in range(4)]
+        self.slots = int(slots)
+        self.packed = torch.zeros(self.slots, I, 49, dtype=torch.float32, device=dev)
+        self.results = torch.zeros(self.slots, I, 6, dtype=torch.float32, device=dev)
+        self.result = self.results[0]
+        self.patches = torch.zeros(self.slots, I, 2, 8, 8, 8, dtype=torch.float32, device=dev) if patch else None
+        self._staging = [torch.zeros(I, 49, dtype=torch.float32, pin_memory=True) for _ in range(4 * self.slots)]
         self._staged = [None] * len(self._staging)
         self._stage_next = 0
 
     def _ptr(self, tensor):
         return ctypes.c_void_p(tensor.data_ptr())
 
+    def _steps(self, duration_s):
+        s = self.solver
+        steps = int(round(duration_s / s.config.dt))
+        if steps < 1 or abs(steps * s.config.dt - duration_s) > 1.0e-12 + 1.0e-10 * abs(duration_s):
+            raise ValueError("Coupling duration must be an integer number of material substeps")
+        return steps
+
     def submit(self, states, duration_s, *, cutting=True, cut_index=0):
         s = self.solver
         if len(states) != s.instances:
             raise ValueError(f"{len(states)} tool states for {s.instances} instances")
         states = [st.checked() for st in states]
-        steps = int(round(duration_s / s.config.dt))
-        if steps < 1 or abs(steps * s.config.dt - duration_s) > 1.0e-12 + 1.0e-10 * abs(duration_s):
-            raise ValueError("Coupling duration must be an integer number of material substeps")
+        records = np.stack([np.concatenate([st.position, st.orientation_xyzw, st.linear_velocity,
+                                            st.angular_velocity, np.asarray(st.mobility).ravel()]) for st in states])
+        steps = self.submit_records(records, duration_s, cutting=cutting, cut_index=cut_index)
+        return states, steps
+
+    def submit_records(self, records, duration_s, *, cutting=True, cut_index=0, slot=0):
+        """Enqueue one interval; records [instances][49] in the instances' own frames."""
+        s = self.solver
+        records = np.asarray(records, dtype=np.float64)
+        if records.shape != (s.instances, 49) or not np.isfinite(records).all():
+            raise ValueError(f"Tool records must be a finite ({s.instances}, 49) array")
+        if not 0 <= slot < self.slots:
+            raise ValueError(f"Slot {slot} outside [0, {self.slots})")
+        steps = self._steps(duration_s)
         cut_index = s._set_cut(cut_index)
-        host = np.zeros((s.instances, 49), dtype=np.float32)
-        for i, st in enumerate(states):
-            host[i] = np.concatenate([np.asarray(st.position) + s.offsets[i], st.orientation_xyzw, st.linear_velocity,
-                                      st.angular_velocity, np.asarray(st.mobility).ravel()])
+        host = records.astype(np.float32)
+        host[:, :3] += s.offsets.astype(np.float32)
         k = self._stage_next
         self._stage_next = (k + 1) % len(self._staging)
         if self._staged[k] is not None:
             self._staged[k].synchronize()
         self._staging[k].copy_(torch.from_numpy(host))
-        self.packed.copy_(self._staging[k], non_blocking=True)
+        packed, result = self.packed[slot], self.results[slot]
+        packed.copy_(self._staging[k], non_blocking=True)
         done = torch.cuda.Event()
         done.record(torch.cuda.current_stream(s.device.torch))
         self._staged[k] = done
 
         def load():
-            s.launch("bm_load_external_batch", self._ptr(self.packed))
+            s.launch("bm_load_external_batch", self._ptr(packed))
 
         def finish():
             s.launch("bm_validate_interval")
-            s.launch("bm_store_interval_batch", self._ptr(self.result))
+            s.launch("bm_store_interval_batch", self._ptr(result))
+            if self.patches is not None:
+                s.blade_patch(self.patches[slot])
 
         unused_targets = np.zeros((steps, 3), dtype=np.float32)
         _, _, graph = s.prepare(unused_targets, cutting, cut_index, before=[load], after=[finish],
-                                tag=("batch", id(self)))
+                                tag=("batch", id(self), slot))
         graph.replay()
         s.step_count += steps
-        return states, steps
+        return steps
+
+    def load_records(self, records):
+        """Set every knife's state (records as for `submit_records`) without running an interval."""
+        s = self.solver
+        host = np.asarray(records, dtype=np.float32).reshape(s.instances, 49).copy()
+        host[:, :3] += s.offsets.astype(np.float32)
+        self.packed[0].copy_(torch.from_numpy(host))
+        s.launch("bm_load_external_batch", self._ptr(self.packed[0]))
+
+    def _check_errors(self):
+        """The checks of `ExternalToolPort.collect` (coupling.py:183-188): any device error
+        count (storage overflow, inverted material, coincident proxies) invalidates the run."""
+        errors = self.solver.errors.numpy()
+        if errors.any():
+            raise RuntimeError(f"Invalid material coupling interval: GPU errors {errors.astype(int).tolist()}")
+
+    def _about_tools(self, out, positions):
+        """Per slot and instance (impulse, angular impulse about the tool origin) from the raw
+        sums about the packed domain's origin."""
+        impulse = out[..., :3]
+        tool = np.asarray(positions, dtype=np.float64) + self.solver.offsets
+        return impulse, out[..., 3:] - np.cross(tool, impulse)
 
     def collect(self, states, duration_s):
-        s = self.solver
-        errors = s.errors.numpy()
-        if errors[1]:
-            raise RuntimeError("Inverted or nonfinite material; run is invalid")
-        out = self.result.cpu().numpy().astype(np.float64)
-        reactions = []
-        for i, st in enumerate(states):
-            impulse = out[i, :3]
-            angular = out[i, 3:] - np.cross(s.offsets[i], impulse)
-            reactions.append(Reaction(duration_s=duration_s, impulse_ns=impulse, angular_impulse_nms=angular,
-                                      origin_world_m=np.asarray(st.position, dtype=float)))
-        return reactions
+        self._check_errors()
+        out = self.results[0].cpu().numpy().astype(np.float64)
+        impulse, angular = self._about_tools(out, np.stack([np.asarray(st.position, dtype=float) for st in states]))
+        return [Reaction(duration_s=duration_s, impulse_ns=impulse[i], angular_impulse_nms=angular[i],
+                         origin_world_m=np.asarray(st.position, dtype=float)) for i, st in enumerate(states)]
+
+    def collect_wrenches(self, positions, duration_s, slots=None, extra=None):
+        """Wait for the submitted intervals and return their wrenches on the knives,
+        [len(slots)][instances][6] (force 3, torque 3 about the tool origin, averaged over the
+        interval); positions [len(slots)][instances][3] are the tool origins each interval
+        started from.  `extra`, a device tensor, is read back in the same copy and returned
+        second."""
+        slots = list(range(self.slots)) if slots is None else list(slots)
+        self._check_errors()
+        raw = self.results[slots].reshape(len(slots), -1)
+        if extra is not None:
+            raw = torch.cat([raw.reshape(-1), extra.reshape(-1).to(torch.float32)])
+        host = raw.cpu().numpy().astype(np.float64)
+        n = len(slots) * self.solver.instances * 6
+        out = host[:n].reshape(len(slots), self.solver.instances, 6)
+        impulse, angular = self._about_tools(out, np.asarray(positions, dtype=np.float64).reshape(len(slots), -1, 3))
+        wrench = np.concatenate([impulse, angular], axis=-1) / duration_s
+        if extra is not None:
+            return wrench, host[n:].reshape(tuple(extra.shape))
+        return wrench
 
     def advance(self, states, duration_s, *, cutting=True, cut_index=0):
         states, _ = self.submit(states, duration_s, cutting=cutting, cut_index=cut_index)

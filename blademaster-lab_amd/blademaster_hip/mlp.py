@@ -32,15 +32,16 @@ class Mlp:
     capacity: largest batch of `forward`, `backward` and `train_step` (activation buffers).
     """
 
-    def __init__(self, inputs=96, hidden=256, outputs=6, capacity=4096, device="cuda:0", seed=0, lr=1.0e-3):
+    def __init__(self, inputs=96, hidden=256, outputs=6, capacity=4096, device="cuda:0", seed=0, lr=1.0e-3,
+                 weight_decay=0.0):
         if not torch.cuda.is_available():
             raise ValueError("CUDA device required")
         self._lib = _lib.load()
         self.inputs, self.hidden, self.outputs, self.capacity = int(inputs), int(hidden), int(outputs), int(capacity)
         self.device = torch.device(device)
-        count, packed = ctypes.c_int64(), ctypes.c_int64()
+        count, packed, work = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(self._lib.bm_mlp_sizes(self.inputs, self.hidden, self.outputs, ctypes.byref(count),
-                                          ctypes.byref(packed)), "bm_mlp_sizes")
+                                          ctypes.byref(packed), ctypes.byref(work)), "bm_mlp_sizes")
         dev, f32 = self.device, torch.float32
         self.params = torch.tensor(self.initial_params(seed), dtype=f32, device=dev)
         assert self.params.numel() == count.value
@@ -50,18 +51,20 @@ class Mlp:
         self.d_h1, self.d_h2, self.d_out = z(capacity, hidden), z(capacity, hidden), z(capacity, outputs)
         self.loss = z(1)
         self.step = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.lr = torch.full((1,), float(lr), dtype=f32, device=dev)
+        # Learning rate and decoupled weight decay of the AdamW step, read on the device.
+        self.lr = torch.tensor([float(lr), float(weight_decay)], dtype=f32, device=dev)
         self.packed = torch.zeros(packed.value, dtype=torch.int16, device=dev)
+        self.work = z(work.value)
         self.x_shift, self.x_scale = z(inputs), torch.ones(inputs, dtype=f32, device=dev)
         self.y_shift, self.y_scale = z(outputs), torch.ones(outputs, dtype=f32, device=dev)
         self._packed_current = False
-        self._graphs = {}
         m = _lib.Mlp(inputs=self.inputs, hidden=self.hidden, outputs=self.outputs)
         for name in ("params", "grads", "adam_m", "adam_v", "h1", "h2", "out", "d_out", "d_h2", "d_h1", "loss", "lr",
-                     "x_shift", "x_scale", "y_shift", "y_scale"):
+                     "x_shift", "x_scale", "y_shift", "y_scale", "work"):
             setattr(m, name, _fptr(getattr(self, name)))
         m.step = ctypes.cast(_ptr(self.step), _lib.c_i32p)
         m.packed = ctypes.cast(_ptr(self.packed), ctypes.POINTER(ctypes.c_uint16))
+        m.work_len = self.work.numel()
         self._m = m
 
     def initial_params(self, seed):
@@ -111,71 +114,104 @@ class Mlp:
         self._packed_current = False
         return self.loss
 
-    def standardize(self, X, Y):
-        """Set the input/output standardisation from a data set (mean 0, std 1 per column)."""
+    def standardize(self, X, Y, x_floor=None, y_floor=None):
+        """Set the input/output standardisation from a data set (mean 0, std 1 per column).
+
+        x_floor, y_floor: smallest standard deviation per column (a column that barely varies
+        in the data must not be amplified beyond its physical resolution)."""
         x_std = X.std(0).clamp_min(1.0e-6)
         y_std = Y.std(0).clamp_min(1.0e-9)
+        if x_floor is not None:
+            x_std = torch.maximum(x_std, torch.as_tensor(x_floor, dtype=torch.float32, device=self.device))
+        if y_floor is not None:
+            y_std = torch.maximum(y_std, torch.as_tensor(y_floor, dtype=torch.float32, device=self.device))
         self.x_shift.copy_(X.mean(0))
         self.x_scale.copy_(1.0 / x_std)
         self.y_shift.copy_(Y.mean(0))
         self.y_scale.copy_(y_std)
         self._packed_current = False
 
-    def fit(self, X, Y, epochs=10, batch=1024, lr=None, seed=0, graph=True, log=None):
-        """Minibatch Adam on the standardised data set (X, Y device tensors).
+    def loss_on(self, X, Y):
+        """Mean squared error on a data set in the standardised space (float32 forward)."""
+        total, n = 0.0, self._batch(X, self.inputs)
+        for a in range(0, n, self.capacity):
+            xs = ((X[a:a + self.capacity] - self.x_shift) * self.x_scale).contiguous()
+            out = self.forward(xs)
+            total += float(((out - (Y[a:a + self.capacity] - self.y_shift) / self.y_scale) ** 2).sum())
+        return total / max(n * self.outputs, 1)
+
+    def fit(self, X, Y, epochs=10, batch=1024, lr=None, seed=0, graph=True, log=None, lr_end=None,
+            weight_decay=None, validation=None):
+        """Minibatch AdamW on the standardised data set (X, Y device tensors).
 
         Each step gathers its shuffled rows into static buffers and replays a captured graph
-        of the training step.  Returns the mean loss of every epoch.
+        of the training step.  lr_end: cosine decay of the learning rate from lr to lr_end over
+        the epochs.  validation: (Xv, Yv); the parameters of the epoch with the lowest
+        validation loss are kept.  Returns the mean training loss of every epoch (and sets
+        `validation_history`).
         """
         n = self._batch(X, self.inputs)
         if self._batch(Y, self.outputs) != n:
             raise ValueError("X and Y differ in length")
         batch = min(int(batch), self.capacity, n)
-        if lr is not None:
-            self.lr.fill_(float(lr))
+        lr0 = float(self.lr[0]) if lr is None else float(lr)
+        if weight_decay is not None:
+            self.lr[1] = float(weight_decay)
         Xs = (X - self.x_shift) * self.x_scale
         Ys = (Y - self.y_shift) / self.y_scale
-        xb = torch.empty(batch, self.inputs, dtype=torch.float32, device=self.device)
-        yb = torch.empty(batch, self.outputs, dtype=torch.float32, device=self.device)
+        dev = self.device
+        xb = torch.empty(batch, self.inputs, dtype=torch.float32, device=dev)
+        yb = torch.empty(batch, self.outputs, dtype=torch.float32, device=dev)
+        idx = torch.arange(batch, dtype=torch.int64, device=dev)
+        total = torch.zeros(1, dtype=torch.float32, device=dev)
+
+        def step():
+            torch.index_select(Xs, 0, idx, out=xb)
+            torch.index_select(Ys, 0, idx, out=yb)
+            self.train_step(xb, yb)
+            total.add_(self.loss)
+
         step_graph = None
         if graph:
-            key = ("train", batch)
-            if key not in self._graphs:
-                xb.copy_(Xs[:batch])
-                yb.copy_(Ys[:batch])
-                # Warm-up on a side stream (required before capture); restore the state after.
-                saved = [t.clone() for t in (self.params, self.adam_m, self.adam_v, self.step)]
-                side = torch.cuda.Stream(self.device)
-                side.wait_stream(torch.cuda.current_stream(self.device))
-                with torch.cuda.stream(side):
-                    self.train_step(xb, yb)
-                torch.cuda.current_stream(self.device).wait_stream(side)
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g):
-                    self.train_step(xb, yb)
-                for t, s in zip((self.params, self.adam_m, self.adam_v, self.step), saved):
-                    t.copy_(s)
-                self._graphs[key] = (g, xb, yb)
-            step_graph, xb, yb = self._graphs[key]
-        gen = torch.Generator(device=self.device)
+            # One graph per call (it reads this call's data set): gather, forward, backward, AdamW
+            # and the loss sum.  Warm-up on a side stream first, then restore the optimiser state.
+            saved = [t.clone() for t in (self.params, self.adam_m, self.adam_v, self.step)]
+            side = torch.cuda.Stream(dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                step()
+            torch.cuda.current_stream(dev).wait_stream(side)
+            step_graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(step_graph):
+                step()
+            for t, v in zip((self.params, self.adam_m, self.adam_v, self.step), saved):
+                t.copy_(v)
+        gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
         history = []
+        self.validation_history = []
+        best = None
         steps = n // batch
         for epoch in range(epochs):
-            perm = torch.randperm(n, device=self.device, generator=gen)
-            total = torch.zeros((), dtype=torch.float32, device=self.device)
+            rate = lr0 if lr_end is None else lr_end + 0.5 * (lr0 - lr_end) * (1 + np.cos(np.pi * epoch / epochs))
+            self.lr[0] = rate
+            perm = torch.randperm(n, device=dev, generator=gen)
+            total.zero_()
             for s in range(steps):
-                idx = perm[s * batch:(s + 1) * batch]
-                torch.index_select(Xs, 0, idx, out=xb)
-                torch.index_select(Ys, 0, idx, out=yb)
+                idx.copy_(perm[s * batch:(s + 1) * batch])
                 if step_graph is not None:
                     step_graph.replay()
                 else:
-                    self.train_step(xb, yb)
-                total += self.loss[0]
-            history.append(float(total) / max(steps, 1))
+                    step()
+            history.append(float(total[0]) / max(steps, 1))
+            if validation is not None:
+                self.validation_history.append(self.loss_on(*validation))
+                if best is None or self.validation_history[-1] < best[0]:
+                    best = (self.validation_history[-1], self.params.clone())
             if log is not None:
                 log(epoch, history[-1])
+        if best is not None:
+            self.params.copy_(best[1])
         self._packed_current = False
         return history
 

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "bm_device.h"
 #include "bm_hip.h"
 
@@ -73,31 +75,35 @@ struct OperandA {
 };
 
 // C [M][N] = op(A) op(B), op(B) [K][N] (trans_b: B stored [N][K]).  relu: max(., 0);
-// mask (layout of C): zero where mask <= 0.
+// mask (layout of C): zero where mask <= 0.  Split K: workgroup z sums k in
+// [z k_chunk, (z + 1) k_chunk) into C + z split_stride.
 __global__ void __launch_bounds__(kThreads) k_gemm(OperandA A, const float* B, float* C, const float* mask, int M,
-                                                   int N, int K, int ldb, int ldc, int trans_b, int relu) {
+                                                   int N, int K, int ldb, int ldc, int trans_b, int relu, int k_chunk,
+                                                   int64_t split_stride) {
   __shared__ float As[kSlice][kPitch];
   __shared__ float Bs[kSlice][kPitch];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row0 = blockIdx.y * kTile, col0 = blockIdx.x * kTile;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int k_lo = blockIdx.z * k_chunk, k_hi = min(K, k_lo + k_chunk);
+  C += blockIdx.z * split_stride;
   floatx4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kSlice) {
+  for (int k0 = k_lo; k0 < k_hi; k0 += kSlice) {
 #pragma unroll
     for (int q = 0; q < (kSlice * kTile) / kThreads; ++q) {
       int e = tid + q * kThreads;
       // op(A) tile: read along k when A is row-major, along m when it is transposed.
       int ar = A.trans ? e % kTile : e / kSlice;
       int ak = A.trans ? e / kTile : e % kSlice;
-      As[ak][ar] = (row0 + ar < M && k0 + ak < K) ? A.at(row0 + ar, k0 + ak) : 0.0f;
+      As[ak][ar] = (row0 + ar < M && k0 + ak < k_hi) ? A.at(row0 + ar, k0 + ak) : 0.0f;
       int bc = trans_b ? e / kSlice : e % kTile;
       int bk = trans_b ? e % kSlice : e / kTile;
       float bv = 0.0f;
-      if (col0 + bc < N && k0 + bk < K)
+      if (col0 + bc < N && k0 + bk < k_hi)
         bv = trans_b ? B[int64_t(col0 + bc) * ldb + k0 + bk] : B[int64_t(k0 + bk) * ldb + col0 + bc];
       Bs[bk][bc] = bv;
     }
@@ -132,6 +138,15 @@ __global__ void __launch_bounds__(kThreads) k_gemm(OperandA A, const float* B, f
     }
 }
 
+// C[i] = sum over z of partial[z][i], in split order.
+__global__ void k_sum_splits(const float* partial, int splits, int64_t count, float* C) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.0f;
+  for (int z = 0; z < splits; ++z) s += partial[z * count + i];
+  C[i] = s;
+}
+
 // Mean squared error over n values, d = 2 (p - t) / n, in one workgroup (fixed order); thread 0
 // also counts the training step.
 constexpr int kLossThreads = 1024;
@@ -155,8 +170,10 @@ __global__ void __launch_bounds__(kLossThreads) k_mse(const float* P, const floa
   }
 }
 
-// Adam (Kingma and Ba, 2015) with bias correction; step[0] is the 1-based step of this update.
-__global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, const float* lr,
+// Adam (Kingma and Ba, 2015) with bias correction and decoupled weight decay (Loshchilov and
+// Hutter, 2019): w -= lr (m^ / (sqrt(v^) + eps) + decay w); hyper = {lr, decay}; step[0] is the
+// 1-based step of this update.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, const float* hyper,
                        const int32_t* step) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -168,7 +185,8 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, int64_t n, 
   float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  w[i] -= lr[0] * (mi / c1) / (sqrtf(vi / c2) + eps);
+  float wi = w[i];
+  w[i] = wi - hyper[0] * ((mi / c1) / (sqrtf(vi / c2) + eps) + hyper[1] * wi);
 }
 
 // Wt [cols_pad][rows_pad] = bf16(Wa [rows][cols])^T, zero padded.
@@ -316,10 +334,36 @@ bool shape_ok(const bm_mlp* m) {
   return m && m->inputs > 0 && m->hidden > 0 && m->outputs > 0 && m->params && m->h1 && m->h2 && m->out;
 }
 
+// Largest number of K splits of a weight-gradient product, and the K each split covers at least.
+constexpr int kMaxSplits = 16;
+constexpr int kSplitK = 256;
+
 void gemm(OperandA A, const float* B, float* C, const float* mask, int M, int N, int K, int ldb, int ldc, int trans_b,
           int relu, hipStream_t st) {
   dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(kThreads), 0, st, A, B, C, mask, M, N, K, ldb, ldc, trans_b, relu);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(kThreads), 0, st, A, B, C, mask, M, N, K, ldb, ldc, trans_b, relu, K,
+                     int64_t(0));
+}
+
+// C [M][N] (ldc = N) = op(A) op(B) with K split over workgroups when the tile grid alone is
+// too small to fill the device: partial products in `work`, then a fixed-order sum.
+void gemm_split(OperandA A, const float* B, float* C, int M, int N, int K, int ldb, int trans_b, float* work,
+                int64_t work_len, hipStream_t st) {
+  int tiles = ((N + kTile - 1) / kTile) * ((M + kTile - 1) / kTile);
+  int splits = std::min(kMaxSplits, std::max(1, K / kSplitK));
+  while (splits > 1 && int64_t(splits) * M * N > work_len) --splits;
+  if (!work || splits < 2 || tiles >= 256) {
+    gemm(A, B, C, nullptr, M, N, K, ldb, N, trans_b, 0, st);
+    return;
+  }
+  int chunk = (K + splits - 1) / splits;
+  chunk = (chunk + kSlice - 1) / kSlice * kSlice;
+  splits = (K + chunk - 1) / chunk;
+  dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits);
+  int64_t count = int64_t(M) * N;
+  hipLaunchKernelGGL(k_gemm, grid, dim3(kThreads), 0, st, A, B, work, nullptr, M, N, K, ldb, N, trans_b, 0, chunk,
+                     count);
+  hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(count)), dim3(256), 0, st, work, splits, count, C);
 }
 
 // out = net(x): [x,1] Wa1 -> ReLU -> [h1,1] Wa2 -> ReLU -> [h2,1] Wa3.
@@ -345,7 +389,8 @@ void backward(const bm_mlp& m, const float* x, const float* target, int batch, h
   for (int l = 2; l >= 0; --l) {
     // dWa_l = [in, 1]^T d_l: op(A) [rows + 1][batch] from in [batch][rows].
     OperandA At{in[l], L.rows[l], 1, L.rows[l], batch, 2};
-    gemm(At, d_of[l], m.grads + L.off[l], nullptr, L.rows[l] + 1, L.cols[l], batch, L.cols[l], L.cols[l], 0, 0, st);
+    gemm_split(At, d_of[l], m.grads + L.off[l], L.rows[l] + 1, L.cols[l], batch, L.cols[l], 0, m.work, m.work_len,
+               st);
     if (l > 0) {
       // d_in = (d_l Wa_l[:rows]^T) masked by ReLU(in) > 0.
       OperandA D{d_of[l], L.cols[l], 0, batch, L.cols[l], 0};
@@ -358,10 +403,13 @@ void backward(const bm_mlp& m, const float* x, const float* target, int batch, h
 
 extern "C" {
 
-int bm_mlp_sizes(int32_t inputs, int32_t hidden, int32_t outputs, int64_t* params, int64_t* packed) {
-  if (inputs < 1 || hidden < 1 || outputs < 1 || !params || !packed) return -1;
-  *params = int64_t(inputs + 1) * hidden + int64_t(hidden + 1) * hidden + int64_t(hidden + 1) * outputs;
+int bm_mlp_sizes(int32_t inputs, int32_t hidden, int32_t outputs, int64_t* params, int64_t* packed,
+                 int64_t* work) {
+  if (inputs < 1 || hidden < 1 || outputs < 1 || !params || !packed || !work) return -1;
+  int64_t l1 = int64_t(inputs + 1) * hidden, l2 = int64_t(hidden + 1) * hidden, l3 = int64_t(hidden + 1) * outputs;
+  *params = l1 + l2 + l3;
   *packed = int64_t(kHidden) * kK1 + int64_t(kHidden) * kK2 + int64_t(kOutPad) * kK2;
+  *work = kMaxSplits * std::max(l1, std::max(l2, l3));
   return 0;
 }
 
@@ -371,6 +419,15 @@ int bm_mlp_gemm(const float* A, const float* B, float* C, int32_t M, int32_t N, 
   if (M == 0 || N == 0) return 0;
   OperandA op{A, lda, trans_a ? 1 : 0, M, K, 0};
   gemm(op, B, C, nullptr, M, N, K, ldb, ldc, trans_b ? 1 : 0, 0, (hipStream_t)stream);
+  return status();
+}
+
+int bm_mlp_gemm_split(const float* A, const float* B, float* C, int32_t M, int32_t N, int32_t K, int32_t lda,
+                      int32_t ldb, int32_t trans_a, int32_t trans_b, float* work, int64_t work_len, void* stream) {
+  if (!A || !B || !C || !work || M < 0 || N < 0 || K < 0 || work_len < 0) return -1;
+  if (M == 0 || N == 0) return 0;
+  OperandA op{A, lda, trans_a ? 1 : 0, M, K, 0};
+  gemm_split(op, B, C, M, N, K, ldb, trans_b ? 1 : 0, work, work_len, (hipStream_t)stream);
   return status();
 }
 
@@ -396,8 +453,8 @@ int bm_mlp_train_step(const bm_mlp* m, const float* x, const float* target, int3
     return -1;
   hipStream_t st = (hipStream_t)stream;
   backward(*m, x, target, batch, st);
-  int64_t count = 0, packed = 0;
-  bm_mlp_sizes(m->inputs, m->hidden, m->outputs, &count, &packed);
+  int64_t count = 0, packed = 0, work = 0;
+  bm_mlp_sizes(m->inputs, m->hidden, m->outputs, &count, &packed, &work);
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(count)), dim3(256), 0, st, m->params, m->grads, m->adam_m, m->adam_v,
                      count, m->lr, m->step);
   return status();

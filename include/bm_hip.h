@@ -310,20 +310,23 @@ typedef struct bm_mlp {
   float *d_out, *d_h2, *d_h1;    /* their gradients */
   float *loss;                   /* [1] mean squared error of the last forward/backward */
   int32_t *step;                 /* [1] Adam steps taken; incremented by every training step */
-  const float *lr;               /* [1] Adam learning rate */
+  const float *lr;               /* [2] Adam learning rate, decoupled weight decay (AdamW) */
   uint16_t *packed;              /* bf16 inference weights; bm_mlp_sizes gives the length */
+  float *work;                   /* split-K partial products of the weight gradients (may be null) */
+  int64_t work_len;              /* floats in work; bm_mlp_sizes gives the largest useful length */
   /* Standardisation applied by bm_mlp_infer only (each null: identity), in float32:
    * x' = (x - x_shift) x_scale per input, y = y' y_scale + y_shift per output. */
   const float *x_shift, *x_scale, *y_shift, *y_scale;
 } bm_mlp;
 
-/* Lengths of params and of packed for a network shape. */
-int bm_mlp_sizes(int32_t inputs, int32_t hidden, int32_t outputs, int64_t* params, int64_t* packed);
+/* Lengths of params, of packed and of the largest useful work buffer for a network shape. */
+int bm_mlp_sizes(int32_t inputs, int32_t hidden, int32_t outputs, int64_t* params, int64_t* packed,
+                 int64_t* work);
 /* out = net(x), float32; x [batch][inputs]. */
 int bm_mlp_forward(const bm_mlp* m, const float* x, int32_t batch, void* stream);
 /* Forward, mean squared error against target [batch][outputs], and grads; params unchanged. */
 int bm_mlp_backward(const bm_mlp* m, const float* x, const float* target, int32_t batch, void* stream);
-/* bm_mlp_backward followed by one Adam step (beta1 0.9, beta2 0.999, eps 1e-8). */
+/* bm_mlp_backward followed by one AdamW step (beta1 0.9, beta2 0.999, eps 1e-8). */
 int bm_mlp_train_step(const bm_mlp* m, const float* x, const float* target, int32_t batch, void* stream);
 /* packed = bf16 copy of params (transposed, zero-padded); needs hidden 256, inputs < 128,
  * outputs <= 16. */
@@ -334,6 +337,10 @@ int bm_mlp_infer(const bm_mlp* m, const float* x, float* y, int32_t batch, void*
  * trans_a: A is stored [K][M]; trans_b: B is stored [N][K]. */
 int bm_mlp_gemm(const float* A, const float* B, float* C, int32_t M, int32_t N, int32_t K, int32_t lda,
                 int32_t ldb, int32_t ldc, int32_t trans_a, int32_t trans_b, void* stream);
+/* The same product with K split over workgroups (partials in work, then a fixed-order sum);
+ * ldc = N.  This is how the weight gradients are formed when the tile grid is small. */
+int bm_mlp_gemm_split(const float* A, const float* B, float* C, int32_t M, int32_t N, int32_t K, int32_t lda,
+                      int32_t ldb, int32_t trans_a, int32_t trans_b, float* work, int64_t work_len, void* stream);
 
 /* Occupancy/damage patch around each knife edge (input of the surrogate).  The box spans
  * [-4h, 4h] across the blade (local x), the blade's half length along the edge (local y) and

@@ -9,7 +9,8 @@ Pins (tests/test_surrogate_oracle.py):
 - `forward` and `backward` against torch autograd on the CPU (an independent implementation)
   and `backward` against central finite differences of `loss` in float64;
 - `adam` against its closed form on the first step (each weight moves by -lr sign(g)) and on
-  a constant gradient (the bias-corrected moments equal g and g^2 at every step);
+  a constant gradient (the bias-corrected moments equal g and g^2 at every step), and its
+  weight decay against the zero-gradient case (w shrinks by the factor 1 - lr decay);
 - `bf16` against torch's float32 -> bfloat16 conversion (round to nearest, ties to even);
 - `patch` against a brute-force count over the cells on tiny inputs.
 
@@ -78,13 +79,14 @@ def backward(params, x, target, inputs, hidden, outputs):
     return value, np.concatenate([g1.ravel(), g2.ravel(), g3.ravel()])
 
 
-def adam(params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1.0e-8):
-    """One Adam update; step is 1-based.  Returns (params, m, v) as new arrays."""
+def adam(params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1.0e-8, weight_decay=0.0):
+    """One Adam update with decoupled weight decay (AdamW, Loshchilov and Hutter 2019,
+    Algorithm 2 with schedule multiplier 1); step is 1-based.  Returns (params, m, v)."""
     m = beta1 * m + (1 - beta1) * grad
     v = beta2 * v + (1 - beta2) * grad * grad
     mhat = m / (1 - beta1 ** step)
     vhat = v / (1 - beta2 ** step)
-    return params - lr * mhat / (np.sqrt(vhat) + eps), m, v
+    return params - lr * (mhat / (np.sqrt(vhat) + eps) + weight_decay * params), m, v
 
 
 def bf16(values):

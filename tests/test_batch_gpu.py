@@ -55,6 +55,10 @@ def test_batched_instances_follow_their_own_solvers():
             r = port.advance(knives[i][k], dt, cutting=True)
             scale = max(np.abs(r.impulse_ns).max(), 1e-12)
             assert np.abs(rb[i].impulse_ns - r.impulse_ns).max() <= 0.01 * scale + 1e-12, (k, i)
+            # Angular impulse about the tool origin, as ExternalToolPort reports it (lever <= 0.05 m).
+            ang_tol = 0.01 * (np.abs(r.angular_impulse_nms).max() + 0.05 * scale) + 1e-12
+            assert np.abs(rb[i].angular_impulse_nms - r.angular_impulse_nms).max() <= ang_tol, (k, i)
+            np.testing.assert_array_equal(rb[i].origin_world_m, r.origin_world_m)
     for i, s in enumerate(singles):
         np.testing.assert_array_equal(batch.instance_state(i, "codes"), s.codes.numpy())
         x = s.x.numpy()

@@ -242,8 +242,9 @@ def test_library_argument_errors_need_no_device():
     assert lib.bm_mlp_forward(None, None, 1, None) < 0
     m = _lib.Mlp(inputs=96, hidden=128, outputs=6)
     assert lib.bm_mlp_infer(m, None, None, 1, None) < 0
-    params, packed = ctypes.c_int64(), ctypes.c_int64()
-    assert lib.bm_mlp_sizes(96, 256, 6, ctypes.byref(params), ctypes.byref(packed)) == 0
+    params, packed, work = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    assert lib.bm_mlp_sizes(96, 256, 6, ctypes.byref(params), ctypes.byref(packed), ctypes.byref(work)) == 0
+    assert work.value == 16 * 257 * 256
     assert params.value == 97 * 256 + 257 * 256 + 257 * 6
     assert packed.value == 256 * 128 + 256 * 288 + 16 * 288
 

@@ -40,6 +40,24 @@ def test_gemm_matches_float64(M, N, K, ta, tb):
     assert np.all(np.abs(got - expected) <= 1e-5 * scale + 1e-30)
 
 
+@pytest.mark.parametrize("M,N,K,work", [(257, 256, 4096, 16 * 257 * 256), (97, 256, 1000, 16 * 97 * 256),
+                                         (257, 6, 4096, 3 * 257 * 6), (33, 17, 300, 0)])
+def test_split_gemm_matches_float64(M, N, K, work):
+    """Weight-gradient form (A transposed, K = batch), K split over workgroups."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.normal(size=(K, M)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    expected = A.T.astype(np.float64) @ B.astype(np.float64)
+    net = Mlp(8, 16, 2, capacity=4, device=DEV)
+    C = torch.full((M, N), np.nan, dtype=torch.float32, device=DEV)
+    W = torch.zeros(max(work, 1), dtype=torch.float32, device=DEV)
+    code = net._lib.bm_mlp_gemm_split(*(net_ptr(t) for t in (_dev(A), _dev(B), C)), M, N, K, M, N, 1, 0,
+                                      net_ptr(W), work, net._stream())
+    assert code == 0
+    scale = np.abs(A.T).astype(np.float64) @ np.abs(B).astype(np.float64)
+    assert np.all(np.abs(C.cpu().numpy() - expected) <= 1e-5 * scale + 1e-30)
+
+
 def net_ptr(t):
     import ctypes
     return ctypes.c_void_p(t.data_ptr())
@@ -78,15 +96,15 @@ def test_forward_and_gradients_match_oracle():
 
 def test_adam_steps_match_oracle():
     net, x, t = _net_and_data(batch=128, seed=1)
-    lr = 1e-3
-    net.lr.fill_(lr)
+    lr, decay = 1e-3, 0.05
+    net.lr.copy_(torch.tensor([lr, decay]))
     params = net.params.cpu().numpy().astype(np.float64)
     m = np.zeros_like(params)
     v = np.zeros_like(params)
     xs, ts = _dev(x), _dev(t)
     for step in range(1, 4):
         _, grad = o.backward(params, x, t, *SHAPE)
-        params, m, v = o.adam(params, grad, m, v, step, lr)
+        params, m, v = o.adam(params, grad, m, v, step, lr, weight_decay=decay)
         net.train_step(xs, ts)
     assert int(net.step.cpu()[0]) == 3
     got = net.params.cpu().numpy()

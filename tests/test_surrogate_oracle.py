@@ -62,6 +62,9 @@ def test_adam_closed_forms():
         prev = w1
         w1, m, v = o.adam(w1, g, m, v, step, lr)
         np.testing.assert_allclose(w1 - prev, -lr * g / (np.abs(g) + 1e-8), rtol=1e-9)
+    # Zero gradient: only the decoupled decay acts, w <- (1 - lr decay) w.
+    w2, _, _ = o.adam(w, np.zeros(50), np.zeros(50), np.zeros(50), 3, lr, weight_decay=0.1)
+    np.testing.assert_allclose(w2, (1 - lr * 0.1) * w, rtol=1e-12)
 
 
 def test_bf16_matches_torch():
