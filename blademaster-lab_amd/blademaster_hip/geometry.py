@@ -32,6 +32,13 @@ def levelset(x, size, shape):
         radial = np.hypot(x[..., 0], x[..., 1]) - min(size[0], size[1]) / 2
         axial = np.abs(x[..., 2] - size[2] / 2) - size[2] / 2
         return np.maximum(radial, axial)
+    if shape == "horizontal_cylinder":
+        # Axis along x through (y = 0, z = r), lying on the board; r = min(sy, sz) / 2.  Not in
+        # the reference, whose cylinder stands on its end: the three sequential cuts of a log
+        # need the axis along x.
+        r = min(size[1], size[2]) / 2
+        radial = np.hypot(x[..., 1], x[..., 2] - r) - r
+        return np.maximum(np.abs(x[..., 0]) - size[0] / 2, radial)
     if shape == "banana":
         u = x[..., 0] / (size[0] / 2)
         centre_y = 0.035 * (u * u - 0.5)
@@ -74,7 +81,7 @@ def points(size, spacing, shape, seed=0):
     x = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
     if seed:
         x = x + np.random.default_rng(seed).uniform(-spacing / 4, spacing / 4, size=x.shape)
-    if shape in ("banana", "cylinder"):
+    if shape in ("banana", "cylinder", "horizontal_cylinder"):
         x = x[levelset(x, size, shape) <= 0]
     elif shape == "block":
         # Partial cells at the far faces when the spacing does not divide the size.
