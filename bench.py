@@ -246,6 +246,121 @@ def run_batch(args):
     }
 
 
+# MLS-MPM baseline (mlsmpm.py): x, v 24 r + 24 w; C, F 72 r + 72 w; damage and its
+# reference 8 r; friction histories 24 r + 24 w = 248 B per particle-substep, charged as above:
+# contact reads x, v and reads/writes the histories; P2G reads C, F and damage; G2P writes
+# x, v, C, F and reads the damage reference.
+MLS_ALG_BYTES = {"k_rigid": 72, "k_mls_p2g": 76, "k_mls_g2p": 100}
+MLS_SUBSTEP_BYTES = 248
+# Node updates per particle of the MLS P2G: 27 nodes x (mass + 3 momentum components), summed
+# per workgroup in LDS before one global atomic per node and component.
+MLS_P2G_UPDATES = 108
+
+
+def run_mls(args):
+    """The same 1M-particle block and knife through the MLS-MPM baseline: the knife pose of
+    each 1 ms interval held over its 40 substeps (MlsSolver.advance, mlsmpm.py:395-426)."""
+    import ctypes
+    import torch
+    from blademaster_hip import ToolState
+    from blademaster_hip.mlsmpm import MlsSolver
+
+    cfg, q, edge_z0, hz, E = workload()
+    s = MlsSolver(cfg)
+    interval = INTERVAL_SUBSTEPS * cfg.dt
+    vel = np.array([0.0, 0.0, -0.1])
+
+    def advance(k, check=True):
+        t = k * interval
+        return s.advance(ToolState(pose_at(t, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0]), interval, check=check)
+
+    total = args.steps + args.warmup
+    intervals = -(-total // INTERVAL_SUBSTEPS)
+    warm = -(-args.warmup // INTERVAL_SUBSTEPS)
+    for k in range(warm):
+        advance(k)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    gpu_ms = 0.0
+    for k in range(warm, intervals):
+        # Each interval ends with the reaction's readback: time the device work of each
+        # interval and add them (the domain check runs once, after the timed intervals).
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        advance(k, check=False)
+        b.record(stream)
+        torch.cuda.synchronize()
+        gpu_ms += a.elapsed_time(b)
+    steps = (intervals - warm) * INTERVAL_SUBSTEPS
+    rate = s.n * steps / (gpu_ms * 1e-3)
+    lost = s.diagnostics()["left_domain"]
+    mp, ms = ctypes.byref(s._mls_params), ctypes.byref(s._mls_state)
+    lib = s._lib
+    entries = {
+        "k_rigid": lambda st: lib.bm_rigid_contacts(ctypes.byref(s._params), ctypes.byref(s._state), 0, st),
+        "k_mls_p2g": lambda st: lib.bm_mls_p2g(mp, ms, st),
+        "k_mls_grid": lambda st: lib.bm_mls_grid_update(mp, ms, st),
+        "k_mls_g2p": lambda st: lib.bm_mls_g2p(mp, ms, st),
+    }
+    saved = {k: t.clone() for k, t in s._t.items()}
+    per_kernel_ms = {}
+    for name, fn in entries.items():
+        st = ctypes.c_void_p(stream.cuda_stream)
+        fn(st)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cap = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            for _ in range(args.kernel_reps):
+                # P2G adds into the grid: clear it between launches as the grid update does.
+                fn(cap)
+                if name == "k_mls_p2g":
+                    lib.bm_mls_grid_update(mp, ms, cap)
+        g.replay()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        g.replay()
+        b.record(stream)
+        torch.cuda.synchronize()
+        per_kernel_ms[name] = a.elapsed_time(b) / args.kernel_reps
+        del g
+        for k, t in saved.items():
+            s._t[k].copy_(t)
+    per_kernel_ms["k_mls_p2g"] -= per_kernel_ms["k_mls_grid"]
+    dominant = max(MLS_ALG_BYTES, key=lambda k: per_kernel_ms[k])
+    achieved = MLS_ALG_BYTES[dominant] * s.n / (per_kernel_ms[dominant] * 1e-3) / 1e9
+    atomics = MLS_P2G_UPDATES * s.n / (per_kernel_ms["k_mls_p2g"] * 1e-3)
+    return {
+        "metric": "particle-substeps/s",
+        "value": rate,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu_ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "MLS-MPM baseline: block 200x100x50 = 1M particles, 1.5 mm lattice, 3 mm dense grid, "
+            "blade 10 deg tilt re-posed each 1 ms (0.1 m/s plunge), dt 25 us",
+            "particles": s.n,
+            "grid_nodes": int(np.prod(s.dims)),
+            "young_modulus": E,
+        },
+        "hbm_peak_frac": MLS_SUBSTEP_BYTES * rate / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_particle": MLS_ALG_BYTES[dominant]},
+        "p2g_node_updates_per_s": atomics,
+        "kernel_ms": {k: round(v, 5) for k, v in per_kernel_ms.items()},
+        "left_domain": lost,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +374,7 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="measure the batched sweep of uneven instances instead of the 1M-particle block")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--mls", action="store_true", help="measure the MLS-MPM baseline on the 1M-particle block")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times (the calibration kernel of "
                     "HBM byte counters)")
@@ -269,6 +385,9 @@ def main():
         sys.exit(2)
     if args.batch:
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.mls:
+        print(json.dumps(run_mls(args)), flush=True)
         return
     r = run_gpu(args)
     s, cfg = r["solver"], r["cfg"]
