@@ -900,13 +900,37 @@ __device__ __forceinline__ int field_of(uint2 idx, int a) {
 // the window to a per-brick partial buffer with plain stores.  The grid kernels gather, for
 // every node, the windows of the (at most 8) bricks that contain it in a fixed order: P2G and
 // the MUSL re-scatter need no global atomics and their sums are deterministic.
+// Division by launch constants without the integer-divide expansion (about 15 VALU per
+// quotient on gfx950): floor(n / d) = mulhi(n, m) with m = ceil(2^32 / d) is exact for
+// n < 2^16 (Div32), and with m = ceil(2^64 / d) and a 64-bit high product for n < 2^32,
+// d < 2^32 (Div64); d = 1 is stored as m = 0.  The host fills them per launch (divisors_of).
+struct Div32 {
+  uint32_t m, d;
+};
+struct Div64 {
+  uint64_t m;
+  uint32_t d;
+};
+__device__ __forceinline__ int quot(const Div32& v, int n) { return v.m ? int(__umulhi(uint32_t(n), v.m)) : n; }
+__device__ __forceinline__ int64_t quot(const Div64& v, int64_t n) {
+  return v.m ? int64_t(__umul64hi(uint64_t(n), v.m)) : n;
+}
+struct Divisors {
+  Div32 bx, by, bz;   // brick cells per axis (node coordinate -> brick)
+  Div32 bybz;         // cells per brick column: cell index -> x within the brick
+  Div32 wz, wywz;     // window nodes along z, per x plane: window index -> (x, y, z)
+  Div64 nbz, nbynbz;  // bricks along z, per x plane: brick index -> (x, y, z)
+  Div64 dz, dyz;      // grid nodes along z, per x plane: node -> (x, y, z)
+};
+
 struct Window {
   int x0, y0, z0;
   int wy, wz, W;
 };
-__device__ __forceinline__ Window window_of(const bm_params& P, int b) {
+__device__ __forceinline__ Window window_of(const bm_params& P, const Divisors& D, int b) {
   int nby = int(P.bricks[1]), nbz = int(P.bricks[2]);
-  int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nby * nbz);
+  int qz = int(quot(D.nbz, b)), bx = int(quot(D.nbynbz, b));
+  int bz = b - qz * nbz, by = qz - bx * nby;
   Window w;
   w.x0 = bx * int(P.brick[0]);
   w.y0 = by * int(P.brick[1]);
@@ -929,9 +953,17 @@ __device__ __forceinline__ int window_base(const bm_params& P, const Window& w, 
 __device__ __forceinline__ int window_corner(const Window& w, int a) {
   return ((a >> 2) & 1) * w.wy * w.wz + ((a >> 1) & 1) * w.wz + (a & 1);
 }
+// (x, y, z) of window index i.
+__device__ __forceinline__ void window_coords(const Window& w, const Divisors& D, int i, int& ix, int& iy, int& iz) {
+  int qz = quot(D.wz, i);
+  ix = quot(D.wywz, i);
+  iz = i - qz * w.wz;
+  iy = qz - ix * w.wy;
+}
 // Global node of window index i, or -1 outside the grid.
-__device__ __forceinline__ int64_t window_node(const bm_params& P, const Window& w, int i) {
-  int iz = i % w.wz, iy = (i / w.wz) % w.wy, ix = i / (w.wz * w.wy);
+__device__ __forceinline__ int64_t window_node(const bm_params& P, const Window& w, const Divisors& D, int i) {
+  int ix, iy, iz;
+  window_coords(w, D, i, ix, iy, iz);
   int nx = w.x0 + ix, ny = w.y0 + iy, nz = w.z0 + iz;
   if (nx >= P.dims[0] || ny >= P.dims[1] || nz >= P.dims[2]) return -1;
   return (int64_t(nx) * P.dims[1] + ny) * P.dims[2] + nz;
@@ -959,9 +991,10 @@ __device__ __forceinline__ int brick_of_block(int blk, int nb) {
 }
 
 // Window index of corner 0 of the c-th cell of a brick (cells z-fastest).
-__device__ __forceinline__ int cell_window_base(const bm_params& P, const Window& w, int c) {
+__device__ __forceinline__ int cell_window_base(const bm_params& P, const Window& w, const Divisors& D, int c) {
   int bz = int(P.brick[2]), by = int(P.brick[1]);
-  int lz = c % bz, ly = (c / bz) % by, lx = c / (bz * by);
+  int qz = quot(D.bz, c), lx = quot(D.bybz, c);
+  int lz = c - qz * bz, ly = qz - lx * by;
   return (lx * w.wy + ly) * w.wz + lz;
 }
 __device__ __forceinline__ int cells_per_brick(const bm_params& P) { return int(P.brick[0] * P.brick[1] * P.brick[2]); }
@@ -983,10 +1016,11 @@ __device__ __forceinline__ bool cell_occupied(bool live) {
 // Sum over the (up to 8) cells of the brick around window node wi of their corner entries
 // cc[k * CS + 8 c + a], in fixed corner order.
 template <int K>
-__device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w, const float* cc, int CS, int wi,
-                                             float* out) {
+__device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w, const Divisors& D, const float* cc,
+                                             int CS, int wi, float* out) {
   int bx = int(P.brick[0]), by = int(P.brick[1]), bz = int(P.brick[2]);
-  int iz = wi % w.wz, iy = (wi / w.wz) % w.wy, ix = wi / (w.wz * w.wy);
+  int ix, iy, iz;
+  window_coords(w, D, wi, ix, iy, iz);
 #pragma unroll
   for (int k = 0; k < K; ++k) out[k] = 0.0f;
 #pragma unroll
@@ -1006,7 +1040,7 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // stores; a particle on another field slot (a cut through the node) adds its values to an LDS
 // node window with atomics.  Each window node then gathers its cells' corner sums in fixed
 // order, adds the window, and the brick's partial goes out with plain stores.
-__global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags) {
+__global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
@@ -1019,7 +1053,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       S.pair_epoch[0] += 1;
     }
   }
-  Window w = window_of(P, b);
+  Window w = window_of(P, D, b);
   int F = int(P.fields);
   int FW = F * w.W;
   int cpb = cells_per_brick(P);
@@ -1030,7 +1064,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   int c = slot >> 3;
   int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
   bool live = p >= 0;
-  int64_t my_node = int(threadIdx.x) < w.W ? window_node(P, w, int(threadIdx.x)) : -1;
+  int64_t my_node = int(threadIdx.x) < w.W ? window_node(P, w, D, int(threadIdx.x)) : -1;
   int my_nf = my_node >= 0 ? S.node_nf[my_node] : 0;
   if (F > 1) {
     for (int k = 0; k < 7; ++k)
@@ -1053,7 +1087,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       lp = ld3(S.local, p);
       fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
     }
-    int wb = cell_window_base(P, w, c < cpb ? c : 0);
+    int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
 #pragma unroll 2
     for (int a = 0; a < 8; ++a) {
       int f = field_of(fidx, a);
@@ -1088,7 +1122,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
     float sum[7];
     if (i < w.W) {
-      gather_cells<7>(P, w, cc, CS, i, sum);
+      gather_cells<7>(P, w, D, cc, CS, i, sum);
     } else {
 #pragma unroll
       for (int k = 0; k < 7; ++k) sum[k] = 0.0f;
@@ -1103,23 +1137,23 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
 }
 // Bricks along one axis whose window holds node coordinate c: c / B, and c / B - 1 when c
 // is on a brick boundary.
-__device__ __forceinline__ int bricks_at(int c, int B, int nb, int* out) {
-  int k = 0, b0 = c / B;
+__device__ __forceinline__ int bricks_at(int c, const Div32& B, int nb, int* out) {
+  int k = 0, b0 = quot(B, c);
   if (b0 < nb) out[k++] = b0;
-  if (c % B == 0 && b0 > 0) out[k++] = b0 - 1;
+  if (c == b0 * int(B.d) && b0 > 0) out[k++] = b0 - 1;
   return k;
 }
 
 // Sum of the window entries of node (nx, ny, nz), field f, components [0, C) over its (at
 // most 2 x 2 x 2) bricks, in brick order.  All loads are issued before the first add.
 template <int C>
-__device__ __forceinline__ void gather_node(const bm_params& P, const float* partial, int nx, int ny, int nz, int f,
-                                            float* out) {
+__device__ __forceinline__ void gather_node(const bm_params& P, const Divisors& D, const float* partial, int nx, int ny,
+                                            int nz, int f, float* out) {
   int bx[2], by[2], bz[2];
   int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
-  int kx = bricks_at(nx, BX, int(P.bricks[0]), bx);
-  int ky = bricks_at(ny, BY, int(P.bricks[1]), by);
-  int kz = bricks_at(nz, BZ, int(P.bricks[2]), bz);
+  int kx = bricks_at(nx, D.bx, int(P.bricks[0]), bx);
+  int ky = bricks_at(ny, D.by, int(P.bricks[1]), by);
+  int kz = bricks_at(nz, D.bz, int(P.bricks[2]), bz);
   int wy = BY + 1, wz = BZ + 1, W = (BX + 1) * wy * wz;
   int F = int(P.fields);
   float v[8][C];
@@ -1149,19 +1183,20 @@ __device__ __forceinline__ void gather_node(const bm_params& P, const float* par
     }
 }
 
-__device__ __forceinline__ void node_coords(const bm_params& P, int64_t node, int& nx, int& ny, int& nz) {
-  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
-  nx = int(node / dyz);
-  int rem = int(node - int64_t(nx) * dyz);
-  ny = rem / dz;
-  nz = rem - ny * dz;
+__device__ __forceinline__ void node_coords(const bm_params& P, const Divisors& D, int64_t node, int& nx, int& ny,
+                                            int& nz) {
+  int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
+  nx = int(quot(D.dyz, node));
+  int64_t rem = node - int64_t(nx) * dyz;
+  ny = int(quot(D.dz, rem));
+  nz = int(rem - int64_t(ny) * dz);
 }
 
 // Compatible-field exchange of one node (Eq. 8, material.py:252-285) on the gathered P2G
 // sums: x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b; then
 // old_v = p / m, new_v = (p + j) / m.
 
-__global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S) {
+__global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S, const Divisors D) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
@@ -1169,11 +1204,11 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   if (nf == 0) return;
   int64_t N = P.node_count;
   int nx, ny, nz;
-  node_coords(P, node, nx, ny, nz);
+  node_coords(P, D, node, nx, ny, nz);
   constexpr int kMaxLocal = 4;
   if (nf == 1) {
     float g[7];
-    gather_node<7>(P, S.p2g_partial, nx, ny, nz, 0, g);
+    gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, 0, g);
     S.grid_mass[node] = g[0];
     if (g[0] > 0.0f) {
       f3 pm = f3{g[1], g[2], g[3]}, j = f3{g[4], g[5], g[6]};
@@ -1191,7 +1226,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
 #pragma unroll
     for (int f = 0; f < kMaxLocal; ++f) {
       if (f < nf) {
-        gather_node<7>(P, S.p2g_partial, nx, ny, nz, f, g[f]);
+        gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, f, g[f]);
         code[f] = S.field_code[int64_t(f) * N + node];
         S.grid_mass[int64_t(f) * N + node] = g[f][0];
         total += g[f][0];
@@ -1220,7 +1255,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   float total = 0.0f;
   for (int f = 0; f < nf; ++f) {
     float g[7];
-    gather_node<7>(P, S.p2g_partial, nx, ny, nz, f, g);
+    gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, f, g);
     int64_t s = int64_t(f) * N + node;
     S.grid_mass[s] = g[0];
     st3(S.momentum, s, f3{g[1], g[2], g[3]});
@@ -1256,11 +1291,12 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 // particle is also labelled (assign, contact.py:996-1039) and checked against the pair
 // list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.  The
 // particle loads are issued before the window staging so the two latencies overlap.
-__global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags) {
+__global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags,
+                                              const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
-  Window w = window_of(P, b);
+  Window w = window_of(P, D, b);
   int F = int(P.fields);
   int FW = F * w.W;
   int cpb = cells_per_brick(P);
@@ -1283,7 +1319,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   }
   int my_nf = 0;
   for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
-    int64_t node = window_node(P, w, i);
+    int64_t node = window_node(P, w, D, i);
     int nf = node >= 0 ? S.node_nf[node] : 0;
     my_nf = max(my_nf, nf);
     for (int f = 0; f < F; ++f) {
@@ -1310,7 +1346,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   int lane = int(threadIdx.x & 63);
   float mine[3] = {0.f, 0.f, 0.f};
   if (__ballot(live) != 0) {
-    int wb = cell_window_base(P, w, c < cpb ? c : 0);
+    int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
     f3 vp = f3{0.f, 0.f, 0.f};
     if (live) {
       float wt[8];
@@ -1365,7 +1401,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   float* out = S.remap_partial + int64_t(b) * FW * 3;
   for (int i = threadIdx.x; i < Fb * w.W; i += blockDim.x) {
     float sum[3] = {0.f, 0.f, 0.f};
-    if (i < w.W) gather_cells<3>(P, w, cc, CS, i, sum);
+    if (i < w.W) gather_cells<3>(P, w, D, cc, CS, i, sum);
     sum[0] += win[i];
     sum[1] += win[FW + i];
     sum[2] += win[2 * FW + i];
@@ -1376,7 +1412,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
 }
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
-__global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S) {
+__global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
@@ -1384,10 +1420,10 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
   if (nf == 0) return;
   int64_t N = P.node_count;
   int nx, ny, nz;
-  node_coords(P, node, nx, ny, nz);
+  node_coords(P, D, node, nx, ny, nz);
   if (nf == 1) {
     float g[3];
-    gather_node<3>(P, S.remap_partial, nx, ny, nz, 0, g);
+    gather_node<3>(P, D, S.remap_partial, nx, ny, nz, 0, g);
     float m = S.grid_mass[node];
     f3 vel = f3{0.f, 0.f, 0.f};
     if (m > 0.0f) vel = f3{g[0], g[1], g[2]} / m;
@@ -1402,7 +1438,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
 #pragma unroll
     for (int f = 0; f < kMaxLocal; ++f) {
       if (f < nf) {
-        gather_node<3>(P, S.remap_partial, nx, ny, nz, f, g[f]);
+        gather_node<3>(P, D, S.remap_partial, nx, ny, nz, f, g[f]);
         m[f] = S.grid_mass[int64_t(f) * N + node];
         code[f] = S.field_code[int64_t(f) * N + node];
         total += m[f];
@@ -1431,7 +1467,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
   float total = 0.0f;
   for (int f = 0; f < nf; ++f) {
     float g[3];
-    gather_node<3>(P, S.remap_partial, nx, ny, nz, f, g);
+    gather_node<3>(P, D, S.remap_partial, nx, ny, nz, f, g);
     int64_t s = int64_t(f) * N + node;
     st3(S.remapped, s, f3{g[0], g[1], g[2]});
     total += S.grid_mass[s];
@@ -1532,10 +1568,10 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 // update_F_plastic (material.py:445-475) for one brick, one lane per particle slot, with the
 // field velocities of the window staged in LDS; the particle loads are issued first so they
 // overlap the staging.
-__global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S) {
+__global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D) {
   extern __shared__ float lds[];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
-  Window w = window_of(P, b);
+  Window w = window_of(P, D, b);
   int F = int(P.fields);
   int FW = F * w.W;
   int64_t N = P.node_count;
@@ -1559,7 +1595,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
     }
   }
   for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
-    int64_t node = window_node(P, w, i);
+    int64_t node = window_node(P, w, D, i);
     int nf = node >= 0 ? S.node_nf[node] : 0;
     for (int f = 0; f < F; ++f) {
       f3 v = f3{0.f, 0.f, 0.f};
@@ -1573,7 +1609,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
   __syncthreads();
   if (p < 0) return;
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  int wb = cell_window_base(P, w, slot >> 3);
+  int wb = cell_window_base(P, w, D, slot >> 3);
   m3 rate = mzero();
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -2059,6 +2095,23 @@ int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p
 int64_t brick_launch(const bm_params* p) { return p->active_bricks > 0 ? p->active_bricks : brick_count(p); }
 int64_t node_launch(const bm_params* p) { return p->active_nodes > 0 ? p->active_nodes : p->node_count; }
 int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
+Div32 div32(int64_t d) { return Div32{d > 1 ? uint32_t(0xFFFFFFFFu / uint32_t(d)) + 1u : 0u, uint32_t(d)}; }
+Div64 div64(int64_t d) { return Div64{d > 1 ? ~uint64_t(0) / uint64_t(d) + 1u : 0u, uint32_t(d)}; }
+// Division constants of a launch (the index ranges they need are checked by brick_args_ok).
+Divisors divisors_of(const bm_params* p) {
+  Divisors D;
+  D.bx = div32(p->brick[0]);
+  D.by = div32(p->brick[1]);
+  D.bz = div32(p->brick[2]);
+  D.bybz = div32(p->brick[1] * p->brick[2]);
+  D.wz = div32(p->brick[2] + 1);
+  D.wywz = div32((p->brick[1] + 1) * (p->brick[2] + 1));
+  D.nbz = div64(p->bricks[2]);
+  D.nbynbz = div64(p->bricks[1] * p->bricks[2]);
+  D.dz = div64(p->dims[2]);
+  D.dyz = div64(p->dims[1] * p->dims[2]);
+  return D;
+}
 // LDS of a brick kernel: floats_per_fw floats per (field slot, window node) plus
 // floats_per_corner per (cell, corner) of the brick.
 size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
@@ -2074,6 +2127,11 @@ bool brick_args_ok(const bm_params* p, const bm_state* s) {
     return false;
   // One lane per particle slot, and one per window node in k_p2g's field count.
   if (8 * p->brick[0] * p->brick[1] * p->brick[2] > 1024 || window_nodes(p) > 1024) return false;
+  // Ranges of the division constants: node coordinates below 2^16 (Div32 of the brick
+  // sizes), brick and node indices below 2^32 (Div64).
+  if (p->dims[0] >= 65536 || p->dims[1] >= 65536 || p->dims[2] >= 65536 || brick_count(p) >= (int64_t(1) << 32) ||
+      p->node_count >= (int64_t(1) << 32))
+    return false;
   if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
       p->bricks[2] * p->brick[2] < p->dims[2] - 1)
     return false;
@@ -2098,13 +2156,14 @@ void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
 void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
                       hipStream_t st) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
-  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags);
-  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s);
+  const Divisors D = divisors_of(p);
+  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
+  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
-  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags);
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s);
-  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3, 0), st, *p, *s);
+  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags, D);
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
+  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3, 0), st, *p, *s, D);
 }
 }  // namespace
 
@@ -2135,35 +2194,35 @@ int bm_register(const bm_params* p, const bm_state* s, void* stream) {
 int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_p2g, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 7, 7), (hipStream_t)stream,
-                     *p, *s, 0);
+                     *p, *s, 0, divisors_of(p));
   return launch_status();
 }
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
-                     *s);
+                     *s, divisors_of(p));
   return launch_status();
 }
 
 int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_g2p, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 9, 3), (hipStream_t)stream,
-                     *p, *s, int64_t(1), 0);
+                     *p, *s, int64_t(1), 0, divisors_of(p));
   return launch_status();
 }
 
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
-                     *s);
+                     *s, divisors_of(p));
   return launch_status();
 }
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_update_F, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 0),
-                     (hipStream_t)stream, *p, *s);
+                     (hipStream_t)stream, *p, *s, divisors_of(p));
   return launch_status();
 }
 

@@ -281,3 +281,18 @@ def test_brick_bins_cover_every_particle_window(shape, size, spacing, fields):
     lx = cl // (brick[1] * brick[2])
     cell = np.stack([bx * brick[0] + lx, by * brick[1] + ly, bz * brick[2] + lz], 1)
     np.testing.assert_array_equal(base_cell[cell_slots[filled]], cell)
+
+
+def test_multiply_high_division_constants_are_exact():
+    """The brick kernels divide window, cell and node indices by launch constants as
+    floor(n / d) = (n * m) >> 32, m = ceil(2^32 / d), for n < 2^16; and (n * m) >> 64,
+    m = ceil(2^64 / d), for n < 2^32 (bm_kernels.hip, Divisors)."""
+    n = np.arange(65536, dtype=np.uint64)
+    for d in list(range(2, 4100)) + [65535, 65536]:
+        m = (0xFFFFFFFF // d) + 1
+        np.testing.assert_array_equal((n * np.uint64(m)) >> np.uint64(32), n // np.uint64(d))
+    rng = np.random.default_rng(0)
+    for d in [int(v) for v in rng.integers(2, 2**32 - 1, 500)] + [2, 3, 133926, 2**31, 2**32 - 1]:
+        m = (2**64 - 1) // d + 1
+        for x in [0, d - 1, d, 2**32 - 1] + [int(v) for v in rng.integers(0, 2**32, 20)]:
+            assert (x * m) >> 64 == x // d
