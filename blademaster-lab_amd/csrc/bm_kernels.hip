@@ -1021,13 +1021,17 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
   int bx = int(P.brick[0]), by = int(P.brick[1]), bz = int(P.brick[2]);
   int ix, iy, iz;
   window_coords(w, D, wi, ix, iy, iz);
+  // Entry of corner 0 of cell (ix, iy, iz); corner a's cell is offset by a uniform stride.
+  int e0 = 8 * __mul24(__mul24(ix, by) + iy, bz) + 8 * iz;
+  int sy = 8 * bz, sx = sy * by;
 #pragma unroll
   for (int k = 0; k < K; ++k) out[k] = 0.0f;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
-    int cx = ix - ((a >> 2) & 1), cy = iy - ((a >> 1) & 1), cz = iz - (a & 1);
+    int dx = (a >> 2) & 1, dy = (a >> 1) & 1, dz = a & 1;
+    int cx = ix - dx, cy = iy - dy, cz = iz - dz;
     if (cx < 0 || cy < 0 || cz < 0 || cx >= bx || cy >= by || cz >= bz) continue;
-    int e = 8 * ((cx * by + cy) * bz + cz) + a;
+    int e = e0 - (dx * sx + dy * sy + dz * 8) + a;
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] += cc[k * CS + e];
   }
@@ -1100,7 +1104,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
       bool own = live && f != 0;
       if (own) {
-        int idx = f * w.W + wb + window_corner(w, a);
+        int idx = __mul24(f, w.W) + wb + window_corner(w, a);
 #pragma unroll
         for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
       }
@@ -1354,7 +1358,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
       f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
-        int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+        int idx = __mul24(field_of(fidx, a), w.W) + wb + window_corner(w, a);
         f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
         f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
         pic = pic + wt[a] * nv;
@@ -1378,7 +1382,7 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
       f3 r = (mass * wt) * vp;
       bool own = live && f != 0;
       if (own) {
-        int idx = f * w.W + wb + window_corner(w, a);
+        int idx = __mul24(f, w.W) + wb + window_corner(w, a);
         atomicAdd(win + idx, r.x);
         atomicAdd(win + FW + idx, r.y);
         atomicAdd(win + 2 * FW + idx, r.z);
@@ -1616,7 +1620,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
     float wt;
     f3 g;
     stencil_corner(lp, inv_h, a, wt, g);
-    int idx = field_of(fidx, a) * w.W + wb + window_corner(w, a);
+    int idx = __mul24(field_of(fidx, a), w.W) + wb + window_corner(w, a);
     f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
     rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
     rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
