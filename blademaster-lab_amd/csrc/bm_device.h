@@ -283,6 +283,57 @@ __device__ __forceinline__ m3 corotated(const m3& U, f3 s, const m3& V, const m3
   m3 P = mmul_t(UD, V);     // U D V^T
   return mmul_t(P, pinv);   // (U D V^T) Fp^-T
 }
+// Cofactor matrix of m (= det(m) m^-T).
+__device__ __forceinline__ m3 cofactor(const m3& m) {
+  m3 c;
+  c.a[0] = m.a[4] * m.a[8] - m.a[5] * m.a[7];
+  c.a[1] = m.a[5] * m.a[6] - m.a[3] * m.a[8];
+  c.a[2] = m.a[3] * m.a[7] - m.a[4] * m.a[6];
+  c.a[3] = m.a[2] * m.a[7] - m.a[1] * m.a[8];
+  c.a[4] = m.a[0] * m.a[8] - m.a[2] * m.a[6];
+  c.a[5] = m.a[1] * m.a[6] - m.a[0] * m.a[7];
+  c.a[6] = m.a[1] * m.a[5] - m.a[2] * m.a[4];
+  c.a[7] = m.a[2] * m.a[3] - m.a[0] * m.a[5];
+  c.a[8] = m.a[0] * m.a[4] - m.a[1] * m.a[3];
+  return c;
+}
+
+// Fixed-corotated first Piola stress of f (det f = J > 0) without an SVD:
+// U diag(2 mu (s - 1) + lam (J - 1) J / s) V^T = 2 mu (f - R) + lam (J - 1) cof(f), where
+// R = U V^T is the rotation of the polar decomposition f = R S and cof(f) = J f^-T.  R comes
+// from Newton's iteration X <- (X + X^-T) / 2 from X = f (Higham 1986), which converges
+// quadratically; it stops once an update changes X by less than 1e-6 (Frobenius), after
+// which the iterate is orthogonal to float resolution.  Returns false when that has not
+// happened within 12 iterations (a nearly singular f): the caller then takes the SVD path.
+__device__ inline bool corotated_polar(const m3& f, float J, float mu, float lam, m3& P) {
+  m3 C = cofactor(f);
+  m3 X = f, Cx = C;
+  float det = J;
+  bool converged = false;
+#pragma unroll 1
+  for (int it = 0; it < 12; ++it) {
+    float half_inv = 0.5f * __builtin_amdgcn_rcpf(det);
+    float change = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      float y = 0.5f * X.a[k] + Cx.a[k] * half_inv;
+      float d = y - X.a[k];
+      change += d * d;
+      X.a[k] = y;
+    }
+    if (change <= 1.0e-12f) {
+      converged = true;
+      break;
+    }
+    Cx = cofactor(X);
+    det = X.a[0] * Cx.a[0] + X.a[1] * Cx.a[1] + X.a[2] * Cx.a[2];
+  }
+  float tm = 2.0f * mu, lj = lam * (J - 1.0f);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) P.a[k] = tm * (f.a[k] - X.a[k]) + lj * C.a[k];
+  return converged;
+}
+
 __device__ __forceinline__ bool is_identity(const m3& m) {
   return m.a[0] == 1.0f && m.a[4] == 1.0f && m.a[8] == 1.0f && m.a[1] == 0.0f && m.a[2] == 0.0f && m.a[3] == 0.0f &&
          m.a[5] == 0.0f && m.a[6] == 0.0f && m.a[7] == 0.0f;

@@ -1551,6 +1551,8 @@ __device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const
     atomicAdd(S.errors + 1, 1);
     return mzero();
   }
+  m3 Pk;
+  if (corotated_polar(f, J, mu, lam, Pk)) return plain ? Pk : mmul_t(Pk, pinv);
   m3 U, V;
   f3 sg;
   svd3(f, U, sg, V);
@@ -2033,6 +2035,28 @@ __global__ void k_probe_friction(const float* in, Contact c, float* out, int cou
   st3(out, 2 * i, jt);
   st3(out, 2 * i + 1, ht);
 }
+// Corotated stress of each F (elastic, Fp = I) as the F update computes it; svd_path[i] is 0
+// for the polar iteration, 1 when it fell back to the SVD, -1 for det F <= 0.
+__global__ void k_probe_stress(const float* Fin, float mu, float lam, float* out, int32_t* svd_path, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  m3 F = ld9(Fin, i), P = mzero();
+  float J = mdet(F);
+  int path = -1;
+  if (J > 0.0f && isfinite(J)) {
+    path = corotated_polar(F, J, mu, lam, P) ? 0 : 1;
+    if (path == 1) {
+      m3 U, V, I = mzero();
+      f3 sg;
+      I.a[0] = I.a[4] = I.a[8] = 1.0f;
+      svd3(F, U, sg, V);
+      P = corotated(U, sg, V, I, mu, lam);
+    }
+  }
+  st9(out, i, P);
+  svd_path[i] = path;
+}
+
 __global__ void k_probe_svd(const float* A, float* U, float* s, float* V, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -2381,6 +2405,15 @@ int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk,
   Contact c{dt, 0.0f, mus, mud, gk, gc};
   hipLaunchKernelGGL(k_probe_friction, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, in, c, out,
                      int(count));
+  return launch_status();
+}
+
+int bm_probe_stress(const float* F, float mu, float lam, float* out, int32_t* svd_path, int32_t count,
+                    void* stream) {
+  if (!F || !out || !svd_path || count < 0) return -1;
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_probe_stress, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, F, mu, lam, out,
+                     svd_path, count);
   return launch_status();
 }
 

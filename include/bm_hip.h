@@ -231,6 +231,10 @@ int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk,
                       int32_t count, void* stream);
 /* 3x3 SVD of A[9*count] -> U[9], s[3], V[9] per lane. */
 int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream);
+/* Test probe: fixed-corotated first Piola stress of each F (Fp = I) as the F update computes
+ * it (stress_at, material.py:94-112); svd_path[i] = 0 polar iteration, 1 SVD, -1 det F <= 0. */
+int bm_probe_stress(const float* F, float mu, float lam, float* out, int32_t* svd_path, int32_t count,
+                    void* stream);
 
 /* ---------------------------------------------------------------- piece labels
  * topology.components (topology.py:17-57): connected components of the particles through

@@ -134,3 +134,48 @@ def test_friction_probe_against_oracle():
         scale_h = max(np.abs(ht).max(), 1e-30)
         assert np.abs(got[i, :3] - jt).max() <= 1e-5 * scale_j + 1e-12, (i, got[i, :3], jt)
         assert np.abs(got[i, 3:] - ht).max() <= 1e-5 * scale_h + 1e-12, (i, got[i, 3:], ht)
+
+
+def corotated_reference(F, mu, lam):
+    """Fixed-corotated first Piola stress U diag(2 mu (s - 1) + lam (J - 1) J / s) V^T
+    (material.py:78-91) from numpy's float64 SVD with proper rotations."""
+    U, s, Vt = np.linalg.svd(F)
+    flip = np.linalg.det(U) * np.linalg.det(Vt) < 0
+    s = s.copy()
+    s[flip, 2] *= -1
+    U[flip, :, 2] *= -1
+    J = np.prod(s, axis=1, keepdims=True)
+    d = 2 * mu * (s - 1) + lam * (J - 1) * J / s
+    return U @ (d[:, :, None] * Vt)
+
+
+def test_corotated_stress_probe_against_float64_svd():
+    """The F update's stress (polar-decomposition iteration, SVD fallback) against the SVD
+    formula in float64, over small and large strains, rotations and conditioning up to 100."""
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    A = svd_inputs(rng, 6000).astype(np.float64)
+    keep = np.linalg.det(A) > 0
+    A = A[keep]
+    k = 500
+    q = np.linalg.qr(rng.normal(size=(k, 3, 3)))[0]
+    q[np.linalg.det(q) < 0] *= -1
+    stretch = np.exp(rng.uniform(-np.log(10), np.log(10), (k, 3)))
+    ill = q @ (stretch[:, :, None] * np.swapaxes(q, 1, 2))
+    A = np.concatenate([A, ill, np.eye(3)[None] + 1e-7 * rng.normal(size=(20, 3, 3))]).astype(np.float32)
+    n = len(A)
+    mu, lam = 7.2e4, 1.1e5
+    dA = dev(A.reshape(n, 9))
+    out = torch.zeros(n, 9, device="cuda:0")
+    path = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    _lib.check(lib.bm_probe_stress(ptr(dA), ctypes.c_float(mu), ctypes.c_float(lam), ptr(out), ptr(path), n, stream()),
+               "bm_probe_stress")
+    torch.cuda.synchronize()
+    P = out.cpu().numpy().astype(np.float64).reshape(n, 3, 3)
+    ref = corotated_reference(A.astype(np.float64), mu, lam)
+    scale = (2 * mu + lam) * np.maximum(np.linalg.norm(A.astype(np.float64), axis=(1, 2)), 1.0) ** 3
+    err = np.linalg.norm(P - ref, axis=(1, 2)) / scale
+    assert err.max() < 2e-6, err.max()
+    paths = path.cpu().numpy()
+    assert np.all(paths >= 0)
+    assert (paths == 0).mean() > 0.99  # the polar iteration converges for all but pathological F
