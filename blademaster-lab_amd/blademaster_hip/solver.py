@@ -157,6 +157,7 @@ class Solver:
             T["instance"] = torch.tensor(np.asarray(_layout["instance"], dtype=np.int32), device=dev)
             T["inst_mu"] = torch.tensor(np.asarray(_layout["mu"], dtype=np.float32), device=dev)
             T["inst_lam"] = torch.tensor(np.asarray(_layout["lam"], dtype=np.float32), device=dev)
+            T["inst_sums"] = t((I, 16, 10))
         T["errors"] = t((3,), i32)
         T["topology_dirty"] = t((1,), i32, 1)
         T["fp_identity"] = t((1,), i32, 1)

@@ -272,6 +272,7 @@ __device__ void finish_external_regs(const bm_params& P, const bm_state& S, cons
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag);
 
 constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
+constexpr int kInstSlots = 16;  // slots per instance of the batched contact sums (inst_sums)
 
 // rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
 // Per-block sums go to rigid_partial; k_prep reduces them.  In a substep the same pass also
@@ -388,20 +389,18 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
     // Batched instances: the step sums go to each instance's own scalars (a wave reduction
     // when the whole wave belongs to one instance, else per lane).
     float v[10] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z, torq.x, torq.y, torq.z, work};
-    float* dst[10] = {S.board + 3 * inst, S.board + 3 * inst + 1, S.board + 3 * inst + 2,
-                      S.reaction + 3 * inst, S.reaction + 3 * inst + 1, S.reaction + 3 * inst + 2,
-                      S.torque + 3 * inst, S.torque + 3 * inst + 1, S.torque + 3 * inst + 2, S.work + inst};
+    float* dst = S.inst_sums + (int64_t(inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10;
     int first = __builtin_amdgcn_readfirstlane(inst);
     if (__ballot(inst != first) == 0) {
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
         float t = wave_sum(v[k]);
-        if (lane0() && t != 0.0f) atomicAdd(dst[k], t);
+        if (lane0() && t != 0.0f) atomicAdd(dst + k, t);
       }
     } else {
 #pragma unroll
       for (int k = 0; k < 10; ++k)
-        if (v[k] != 0.0f) atomicAdd(dst[k], v[k]);
+        if (v[k] != 0.0f) atomicAdd(dst + k, v[k]);
     }
     board_j = react = torq = f3{0.f, 0.f, 0.f};
     work = 0.0f;
@@ -489,11 +488,27 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
   if (S.instance != nullptr) {
     // Batched instances: each thread finishes the step of some instances (their sums were
     // accumulated by k_rigid); thread 0 also keeps the shared pair-list scalars.
-    if (mode & kFinishExternal) {
-      for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
+    for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
+      // This step's sums of instance i: its slots, which are cleared for the next step.
+      float sum[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float* slots = S.inst_sums + int64_t(i) * kInstSlots * 10;
+      for (int b = 0; b < kInstSlots; ++b)
+#pragma unroll
+        for (int k = 0; k < 10; ++k) sum[k] += slots[b * 10 + k];
+      for (int b = 0; b < kInstSlots; ++b)
+#pragma unroll
+        for (int k = 0; k < 10; ++k) slots[b * 10 + k] = 0.0f;
+      f3 board = ld3(S.board, i) + f3{sum[0], sum[1], sum[2]};
+      f3 j = ld3(S.reaction, i) + f3{sum[3], sum[4], sum[5]};
+      f3 tau = ld3(S.torque, i) + f3{sum[6], sum[7], sum[8]};
+      S.work[i] += sum[9];
+      if (mode & kFinishExternal) {
         ExternalTool e = load_external_tool(P, S, i);
-        f3 j = ld3(S.reaction, i), tau = ld3(S.torque, i);
         finish_external_regs(P, S, e, j, tau, i);
+      } else {
+        st3(S.board, i, board);
+        st3(S.reaction, i, j);
+        st3(S.torque, i, tau);
       }
     }
     if (threadIdx.x != 0) return;
@@ -2074,7 +2089,9 @@ inline int launch_status() {
 }
 
 bool state_ok(const bm_params* p, const bm_state* s) {
-  return p != nullptr && s != nullptr && p->n >= 0 && s->x != nullptr && s->v != nullptr;
+  if (p == nullptr || s == nullptr || p->n < 0 || s->x == nullptr || s->v == nullptr) return false;
+  // Batched instances carry their Lame constants and the slots of their contact sums.
+  return s->instance == nullptr || (s->inst_mu != nullptr && s->inst_lam != nullptr && s->inst_sums != nullptr);
 }
 
 }  // namespace

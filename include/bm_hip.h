@@ -88,6 +88,11 @@ typedef struct bm_state {
    * interval sums are then per instance ([instances][...]) */
   int32_t *instance;       /* [n] */
   float *inst_mu, *inst_lam; /* [instances] */
+  /* [instances][16][10] contact sums of the contact kernel (board 3, reaction 3, torque 3,
+   * work), spread over 16 slots by workgroup so that the atomics of an instance's many
+   * wavefronts do not all meet on one address; zero between steps (the step's finish adds
+   * and clears them) */
+  float *inst_sums;
   /* pair contacts: ping-pong history tables [2][P] and the partner lists */
   int64_t *pair_keys;      /* [2][P], -1 empty */
   float *pair_history;     /* [2][P][3] */
