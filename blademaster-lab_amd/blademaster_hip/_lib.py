@@ -72,6 +72,7 @@ STATE_FIELDS = [
     ("table", c_f32p), ("interval_impulse", c_f32p), ("interval_angular", c_f32p),
     ("errors", c_i32p), ("topology_dirty", c_i32p), ("fp_identity", c_i32p), ("instance", c_i32p),
     ("inst_mu", c_f32p), ("inst_lam", c_f32p), ("inst_sums", c_f32p),
+    ("inst_buckets", c_i32p), ("inst_pair_builds", c_i32p), ("inst_pair_dirty", c_i32p),
     ("pair_keys", c_i64p), ("pair_history", c_f32p), ("pair_used", c_i32p), ("pair_count", c_i32p),
     ("pair_dirty", c_i32p), ("pair_builds", c_i32p), ("list_reach", c_f32p), ("built_x", c_f32p),
     ("built_codes", c_i64p), ("partners", c_i32p), ("partner_count", c_i32p), ("list_heads", c_i64p),

@@ -164,6 +164,16 @@ class Solver:
         # pair contacts
         P_ = self.pair_capacity
         self.list_buckets = _pow2_at_least(2 * n)
+        if _layout is not None:
+            # One bucket region per instance (a power of two of at least twice its particles),
+            # so each instance's pair list is binned and rebuilt on its own.
+            counts = np.bincount(np.asarray(_layout["instance"]), minlength=I)
+            sizes = np.array([_pow2_at_least(2 * max(int(c), 1)) for c in counts], dtype=np.int64)
+            bases = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            self.list_buckets = int(sizes.sum())
+            T["inst_buckets"] = torch.tensor(np.stack([bases, sizes], 1).astype(np.int32), device=dev)
+            T["inst_pair_builds"] = t((I,), i32)
+            T["inst_pair_dirty"] = t((I,), i32, 1)
         T["pair_keys"] = t((2, P_), i64, -1)
         T["pair_history"] = t((2, P_, 3))
         T["pair_used"] = t((2, P_), i32)

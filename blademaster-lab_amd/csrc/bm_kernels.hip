@@ -269,7 +269,22 @@ __device__ __forceinline__ ExternalTool load_external_tool(const bm_params& P, c
 __device__ void finish_external_regs(const bm_params& P, const bm_state& S, const ExternalTool& e, f3 j, f3 tau,
                                      int i = 0);
 
-__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag);
+// The pair list of one solver, or of one batched instance: the first bucket and bucket mask of
+// its cell list, the tag of its current build and whether it is rebuilt this step.  Batched
+// instances keep separate lists, each rebuilt when its own particles have moved too far (as
+// the separate runs of a sweep are), in a bucket region of its own.
+struct ListRegion {
+  int base, mask, tag;
+  bool build;
+};
+__device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_state& S, int inst) {
+  if (S.instance == nullptr) return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0], S.pair_dirty[0] != 0};
+  return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, S.inst_pair_builds[inst],
+                    S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0};
+}
+
+__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
+                                             const ListRegion& L);
 
 constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
 constexpr int kInstSlots = 16;  // slots per instance of the batched contact sums (inst_sums)
@@ -311,7 +326,10 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
   if (live) {
     f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
     spd = len(vp);
-    if ((mode & kPairPrep) != 0 && S.pair_dirty[0] != 0) bin_particle(P, S, p, xp, S.pair_builds[0] + 1);
+    if ((mode & kPairPrep) != 0) {
+      ListRegion L = list_region(P, S, inst);
+      if (L.build) bin_particle(P, S, p, xp, L);
+    }
     f3 total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
     // board: plane z = 0 inside its rectangle
     float gap = xp.z - float(P.radius);
@@ -385,23 +403,17 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
     }
     st3(S.impulses, p, total);
   }
-  if (S.instance != nullptr) {
-    // Batched instances: the step sums go to each instance's own scalars (a wave reduction
-    // when the whole wave belongs to one instance, else per lane).
+  // Batched instances: the step sums go to each instance's own slots.  The block's first
+  // particle names the instance of the block reduction below; a wave with a live particle of
+  // another instance (at most the waves where instances meet, since they are stored one after
+  // the other) adds its lanes' sums directly instead.
+  const int block_inst = S.instance != nullptr ? S.instance[int64_t(blockIdx.x) * blockDim.x] : 0;
+  if (S.instance != nullptr && __ballot(live && inst != block_inst) != 0) {
     float v[10] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z, torq.x, torq.y, torq.z, work};
     float* dst = S.inst_sums + (int64_t(inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10;
-    int first = __builtin_amdgcn_readfirstlane(inst);
-    if (__ballot(inst != first) == 0) {
 #pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        float t = wave_sum(v[k]);
-        if (lane0() && t != 0.0f) atomicAdd(dst + k, t);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 10; ++k)
-        if (v[k] != 0.0f) atomicAdd(dst + k, v[k]);
-    }
+    for (int k = 0; k < 10; ++k)
+      if (v[k] != 0.0f) atomicAdd(dst + k, v[k]);
     board_j = react = torq = f3{0.f, 0.f, 0.f};
     work = 0.0f;
   }
@@ -432,12 +444,18 @@ __global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_st
     int k = threadIdx.x;
     float acc = wsum[0][k];
     for (int w = 1; w < kBlock / 64; ++w) acc = (k == 10) ? fmaxf(acc, wsum[w][k]) : acc + wsum[w][k];
+    if (S.instance != nullptr && k < 10) {
+      // One add per sum and block into a slot of the block's instance (k_prep reads only the
+      // speed of rigid_partial for batched instances).
+      if (acc != 0.0f) atomicAdd(S.inst_sums + (int64_t(block_inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10 + k, acc);
+      acc = 0.0f;
+    }
     S.rigid_partial[int64_t(blockIdx.x) * kRigidSums + k] = acc;
   }
 }
 
 __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
-__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d);
+__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
 
 // Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
 // the single-thread work of the step: apply the sums, list reach, knife update.  Other
@@ -457,8 +475,13 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0) return;
     int count = min(S.changed_count[0], int(P.changed_capacity));
     int first = int(blockIdx.x) - 1 - insert_blocks, blocks = int(gridDim.x) - 1 - insert_blocks;
-    for (int i = first * int(blockDim.x) + int(threadIdx.x); i < count; i += blocks * int(blockDim.x))
-      mark_pair_neighbours(P, S, S.changed[i]);
+    constexpr int kWaves = kPrepBlock / 64;
+    int lane = int(threadIdx.x) & 63;
+    for (int i = first * kWaves + int(threadIdx.x >> 6); i < count; i += blocks * kWaves) {
+      int32_t d = S.changed[i];
+      ListRegion L = list_region(P, S, instance_of(S, d));
+      if (!L.build) mark_pair_neighbours(P, S, d, lane, L);  // a list rebuilt in full needs no marks
+    }
     return;
   }
   __shared__ float part[kPrepBlock / 64][kRigidSums];
@@ -487,8 +510,14 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
   }
   if (S.instance != nullptr) {
     // Batched instances: each thread finishes the step of some instances (their sums were
-    // accumulated by k_rigid); thread 0 also keeps the shared pair-list scalars.
+    // accumulated by k_rigid) and counts the builds of their pair lists; thread 0 also keeps
+    // the shared pair-list scalars.
+    int built = 0;
     for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
+      if ((mode & kPairPrep) != 0 && (S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0)) {
+        S.inst_pair_builds[i] += 1;
+        built = 1;
+      }
       // This step's sums of instance i: its slots, which are cleared for the next step.
       float sum[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float* slots = S.inst_sums + int64_t(i) * kInstSlots * 10;
@@ -511,14 +540,14 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
         st3(S.torque, i, tau);
       }
     }
+    built = __syncthreads_or(built);
     if (threadIdx.x != 0) return;
     const float dt = float(P.dt);
     float speed = part[0][10];
     S.speed[0] = speed;
     if (mode & kPairPrep) {
-      int dirty = S.pair_dirty[0];
       S.pair_count[parity_new] = 0;
-      if (dirty) {
+      if (built) {
         float r = float(P.radius), skin = float(P.pair_skin);
         float value = 2.0f * r + skin;
         if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
@@ -611,18 +640,22 @@ __device__ __forceinline__ int64_t table_find(const int64_t* keys, int64_t cap, 
 struct Cell {
   int x, y, z;
 };
-__device__ __forceinline__ Cell cell_of(f3 x, float size) {
-  return Cell{int(floorf(x.x / size)), int(floorf(x.y / size)), int(floorf(x.z / size))};
+// Cell of a point: floor(x / size), taken as floor(x * (1 / size)) so the candidate checks of
+// a search cost multiplies instead of divisions.  Binning and every search use this one
+// function with the same reciprocal, so they agree on every particle's cell.
+__device__ __forceinline__ Cell cell_of(f3 x, float inv) {
+  return Cell{int(floorf(x.x * inv)), int(floorf(x.y * inv)), int(floorf(x.z * inv))};
 }
 // Spatial hash of Teschner et al. (2003), "Optimized Spatial Hashing for Collision
 // Detection of Deformable Objects".
 __device__ __forceinline__ int bucket_of(Cell c, int mask) {
   return int((uint32_t(c.x) * 73856093u) ^ (uint32_t(c.y) * 19349663u) ^ (uint32_t(c.z) * 83492791u)) & mask;
 }
-__device__ __forceinline__ int bucket_of_point(f3 x, float cell, int mask) { return bucket_of(cell_of(x, cell), mask); }
+__device__ __forceinline__ int bucket_of_point(f3 x, float inv, int mask) { return bucket_of(cell_of(x, inv), mask); }
 // The cell list bins particles in cells of the list's base reach 2 r + skin; a rebuild
 // whose reach is larger (fast particles) searches ceil(reach / cell) rings of cells.
 __device__ __forceinline__ float list_cell_size(const bm_params& P) { return 2.0f * float(P.radius) + float(P.pair_skin); }
+__device__ __forceinline__ float list_cell_inv(const bm_params& P) { return 1.0f / list_cell_size(P); }
 // At most 4 rings: only a diverged run (non-finite or absurd speeds) asks for more, and its
 // contacts are void anyway (the run fails its state checks).
 __device__ __forceinline__ int list_rings(const bm_params& P, float reach) {
@@ -630,6 +663,27 @@ __device__ __forceinline__ int list_rings(const bm_params& P, float reach) {
   if (!(r <= 4.0f)) r = 4.0f;
   return max(1, int(ceilf(r)));
 }
+// Distance along one axis from coordinate x (in cell `home`) to the cell home + d.
+__device__ __forceinline__ float axis_gap(float x, int home, int d, float cell) {
+  float g = d > 0 ? float(home + d) * cell - x : (d < 0 ? x - float(home + d + 1) * cell : 0.0f);
+  return fmaxf(g, 0.0f);
+}
+// A cell of the search cube around x can hold a particle within `reach` of x only if its box
+// is nearer than reach; the others are skipped.  The bound carries a margin of 1e-3 cells
+// over the rounding of the box faces, so no particle within reach is skipped.
+struct CellCull {
+  f3 x;
+  Cell home;
+  float cell, limit2;
+  __device__ __forceinline__ CellCull(f3 xp, Cell h, float c, float reach) : x(xp), home(h), cell(c) {
+    float r = reach + 1.0e-3f * c;
+    limit2 = r * r;
+  }
+  __device__ __forceinline__ bool near(int dx, int dy, int dz) const {
+    float gx = axis_gap(x.x, home.x, dx, cell), gy = axis_gap(x.y, home.y, dy, cell), gz = axis_gap(x.z, home.z, dz, cell);
+    return gx * gx + gy * gy + gz * gz <= limit2;
+  }
+};
 
 __global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
   int count = S.pair_count[parity_new];
@@ -640,8 +694,10 @@ __global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
 
 // bin_particles (contact.py:764-775) into bucket lists without a clearing pass: a head
 // written by an earlier build carries an older tag and reads as an empty bucket.
-__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp, int tag) {
-  int b = bucket_of_point(xp, list_cell_size(P), int(P.list_buckets - 1));
+__device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
+                                             const ListRegion& L) {
+  int tag = L.tag + 1;
+  int b = L.base + bucket_of_point(xp, list_cell_inv(P), L.mask);
   unsigned long long entry = (uint64_t(uint32_t(tag)) << 32) | uint64_t(uint32_t(p));
   unsigned long long old = atomicExch(reinterpret_cast<unsigned long long*>(S.list_heads + b), entry);
   S.list_links[p] = (uint32_t(old >> 32) == uint32_t(tag)) ? int32_t(uint32_t(old)) : -1;
@@ -653,10 +709,10 @@ __device__ __forceinline__ int32_t bucket_head(const bm_state& S, int b, int tag
 
 // Standalone set_pair_reach + binning (contact.py:749-775) for bm_pair_contacts.
 __global__ void k_list_bin(const bm_params P, bm_state S) {
-  if (S.pair_dirty[0] == 0) return;
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  bin_particle(P, S, p, ld3(S.x, p), S.pair_builds[0] + 1);
+  ListRegion L = list_region(P, S, instance_of(S, p));
+  if (L.build) bin_particle(P, S, p, ld3(S.x, p), L);
 }
 __global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
   float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt), sp = S.speed[0];
@@ -664,7 +720,14 @@ __global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
   if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
   S.list_reach[0] = value;
   S.pair_count[parity_new] = 0;
-  if (S.pair_dirty[0] != 0) S.pair_builds[0] += 1;
+  bool built = S.pair_dirty[0] != 0;
+  for (int i = 0; S.instance != nullptr && i < int(P.instances); ++i) {
+    if (S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0) {
+      S.inst_pair_builds[i] += 1;
+      built = true;
+    }
+  }
+  if (built) S.pair_builds[0] += 1;
 }
 
 struct PairCtx {
@@ -722,14 +785,16 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  int tag = S.pair_builds[0];
+  ListRegion L = list_region(P, S, instance_of(S, p));
+  int tag = L.tag;
   float reach = S.list_reach[0];
-  float cell = list_cell_size(P);
+  const float reach2 = reach * reach;
+  float cell = list_cell_size(P), inv = list_cell_inv(P);
   int rings = list_rings(P, reach);
-  int mask = int(P.list_buckets - 1);
+  int base = L.base, mask = L.mask;
   int lanes = int(P.list_lanes);
   int count;
-  bool dirty = S.pair_dirty[0] != 0;
+  bool dirty = L.build;
   // A full rebuild uses the current positions; a rebuild after relabelling only (this
   // particle or a neighbour got a new code) keeps the positions of the last full build, so
   // every list stays valid for the same skin.
@@ -742,11 +807,13 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
     S.built_codes[p] = cp;
     f3 rp = ld3(S.ref, p);
     float two_h = 2.0f * float(P.spacing);
-    Cell home = cell_of(xp, cell);
+    Cell home = cell_of(xp, inv);
     count = 0;
     // The bucket chains of up to 9 cells are walked in lockstep, so each step of the walk has
-    // up to 9 independent loads in flight instead of one.
+    // up to 9 independent loads in flight instead of one.  A lane skips the cells of the
+    // search cube too far away to hold a partner.
     constexpr int kChains = 9;
+    CellCull cull(xp, home, cell, reach);
     int side = 2 * rings + 1, nyz = side * side;
     for (int dx = -rings; dx <= rings; ++dx) {
       for (int g0 = 0; g0 < nyz; g0 += kChains) {
@@ -755,8 +822,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
 #pragma unroll
         for (int j = 0; j < kChains; ++j) {
           int k = g0 + j;
-          cc[j] = Cell{home.x + dx, home.y + k / side - rings, home.z + k % side - rings};
-          cur[j] = k < nyz ? bucket_head(S, bucket_of(cc[j], mask), tag) : -1;
+          int dy = k / side - rings, dz = k % side - rings;
+          cc[j] = Cell{home.x + dx, home.y + dy, home.z + dz};
+          cur[j] = (k < nyz && cull.near(dx, dy, dz)) ? bucket_head(S, base + bucket_of(cc[j], mask), tag) : -1;
         }
         bool more = true;
         while (more) {
@@ -770,9 +838,10 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
             more = more || next >= 0;
             if (q <= p || count < 0) continue;
             f3 xq = ld3(bx, q);
-            Cell cq = cell_of(xq, cell);
+            Cell cq = cell_of(xq, inv);
             if (cq.x != cc[j].x || cq.y != cc[j].y || cq.z != cc[j].z) continue;
-            if (!(len(xq - xp) < reach)) continue;
+            f3 d = xq - xp;
+            if (!(dot(d, d) < reach2)) continue;
             if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
             if (count == lanes) {
               count = -1;
@@ -799,44 +868,54 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   }
   // Overflowing particle: search the buckets of the last build around its build position.
   // In a full build step built_x is being written by this kernel; the build positions are x.
-  Cell home = cell_of(ld3(bx, p), cell);
+  f3 xb = ld3(bx, p);
+  Cell home = cell_of(xb, inv);
+  CellCull cull(xb, home, cell, reach);
   for (int dx = -rings; dx <= rings; ++dx)
     for (int dy = -rings; dy <= rings; ++dy)
       for (int dz = -rings; dz <= rings; ++dz) {
+        if (!cull.near(dx, dy, dz)) continue;
         Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-        for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
+        for (int32_t q = bucket_head(S, base + bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
           if (q <= p) continue;
-          Cell cq = cell_of(ld3(bx, q), cell);
+          Cell cq = cell_of(ld3(bx, q), inv);
           if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
           pair_at(P, S, T, p, q);
         }
       }
 }
-__global__ void k_pairs_done(bm_state S) {
+__global__ void k_pairs_done(const bm_params P, bm_state S) {
   S.pair_dirty[0] = 0;
+  for (int i = 0; S.instance != nullptr && i < int(P.instances); ++i) S.inst_pair_dirty[i] = 0;
   S.changed_count[0] = 0;
   S.pair_epoch[0] += 1;
 }
 
 // Marks relabelled particle d and every particle within the list reach of d at the build
-// positions: the pairs whose candidate status can change with d's code.
-__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d) {
+// positions: the pairs whose candidate status can change with d's code.  One wave per
+// particle: the lanes take the cells of the search cube (those near enough to matter) and walk
+// their bucket chains side by side; every mark written is the same value.
+__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
   int mark = S.pair_epoch[0] + 1;
-  S.pair_mark[d] = mark;
-  int tag = S.pair_builds[0];
+  if (lane == 0) S.pair_mark[d] = mark;
+  int tag = L.tag;
   float reach = S.list_reach[0];
+  const float reach2 = reach * reach;
   float cell = list_cell_size(P);
   int rings = list_rings(P, reach);
-  int mask = int(P.list_buckets - 1);
   f3 xd = ld3(S.built_x, d);
-  Cell home = cell_of(xd, cell);
-  for (int dx = -rings; dx <= rings; ++dx)
-    for (int dy = -rings; dy <= rings; ++dy)
-      for (int dz = -rings; dz <= rings; ++dz) {
-        Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-        for (int32_t q = bucket_head(S, bucket_of(c, mask), tag); q >= 0; q = S.list_links[q])
-          if (len(ld3(S.built_x, q) - xd) < reach) S.pair_mark[q] = mark;
-      }
+  Cell home = cell_of(xd, list_cell_inv(P));
+  CellCull cull(xd, home, cell, reach);
+  int side = 2 * rings + 1;
+  for (int k = lane; k < side * side * side; k += 64) {
+    int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
+    if (!cull.near(dx, dy, dz)) continue;
+    Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+    for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), tag); q >= 0; q = S.list_links[q]) {
+      f3 dd = ld3(S.built_x, q) - xd;
+      if (dot(dd, dd) < reach2) S.pair_mark[q] = mark;
+    }
+  }
 }
 
 // check_pair_list (contact.py:716-735) as its own pass for a standalone pair-contact call.
@@ -846,8 +925,12 @@ __global__ void k_check_pair_list(const bm_params P, bm_state S) {
   float dt = float(P.dt);
   f3 xp = ld3(S.x, p);
   if (len(xp - ld3(S.built_x, p)) + dt * len(ld3(S.v, p)) > 0.5f * float(P.pair_skin) ||
-      S.codes[p] != S.built_codes[p])
-    S.pair_dirty[0] = 1;
+      S.codes[p] != S.built_codes[p]) {
+    if (S.instance != nullptr)
+      S.inst_pair_dirty[S.instance[p]] = 1;
+    else
+      S.pair_dirty[0] = 1;
+  }
 }
 
 // ------------------------------------------------------------------------ topology
@@ -1072,6 +1155,8 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       S.pair_epoch[0] += 1;
     }
   }
+  if ((flags & 2) && S.instance != nullptr && blockIdx.x == 0)
+    for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) S.inst_pair_dirty[i] = 0;
   Window w = window_of(P, D, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -1384,8 +1469,12 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
       st3(S.v, p, vp);
       st3(S.x, p, xp);
       if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
-      if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin))
-        S.pair_dirty[0] = 1;
+      if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin)) {
+        if (S.instance != nullptr)
+          S.inst_pair_dirty[S.instance[p]] = 1;
+        else
+          S.pair_dirty[0] = 1;
+      }
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -2091,7 +2180,9 @@ inline int launch_status() {
 bool state_ok(const bm_params* p, const bm_state* s) {
   if (p == nullptr || s == nullptr || p->n < 0 || s->x == nullptr || s->v == nullptr) return false;
   // Batched instances carry their Lame constants and the slots of their contact sums.
-  return s->instance == nullptr || (s->inst_mu != nullptr && s->inst_lam != nullptr && s->inst_sums != nullptr);
+  return s->instance == nullptr || (s->inst_mu != nullptr && s->inst_lam != nullptr && s->inst_sums != nullptr &&
+                                    s->inst_buckets != nullptr && s->inst_pair_builds != nullptr &&
+                                    s->inst_pair_dirty != nullptr);
 }
 
 }  // namespace
@@ -2132,8 +2223,8 @@ bool pair_args_ok(const bm_params* p, const bm_state* s) {
          s->pair_builds && s->list_reach && s->built_x && s->built_codes && s->partners && s->partner_count &&
          s->list_heads && s->list_links && s->changed && s->changed_count && s->pair_mark && s->pair_epoch &&
          p->changed_capacity >= 0 && p->pair_capacity > 0 && (p->pair_capacity & (p->pair_capacity - 1)) == 0 &&
-         p->list_buckets > 0 && (p->list_buckets & (p->list_buckets - 1)) == 0 &&
-         p->list_buckets <= int64_t(4096) * 4096 && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
+         p->list_buckets > 0 && (s->instance != nullptr || (p->list_buckets & (p->list_buckets - 1)) == 0) &&
+         p->list_buckets <= (int64_t(1) << 28) && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
 }
 int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
 // Workgroups of a brick kernel: one per brick with particles.
@@ -2223,7 +2314,7 @@ int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void
   hipLaunchKernelGGL(k_list_bin, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_list_reach, dim3(1), dim3(1), 0, st, *p, *s, new_p);
   hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
-  hipLaunchKernelGGL(k_pairs_done, dim3(1), dim3(1), 0, st, *s);
+  hipLaunchKernelGGL(k_pairs_done, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
 }
 
@@ -2330,11 +2421,13 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
     mode |= kToolStep | (accumulate ? kAccumulateInterval : 0);
   }
   int nb = blocks_for(p->n);
-  // Registration and marking blocks exit at once on most substeps; keep their count modest.
+  // Registration and marking blocks exit at once on most substeps; keep their count modest:
+  // registration at most 512 blocks (two per CU), marking one wave per relabelled particle
+  // in at most 16 blocks.
   int insert_blocks = int((p->n + kPrepBlock - 1) / kPrepBlock);
-  insert_blocks = insert_blocks < 128 ? insert_blocks : 128;
-  int mark_blocks = int((p->changed_capacity + kPrepBlock - 1) / kPrepBlock);
-  mark_blocks = mark_blocks < 4 ? mark_blocks : 4;
+  insert_blocks = insert_blocks < 512 ? insert_blocks : 512;
+  int mark_blocks = int((p->changed_capacity + kPrepBlock / 64 - 1) / (kPrepBlock / 64));
+  mark_blocks = mark_blocks < 16 ? mark_blocks : 16;
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
   hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
                      ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);

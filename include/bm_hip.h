@@ -93,6 +93,13 @@ typedef struct bm_state {
    * wavefronts do not all meet on one address; zero between steps (the step's finish adds
    * and clears them) */
   float *inst_sums;
+  /* per-instance pair lists: each instance's cell list lives in a bucket region of its own
+   * ([instances][2]: first bucket, bucket count, a power of two; the regions tile
+   * [0, list_buckets)), carries its own build tag and is rebuilt on its own request (set by
+   * the instance's particles; pair_dirty rebuilds every list) */
+  int32_t *inst_buckets;
+  int32_t *inst_pair_builds; /* [instances] */
+  int32_t *inst_pair_dirty;  /* [instances] */
   /* pair contacts: ping-pong history tables [2][P] and the partner lists */
   int64_t *pair_keys;      /* [2][P], -1 empty */
   float *pair_history;     /* [2][P][3] */
