@@ -466,8 +466,31 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
                                                  int parity_new, int insert_blocks) {
   if (blockIdx.x > 0 && int(blockIdx.x) <= insert_blocks) {
     if ((mode & kTopologyPrep) == 0 || S.topology_dirty[0] == 0) return;
+    // Registration, first half: one anchor particle per cell with even coordinates in its
+    // brick (its first particle) claims its code at its 8 nodes.  With even brick sides every
+    // node is a corner of exactly one such cell, so these claims meet no contention; the
+    // other particles register in k_pairs and mostly find their code in place, reading it
+    // instead of competing for the same slots.  Anchors register again there, finding their
+    // own slots.
+    const int ax = int((P.brick[0] + 1) / 2), ay = int((P.brick[1] + 1) / 2), az = int((P.brick[2] + 1) / 2);
+    const int per = ax * ay * az;
+    const int64_t nbricks = P.active_bricks > 0 ? P.active_bricks : P.bricks[0] * P.bricks[1] * P.bricks[2];
+    const int64_t cpb = P.brick[0] * P.brick[1] * P.brick[2];
     int64_t stride = int64_t(insert_blocks) * blockDim.x;
-    for (int64_t p = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; p < P.n; p += stride) insert_particle(P, S, p);
+    for (int64_t a = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; a < nbricks * per; a += stride) {
+      int b = active_brick(S, int(a / per));
+      int k = int(a % per);
+      int cz = 2 * (k % az), cy = 2 * ((k / az) % ay), cx = 2 * (k / (az * ay));
+      int64_t cell = int64_t(b) * cpb + (cx * P.brick[1] + cy) * P.brick[2] + cz;
+      const int32_t* slots = S.perm + 8 * cell;
+      for (int s = 0; s < 8; ++s) {
+        int32_t q = slots[s];
+        if (q >= 0) {
+          insert_particle(P, S, q);
+          break;
+        }
+      }
+    }
     return;
   }
   if (blockIdx.x > 0) {
@@ -782,9 +805,11 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 // binned by the contact kernel: partners within reach that pair_at does not skip as
 // compatible reference neighbours, at most `lanes`, else -1 and a bucket search at every
 // step (pairs_listed, contact.py:825-925).
-__global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old) {
+__global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
+  // Registration, second half (the anchors claimed their slots in k_prep).
+  if (topology && S.topology_dirty[0] != 0) insert_particle(P, S, p);
   ListRegion L = list_region(P, S, instance_of(S, p));
   int tag = L.tag;
   float reach = S.list_reach[0];
@@ -2313,7 +2338,7 @@ int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void
   // Binning tags the bucket heads with the build count after this build's increment.
   hipLaunchKernelGGL(k_list_bin, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_list_reach, dim3(1), dim3(1), 0, st, *p, *s, new_p);
-  hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p);
+  hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p, 0);
   hipLaunchKernelGGL(k_pairs_done, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
 }
@@ -2432,7 +2457,7 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
   hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
                      ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);
   // 3. particle pairs across cuts (partner lists rebuilt first when stale)
-  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p);
+  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p, 1);
   // 4. transfers; G2P labels the moved particles and checks the pair list's skin
   launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st);
   if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
