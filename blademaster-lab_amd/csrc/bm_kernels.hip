@@ -293,7 +293,7 @@ constexpr int kInstSlots = 16;  // slots per instance of the batched contact sum
 // Per-block sums go to rigid_partial; k_prep reduces them.  In a substep the same pass also
 // empties the pair slots of the new table, bins particles for a pair-list rebuild and clears
 // the field table for a re-registration.
-__global__ void __launch_bounds__(kBlock) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
                                                   int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
