@@ -39,7 +39,9 @@ def pack(configs, gap=None):
 
     Shelf packing in x and y with `gap` metres between neighbours: a number, or (x, y) gaps.
     The default is a knife diameter plus two cells, safe for any knife orientation.  Each
-    instance keeps its own lattice: its grid corner lands on a node of the packed grid.
+    instance keeps its own lattice: its grid corner lands on a node of the packed grid, at an
+    even cell index in x and y, so that an instance starts on a brick boundary (bricks are
+    two cells wide in x and y) and a block with even cell counts fills its bricks.
     Returns the layout for `Solver` and, per instance, the offset added to its own coordinates
     and its particle range.
     """
@@ -58,13 +60,17 @@ def pack(configs, gap=None):
     widths = np.array([p[2][0] for p in parts]) + gap_cells[0]
     depths = np.array([p[2][1] for p in parts]) + gap_cells[1]
     row_width = max(int(np.ceil(np.sqrt(float(np.sum(widths * depths))))), int(widths.max()))
+
+    def even(v):
+        return v + (v & 1)
+
     corner = []
     x = y = row_depth = 0
     for w, d in zip(widths, depths):
         if x and x + w > row_width:
-            x, y, row_depth = 0, y + row_depth, 0
+            x, y, row_depth = 0, even(y + row_depth), 0
         corner.append((x, y))
-        x += w
+        x = even(x + w)
         row_depth = max(row_depth, d)
     corner = np.array(corner, dtype=np.int64)
     cells = np.array([max(c[0] + p[2][0] for c, p in zip(corner, parts)),

@@ -808,6 +808,10 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
+  // The particle's two list words are loaded together up front: in the common case (no
+  // rebuild, most lists empty) the thread waits for one round of loads.
+  const int32_t mark_p = S.pair_mark[p];
+  const int32_t listed = S.partner_count[p];
   // Registration, second half (the anchors claimed their slots in k_prep).
   if (topology && S.topology_dirty[0] != 0) insert_particle(P, S, p);
   ListRegion L = list_region(P, S, instance_of(S, p));
@@ -823,7 +827,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   // A full rebuild uses the current positions; a rebuild after relabelling only (this
   // particle or a neighbour got a new code) keeps the positions of the last full build, so
   // every list stays valid for the same skin.
-  bool rebuild = dirty || S.pair_mark[p] == S.pair_epoch[0] + 1;
+  bool rebuild = dirty || mark_p == S.pair_epoch[0] + 1;
   const float* bx = dirty ? S.x : S.built_x;
   if (rebuild) {
     f3 xp = ld3(bx, p);
@@ -880,7 +884,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
     }
     S.partner_count[p] = count;
   } else {
-    count = S.partner_count[p];
+    count = listed;
   }
   if (count == 0) return;
   int parity_new = 1 - parity_old;
