@@ -75,3 +75,25 @@ def test_batched_instances_follow_their_own_solvers():
 def test_autotuner_picks_a_candidate():
     best, timings = autotune_brick_z(configs(), candidates=(4, 8), substeps=4)
     assert best in (4, 8) and all(t > 0 for t in timings.values())
+
+
+def test_pair_lists_are_rebuilt_per_instance():
+    """An instance's pair list is rebuilt when its own particles move more than half the skin
+    (contact.py:716-735 per run), not when another instance's do: instance 0 gets an upward
+    kick of 1 m/s (0.2 mm per substep against a 0.75 mm half skin), instance 1 rests."""
+    cfgs = configs()[:2]
+    batch = BatchSolver(cfgs)
+    port = BatchPort(batch)
+    dt = cfgs[0].dt * 20
+    # Knives well above their blocks: no contact.
+    idle = [ToolState([0.0, 0.0, c.size[2] + 0.05], [0, 0, 0, 1.0], [0, 0, 0], [0, 0, 0]) for c in cfgs]
+    port.advance(idle, dt, cutting=False)
+    before = batch._t["inst_pair_builds"].cpu().numpy().copy()
+    total_before = int(batch.pair_list_builds.numpy()[0])
+    # Written on the device tensor directly: a host assignment would mark every list stale.
+    batch._t["v"][batch.slices[0]] += torch.tensor([0.0, 0.0, 1.0], device=batch._t["v"].device)
+    port.advance(idle, dt, cutting=False)
+    builds = batch._t["inst_pair_builds"].cpu().numpy() - before
+    assert builds[0] >= 3, builds
+    assert builds[1] == 0, builds
+    assert int(batch.pair_list_builds.numpy()[0]) - total_before == builds[0]
