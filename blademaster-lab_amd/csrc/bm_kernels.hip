@@ -1708,58 +1708,41 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 // field velocities of the window staged in LDS; the particle loads are issued first so they
 // overlap the staging.
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D) {
-  extern __shared__ float lds[];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
-  Window w = window_of(P, D, b);
-  int F = int(P.fields);
-  int FW = F * w.W;
-  int64_t N = P.node_count;
   int CS = 8 * cells_per_brick(P);
   int slot = int(threadIdx.x);
-  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
-  f3 lp = f3{0.f, 0.f, 0.f};
-  uint2 fidx = make_uint2(0u, 0u);
-  m3 Fm = mzero(), Fp = mzero();
+  if (slot >= CS) return;
+  int64_t p = S.perm[int64_t(b) * CS + slot];
+  if (p < 0) return;
   // While no plastic flow has happened and the host has not written Fp, every Fp is I.
   bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
-  if (p >= 0) {
-    lp = ld3(S.local, p);
-    fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-    Fm = ld9(S.F, p);
-    if (fp_identity) {
+  f3 lp = ld3(S.local, p);
+  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  int32_t base = S.base_node[p];
+  m3 Fm = ld9(S.F, p), Fp;
+  if (fp_identity) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Fp.a[k] = (k % 4 == 0) ? 1.0f : 0.0f;
-    } else {
-      Fp = ld9(S.Fp, p);
-    }
+    for (int k = 0; k < 9; ++k) Fp.a[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+  } else {
+    Fp = ld9(S.Fp, p);
   }
-  for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
-    int64_t node = window_node(P, w, D, i);
-    int nf = node >= 0 ? S.node_nf[node] : 0;
-    for (int f = 0; f < F; ++f) {
-      f3 v = f3{0.f, 0.f, 0.f};
-      if (f < nf) v = ld3(S.grid_vel, int64_t(f) * N + node);
-      int j = f * w.W + i;
-      lds[j] = v.x;
-      lds[FW + j] = v.y;
-      lds[2 * FW + j] = v.z;
-    }
-  }
-  __syncthreads();
-  if (p < 0) return;
+  int64_t N = P.node_count;
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  int wb = cell_window_base(P, w, D, slot >> 3);
+  // The corner velocities come straight from grid_vel (a node array small enough to stay in
+  // L2): with no staging and no barrier, the waves of a brick run independently.
+  f3 vel[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
   m3 rate = mzero();
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     float wt;
     f3 g;
     stencil_corner(lp, inv_h, a, wt, g);
-    int idx = __mul24(field_of(fidx, a), w.W) + wb + window_corner(w, a);
-    f3 vel = f3{lds[idx], lds[FW + idx], lds[2 * FW + idx]};
-    rate.a[0] += vel.x * g.x; rate.a[1] += vel.x * g.y; rate.a[2] += vel.x * g.z;
-    rate.a[3] += vel.y * g.x; rate.a[4] += vel.y * g.y; rate.a[5] += vel.y * g.z;
-    rate.a[6] += vel.z * g.x; rate.a[7] += vel.z * g.y; rate.a[8] += vel.z * g.z;
+    rate.a[0] += vel[a].x * g.x; rate.a[1] += vel[a].x * g.y; rate.a[2] += vel[a].x * g.z;
+    rate.a[3] += vel[a].y * g.x; rate.a[4] += vel[a].y * g.y; rate.a[5] += vel[a].y * g.z;
+    rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
@@ -2328,7 +2311,7 @@ void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int6
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags, D);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
-  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), lds_bytes(p, 3, 0), st, *p, *s, D);
+  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), 0, st, *p, *s, D);
 }
 }  // namespace
 
@@ -2386,7 +2369,7 @@ int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_update_F, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 0),
+  hipLaunchKernelGGL(k_update_F, dim3(int(brick_launch(p))), dim3(brick_threads(p)), 0,
                      (hipStream_t)stream, *p, *s, divisors_of(p));
   return launch_status();
 }
