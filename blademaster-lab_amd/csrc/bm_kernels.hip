@@ -1434,9 +1434,8 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   int FW = F * w.W;
   int cpb = cells_per_brick(P);
   int CS = 8 * cpb;
-  float* vel = lds;             // [6][F][W]: old_v xyz, new_v xyz
-  float* win = lds + 6 * FW;    // [3][F][W]
-  float* cc = lds + 9 * FW;     // [3][CS]
+  float* win = lds;             // [3][F][W]
+  float* cc = lds + 3 * FW;     // [3][CS]
   int64_t N = P.node_count;
   int slot = int(threadIdx.x);
   int c = slot >> 3;
@@ -1444,67 +1443,62 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   bool live = p >= 0;
   f3 lp = f3{0.f, 0.f, 0.f}, xp = lp, vold = lp;
   uint2 fidx = make_uint2(0u, 0u);
+  int32_t base = 0;
   if (live) {
     lp = ld3(S.local, p);
     fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
     xp = ld3(S.x, p);
     vold = ld3(S.v, p);
+    base = S.base_node[p];
   }
+  // The window's field count and the cleared cross-field accumulator; the barrier that
+  // orders the clearing before the accumulation comes after the velocity gather below, so
+  // the waves wait on their global loads and not on each other.
   int my_nf = 0;
   for (int i = threadIdx.x; i < w.W; i += blockDim.x) {
     int64_t node = window_node(P, w, D, i);
-    int nf = node >= 0 ? S.node_nf[node] : 0;
-    my_nf = max(my_nf, nf);
+    my_nf = max(my_nf, node >= 0 ? S.node_nf[node] : 0);
     for (int f = 0; f < F; ++f) {
-      f3 o = f3{0.f, 0.f, 0.f}, nv = o;
-      if (f < nf) {
-        o = ld3(S.old_v, int64_t(f) * N + node);
-        nv = ld3(S.new_v, int64_t(f) * N + node);
-      }
       int j = f * w.W + i;
-      vel[j] = o.x;
-      vel[FW + j] = o.y;
-      vel[2 * FW + j] = o.z;
-      vel[3 * FW + j] = nv.x;
-      vel[4 * FW + j] = nv.y;
-      vel[5 * FW + j] = nv.z;
       win[j] = 0.0f;
       win[FW + j] = 0.0f;
       win[2 * FW + j] = 0.0f;
     }
   }
   wave_max_store(my_nf, wave_nf);
-  __syncthreads();
   float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
   int lane = int(threadIdx.x & 63);
   float mine[3] = {0.f, 0.f, 0.f};
+  f3 vp = f3{0.f, 0.f, 0.f};
+  if (live) {
+    // The corner velocities come straight from old_v / new_v (node arrays small enough to
+    // stay in L2).
+    int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+    float wt[8];
+    stencil_weights(lp, wt);
+    f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int64_t s = int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz));
+      f3 o = ld3(S.old_v, s), nv = ld3(S.new_v, s);
+      pic = pic + wt[a] * nv;
+      dv = dv + wt[a] * (nv - o);
+    }
+    vp = (1.0f - flip) * pic + flip * (vold + dv);
+    xp = xp + dt * pic;
+    st3(S.v, p, vp);
+    st3(S.x, p, xp);
+    if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
+    if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin)) {
+      if (S.instance != nullptr)
+        S.inst_pair_dirty[S.instance[p]] = 1;
+      else
+        S.pair_dirty[0] = 1;
+    }
+  }
+  __syncthreads();
   if (__ballot(live) != 0) {
     int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
-    f3 vp = f3{0.f, 0.f, 0.f};
-    if (live) {
-      float wt[8];
-      stencil_weights(lp, wt);
-      f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        int idx = __mul24(field_of(fidx, a), w.W) + wb + window_corner(w, a);
-        f3 o = f3{vel[idx], vel[FW + idx], vel[2 * FW + idx]};
-        f3 nv = f3{vel[3 * FW + idx], vel[4 * FW + idx], vel[5 * FW + idx]};
-        pic = pic + wt[a] * nv;
-        dv = dv + wt[a] * (nv - o);
-      }
-      vp = (1.0f - flip) * pic + flip * (vold + dv);
-      xp = xp + dt * pic;
-      st3(S.v, p, vp);
-      st3(S.x, p, xp);
-      if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
-      if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin)) {
-        if (S.instance != nullptr)
-          S.inst_pair_dirty[S.instance[p]] = 1;
-        else
-          S.pair_dirty[0] = 1;
-      }
-    }
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       int f = field_of(fidx, a);
@@ -2283,7 +2277,7 @@ bool brick_args_ok(const bm_params* p, const bm_state* s) {
   if (p->bricks[0] * p->brick[0] < p->dims[0] - 1 || p->bricks[1] * p->brick[1] < p->dims[1] - 1 ||
       p->bricks[2] * p->brick[2] < p->dims[2] - 1)
     return false;
-  return lds_bytes(p, 7, 7) <= 65536 && lds_bytes(p, 9, 3) <= 65536;
+  return lds_bytes(p, 7, 7) <= 65536 && lds_bytes(p, 3, 3) <= 65536;
 }
 // Threads of a brick workgroup: one per particle slot (8 per cell) and at least one per window
 // node, in whole waves (at most 1024 by brick_args_ok and the LDS bound on the window).
@@ -2309,7 +2303,7 @@ void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int6
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
-  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 9, 3), st, *p, *s, digit, g2p_flags, D);
+  hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
   hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), 0, st, *p, *s, D);
 }
@@ -2355,7 +2349,7 @@ int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) 
 
 int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_g2p, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 9, 3), (hipStream_t)stream,
+  hipLaunchKernelGGL(k_g2p, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 3, 3), (hipStream_t)stream,
                      *p, *s, int64_t(1), 0, divisors_of(p));
   return launch_status();
 }
