@@ -1333,15 +1333,18 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
-  int nf = S.node_nf[node];
-  if (nf == 0) return;
   int64_t N = P.node_count;
   int nx, ny, nz;
   node_coords(P, D, node, nx, ny, nz);
+  // Field 0 is gathered in the same round of loads as the node's field count: most nodes
+  // have one field and need nothing else.
+  float g0[7];
+  gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, 0, g0);
+  int nf = S.node_nf[node];
+  if (nf == 0) return;
   constexpr int kMaxLocal = 4;
   if (nf == 1) {
-    float g[7];
-    gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, 0, g);
+    const float* g = g0;
     S.grid_mass[node] = g[0];
     if (g[0] > 0.0f) {
       f3 pm = f3{g[1], g[2], g[3]}, j = f3{g[4], g[5], g[6]};
@@ -1359,7 +1362,12 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
 #pragma unroll
     for (int f = 0; f < kMaxLocal; ++f) {
       if (f < nf) {
-        gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, f, g[f]);
+        if (f == 0) {
+#pragma unroll
+          for (int c = 0; c < 7; ++c) g[0][c] = g0[c];
+        } else {
+          gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, f, g[f]);
+        }
         code[f] = S.field_code[int64_t(f) * N + node];
         S.grid_mass[int64_t(f) * N + node] = g[f][0];
         total += g[f][0];
@@ -1547,15 +1555,18 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
-  int nf = S.node_nf[node];
-  if (nf == 0) return;
   int64_t N = P.node_count;
   int nx, ny, nz;
   node_coords(P, D, node, nx, ny, nz);
+  // Field 0 and its mass are loaded in the same round as the node's field count.
+  float g0[3];
+  gather_node<3>(P, D, S.remap_partial, nx, ny, nz, 0, g0);
+  float m0 = S.grid_mass[node];
+  int nf = S.node_nf[node];
+  if (nf == 0) return;
   if (nf == 1) {
-    float g[3];
-    gather_node<3>(P, D, S.remap_partial, nx, ny, nz, 0, g);
-    float m = S.grid_mass[node];
+    const float* g = g0;
+    float m = m0;
     f3 vel = f3{0.f, 0.f, 0.f};
     if (m > 0.0f) vel = f3{g[0], g[1], g[2]} / m;
     st3(S.grid_vel, node, vel);
@@ -1569,8 +1580,13 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
 #pragma unroll
     for (int f = 0; f < kMaxLocal; ++f) {
       if (f < nf) {
-        gather_node<3>(P, D, S.remap_partial, nx, ny, nz, f, g[f]);
-        m[f] = S.grid_mass[int64_t(f) * N + node];
+        if (f == 0) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) g[0][c] = g0[c];
+        } else {
+          gather_node<3>(P, D, S.remap_partial, nx, ny, nz, f, g[f]);
+        }
+        m[f] = f == 0 ? m0 : S.grid_mass[int64_t(f) * N + node];
         code[f] = S.field_code[int64_t(f) * N + node];
         total += m[f];
       }
