@@ -286,7 +286,13 @@ __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_s
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
                                              const ListRegion& L);
 
+__device__ __forceinline__ int corner_node(int32_t base, int a, int dyz, int dz) { return base + corner_offset(a, dyz, dz); }
+
 constexpr int kRigidSums = 12;  // board 3, reaction 3, torque 3, work, speed, pad
+// topology_dirty bits: register every particle (host writes, a full record of relabelled
+// particles), or only around the particles relabelled last step (`changed`).
+constexpr int kTopoFull = 1;
+constexpr int kTopoChanged = 2;
 constexpr int kInstSlots = 16;  // slots per instance of the batched contact sums (inst_sums)
 
 // rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
@@ -314,11 +320,22 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     const int32_t* used = S.pair_used + int64_t(parity_new) * P.pair_capacity;
     for (int64_t a = p; a < count; a += stride) keys[used[a]] = -1;
   }
-  if ((mode & kTopologyPrep) != 0 && S.topology_dirty[0] != 0) {
+  const int topo = (mode & kTopologyPrep) != 0 ? S.topology_dirty[0] : 0;
+  if (topo & kTopoFull) {
     // register, first half (solver.py:392-393): empty field table before the inserts.
     int64_t nodes = active_node_count(P);
     for (int64_t i = p; i < nodes; i += stride) {
       int64_t node = active_node(S, i);
+      for (int f = 0; f < int(P.fields); ++f) S.field_code[int64_t(f) * P.node_count + node] = -1;
+      S.node_nf[node] = 0;
+    }
+  } else if (topo & kTopoChanged) {
+    // Only the stencil nodes of the particles relabelled last step can change their codes:
+    // empty those (the particles around them register again in k_prep).
+    int64_t count = min(S.changed_count[0], int(P.changed_capacity));
+    int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+    for (int64_t a = p; a < count * 8; a += stride) {
+      int64_t node = corner_node(S.base_node[S.changed[a >> 3]], int(a & 7), dyz, dz);
       for (int f = 0; f < int(P.fields); ++f) S.field_code[int64_t(f) * P.node_count + node] = -1;
       S.node_nf[node] = 0;
     }
@@ -465,7 +482,31 @@ constexpr int kPrepBlock = 1024;  // k_prep: block 0 reduces the partials with 1
 __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
                                                  int parity_new, int insert_blocks) {
   if (blockIdx.x > 0 && int(blockIdx.x) <= insert_blocks) {
-    if ((mode & kTopologyPrep) == 0 || S.topology_dirty[0] == 0) return;
+    const int topo = S.topology_dirty[0];
+    if ((mode & kTopologyPrep) == 0 || topo == 0) return;
+    if ((topo & kTopoFull) == 0) {
+      // After relabelling only: every particle of the 27 cells around a relabelled particle's
+      // cell touches a node emptied in k_rigid and registers again; the other particles' nodes
+      // and slots are unchanged.  A particle met twice registers twice, finding its slots.
+      int64_t count = min(S.changed_count[0], int(P.changed_capacity));
+      int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
+      const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
+      const int64_t cpb = int64_t(BX) * BY * BZ;
+      int64_t stride = int64_t(insert_blocks) * blockDim.x;
+      // One thread per (relabelled particle, neighbouring cell, particle slot).
+      for (int64_t a = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; a < count * 27 * 8; a += stride) {
+        int64_t base = S.base_node[S.changed[a / (27 * 8)]];
+        int k = int((a >> 3) % 27);
+        int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
+          continue;
+        int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
+        int64_t cell = brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ;
+        int32_t q = S.perm[8 * cell + (a & 7)];
+        if (q >= 0) insert_particle(P, S, q);
+      }
+      return;
+    }
     // Registration, first half: one anchor particle per cell with even coordinates in its
     // brick (its first particle) claims its code at its 8 nodes.  With even brick sides every
     // node is a corner of exactly one such cell, so these claims meet no contention; the
@@ -813,7 +854,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   const int32_t mark_p = S.pair_mark[p];
   const int32_t listed = S.partner_count[p];
   // Registration, second half (the anchors claimed their slots in k_prep).
-  if (topology && S.topology_dirty[0] != 0) insert_particle(P, S, p);
+  if (topology && (S.topology_dirty[0] & kTopoFull) != 0) insert_particle(P, S, p);
   ListRegion L = list_region(P, S, instance_of(S, p));
   int tag = L.tag;
   float reach = S.list_reach[0];
@@ -963,7 +1004,6 @@ __global__ void k_check_pair_list(const bm_params P, bm_state S) {
 }
 
 // ------------------------------------------------------------------------ topology
-__device__ __forceinline__ int corner_node(int32_t base, int a, int dyz, int dz) { return base + corner_offset(a, dyz, dz); }
 
 // register (material.py:50-65): the field slot of the particle's transfer code at each of
 // its 8 nodes, claiming the first empty slot when the code is new there.  Used slots stay a
@@ -2013,14 +2053,16 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   if (fabsf(local.x) <= 6.0f * spacing && fabsf(local.y) <= hf.y && (behind || pinched)) {
     int64_t side = local.x < 0.0f ? 1 : 2;
     S.codes[p] = code + digit * side;
-    // A new side code changes the grid fields and the pair candidates: the lists around the
-    // relabelled particle are rebuilt next step (all lists if the record overflows).
-    S.topology_dirty[0] = 1;
+    // A new side code changes the grid fields and the pair candidates: the fields and lists
+    // around the relabelled particle are rebuilt next step (all of them if the record
+    // overflows).
     int k = atomicAdd(S.changed_count, 1);
     if (k < P.changed_capacity) {
       S.changed[k] = int32_t(p);
+      atomicOr(S.topology_dirty, kTopoChanged);
     } else {
       S.pair_dirty[0] = 1;
+      atomicOr(S.topology_dirty, kTopoFull);
     }
   }
 }
