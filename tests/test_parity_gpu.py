@@ -229,3 +229,52 @@ def test_relabelled_pair_lists_match_a_full_rebuild():
     assert count_a == count_b and count_a > 0 and count_a != unchanged
     for name in ("x", "v"):
         assert rel(getattr(a, name).numpy().ravel(), getattr(b, name).numpy().ravel()) < 1e-6, name
+
+
+def field_tables_hold_the_codes(s):
+    """Every particle finds its transfer code in its slot at each of its 8 nodes, and each
+    node's used slots hold exactly the distinct codes of the particles around it."""
+    T = s._t
+    codes = s.codes.numpy()
+    fc = T["field_code"].cpu().numpy()
+    nf = T["node_nf"].cpu().numpy()
+    fi = T["field_idx"].cpu().numpy().astype(np.int64)
+    dz, dyz = int(s.dims[2]), int(s.dims[1] * s.dims[2])
+    offsets = np.array([((a >> 2) & 1) * dyz + ((a >> 1) & 1) * dz + (a & 1) for a in range(8)])
+    nodes = T["base_node"].cpu().numpy().astype(np.int64)[:, None] + offsets[None, :]
+    np.testing.assert_array_equal(fc[fi, nodes], np.repeat(codes[:, None], 8, 1))
+    expected = np.unique(np.stack([nodes.ravel(), np.repeat(codes, 8)], 1), axis=0)
+    touched = np.unique(nodes)
+    slots = [(node, fc[f, node]) for node in touched for f in range(nf[node])]
+    np.testing.assert_array_equal(np.array(sorted(slots)), expected)
+
+
+def test_incremental_registration_matches_a_full_one():
+    """Labels changed on the device (as assign does): registering again only around the
+    relabelled particles gives the field tables of a full registration (material.py:50-65)
+    and the same substep."""
+    cfg = small_cfg(dt=0.00025)
+
+    def make():
+        s = Solver(cfg)
+        s.codes.assign(np.where(s.reference[:, 0] < 0, 1, 2))
+        s.v.assign(np.random.default_rng(5).normal(0, 0.01, (s.n, 3)).astype(np.float32))
+        s.step(cutting=False)
+        return s
+
+    a, b = make(), make()
+    ref = a.reference
+    moved = np.nonzero((ref[:, 0] > 0) & (ref[:, 1] > 0))[0].astype(np.int32)
+    for s in (a, b):
+        T = s._t
+        T["codes"][torch.from_numpy(moved).long().to(T["codes"].device)] = 1
+        T["changed"][: len(moved)] = torch.from_numpy(moved).to(T["changed"].device)
+        T["changed_count"].fill_(len(moved))
+    a._t["topology_dirty"].fill_(2)  # only around the relabelled particles
+    b._t["topology_dirty"].fill_(1)  # every particle
+    for s in (a, b):
+        s.step(cutting=False)
+        field_tables_hold_the_codes(s)
+        assert int(s._t["topology_dirty"].item()) == 0
+    for name in ("x", "v"):
+        assert rel(getattr(a, name).numpy().ravel(), getattr(b, name).numpy().ravel()) < 1e-6, name
