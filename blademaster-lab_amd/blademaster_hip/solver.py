@@ -161,6 +161,7 @@ class Solver:
         T["errors"] = t((3,), i32)
         T["topology_dirty"] = t((1,), i32, 1)
         T["fp_identity"] = t((1,), i32, 1)
+        T["update_pending"] = t((1,), i32)
         # pair contacts
         P_ = self.pair_capacity
         self.list_buckets = _pow2_at_least(2 * n)
@@ -437,8 +438,13 @@ class Solver:
             buffer, rotations, index = self._t["single_target"], self._t["single_rotation"], 0
         self._call("bm_tool_step", ctypes.c_void_p(buffer.data_ptr()), ctypes.c_void_p(rotations.data_ptr()), index)
 
-    def _step_kernels(self, target_edge, cutting, cut_index, buffer=None, index=0, rotations=None, orientation=None):
-        """One substep: contacts, material update, tool update, side labels (solver.py:920-940)."""
+    def _step_kernels(self, target_edge, cutting, cut_index, buffer=None, index=0, rotations=None, orientation=None,
+                      defer=False):
+        """One substep: contacts, material update, tool update, side labels (solver.py:920-940).
+
+        defer: leave this substep's F update to the next substep's P2G when no labels changed
+        (`bm_substep_deferred`); the next call must follow with nothing reading F in between.
+        """
         cfg = self.config
         if buffer is None and not cfg.external_tool:
             self.single_target.assign([target_edge])
@@ -449,7 +455,7 @@ class Solver:
         rptr = ctypes.c_void_p(rotations.data_ptr()) if rotations is not None else ctypes.c_void_p(
             self._t["single_rotation"].data_ptr())
         self._call(
-            "bm_substep",
+            "bm_substep_deferred" if defer else "bm_substep",
             self._cut_bits(cutting),
             int(bool(cutting)),
             3**cut_index,
@@ -512,8 +518,11 @@ class Solver:
                 with torch.cuda.graph(graph):
                     for fn in before:
                         fn()
+                    # Inside the recorded batch each substep's F update may wait for the next
+                    # substep's P2G; the last substep completes its own.
                     for i in range(len(targets)):
-                        self._step_kernels(targets[i], cutting, cut_index, buffer, i, rotations)
+                        self._step_kernels(targets[i], cutting, cut_index, buffer, i, rotations,
+                                           defer=i + 1 < len(targets))
                         if after_step is not None:
                             after_step(i)
                     for fn in after:

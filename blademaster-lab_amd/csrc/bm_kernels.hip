@@ -1211,6 +1211,8 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // stores; a particle on another field slot (a cut through the node) adds its values to an LDS
 // node window with atomics.  Each window node then gathers its cells' corner sums in fixed
 // order, adds the window, and the brick's partial goes out with plain stores.
+__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int64_t p, f3 lp, uint2 fidx);
+
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
@@ -1247,6 +1249,8 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
   int lane = int(threadIdx.x & 63);
   float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  // The last substep's F update, when it was deferred to this pass (bm_substep_deferred).
+  const bool pending = S.update_pending != nullptr && S.update_pending[0] != 0;
   if (__ballot(live) != 0) {
     f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
     m3 Pk = mzero();
@@ -1254,11 +1258,11 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
     if (live) {
       vp = ld3(S.v, p);
       imp = ld3(S.impulses, p);
-      Pk = ld9(S.stress, p);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
       lp = ld3(S.local, p);
       fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+      Pk = pending ? update_particle(P, S, p, lp, fidx) : ld9(S.stress, p);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
     }
     int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
 #pragma unroll 2
@@ -1371,6 +1375,8 @@ __device__ __forceinline__ void node_coords(const bm_params& P, const Divisors& 
 
 __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S, const Divisors D) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  // Every P2G block has read update_pending by now.
+  if (i == 0 && S.update_pending != nullptr) S.update_pending[0] = 0;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
   int64_t N = P.node_count;
@@ -1754,20 +1760,13 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp, mu, lam);
 }
 
-// update_F_plastic (material.py:445-475) for one brick, one lane per particle slot, with the
-// field velocities of the window staged in LDS; the particle loads are issued first so they
-// overlap the staging.
-__global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D) {
-  int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
-  int CS = 8 * cells_per_brick(P);
-  int slot = int(threadIdx.x);
-  if (slot >= CS) return;
-  int64_t p = S.perm[int64_t(b) * CS + slot];
-  if (p < 0) return;
+// update_F_plastic (material.py:445-475) for one particle: F += dt Σ_a v_a ⊗ ∇w_a over its
+// 8 corners, then the stress.  The corner velocities of its field come straight from
+// grid_vel (a node array small enough to stay in L2).  Stores F (and Fp after plastic flow)
+// and the stress, and returns the stress.
+__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int64_t p, f3 lp, uint2 fidx) {
   // While no plastic flow has happened and the host has not written Fp, every Fp is I.
   bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
-  f3 lp = ld3(S.local, p);
-  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
   int32_t base = S.base_node[p];
   m3 Fm = ld9(S.F, p), Fp;
   if (fp_identity) {
@@ -1779,8 +1778,6 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
   int64_t N = P.node_count;
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  // The corner velocities come straight from grid_vel (a node array small enough to stay in
-  // L2): with no staging and no barrier, the waves of a brick run independently.
   f3 vel[8];
 #pragma unroll
   for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
@@ -1797,7 +1794,31 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
   st9(S.F, p, Fm);
-  st9(S.stress, p, stress_of(P, S, p, Fm, Fp));
+  m3 stress = stress_of(P, S, p, Fm, Fp);
+  st9(S.stress, p, stress);
+  return stress;
+}
+
+// The F update of a substep, one lane per particle slot of a brick.  With `defer` and no label
+// change this substep (the field slots stay as they are), it is left to the next P2G, which
+// updates each particle and uses its stress at once: update_pending tells that P2G.
+// The workgroups stride over the bricks (at most kUpdateBlocks of them, a multiple of 8, so a
+// workgroup keeps the bricks of its XCD): a deferred launch then costs little.
+constexpr int kUpdateBlocks = 1024;
+__global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
+                                                   int nbricks) {
+  if (defer && S.update_pending != nullptr && S.topology_dirty[0] == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.update_pending[0] = 1;
+    return;
+  }
+  int CS = 8 * cells_per_brick(P);
+  int slot = int(threadIdx.x);
+  if (slot >= CS) return;
+  for (int k = int(blockIdx.x); k < nbricks; k += int(gridDim.x)) {
+    int b = active_brick(S, brick_of_block(k, nbricks));
+    int64_t p = S.perm[int64_t(b) * CS + slot];
+    if (p >= 0) update_particle(P, S, p, ld3(S.local, p), reinterpret_cast<const uint2*>(S.field_idx)[p]);
+  }
 }
 
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
@@ -2354,7 +2375,7 @@ void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
 // pair_dirty (2) once every reader of this step is done; g2p_flags: assign (1) and the
 // pair-list staleness check (2) on the moved particles.
 void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
-                      hipStream_t st) {
+                      hipStream_t st, int defer = 0) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
   const Divisors D = divisors_of(p);
   hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
@@ -2363,7 +2384,7 @@ void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int6
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
-  hipLaunchKernelGGL(k_update_F, dim3(nb), dim3(tb), 0, st, *p, *s, D);
+  hipLaunchKernelGGL(k_update_F, dim3(nb < kUpdateBlocks ? nb : kUpdateBlocks), dim3(tb), 0, st, *p, *s, D, defer, nb);
 }
 }  // namespace
 
@@ -2421,8 +2442,9 @@ int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_update_F, dim3(int(brick_launch(p))), dim3(brick_threads(p)), 0,
-                     (hipStream_t)stream, *p, *s, divisors_of(p));
+  int nb = int(brick_launch(p));
+  hipLaunchKernelGGL(k_update_F, dim3(nb < kUpdateBlocks ? nb : kUpdateBlocks), dim3(brick_threads(p)), 0,
+                     (hipStream_t)stream, *p, *s, divisors_of(p), 0, nb);
   return launch_status();
 }
 
@@ -2463,9 +2485,9 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
   return launch_status();
 }
 
-int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
-               int32_t parity, int32_t accumulate, const float* targets, const float* rotations, int32_t index,
-               void* stream) {
+namespace {
+int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit, int32_t parity,
+            int32_t accumulate, const float* targets, const float* rotations, int32_t index, void* stream, int defer) {
   if (!brick_args_ok(p, s) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
   if (p->rigid_blocks < blocks_for(p->n)) return -2;
   bool external = p->tool_model < 0;
@@ -2498,9 +2520,22 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
   // 3. particle pairs across cuts (partner lists rebuilt first when stale)
   hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p, 1);
   // 4. transfers; G2P labels the moved particles and checks the pair list's skin
-  launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st);
+  launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st, defer);
   if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
+}
+}  // namespace
+
+int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
+               int32_t parity, int32_t accumulate, const float* targets, const float* rotations, int32_t index,
+               void* stream) {
+  return substep(p, s, cut_bits, cutting, digit, parity, accumulate, targets, rotations, index, stream, 0);
+}
+
+int bm_substep_deferred(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
+                        int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
+                        int32_t index, void* stream) {
+  return substep(p, s, cut_bits, cutting, digit, parity, accumulate, targets, rotations, index, stream, 1);
 }
 
 int bm_load_external_state(const bm_params* p, const bm_state* s, const float* packed49, void* stream) {

@@ -83,6 +83,8 @@ typedef struct bm_state {
   float *interval_impulse, *interval_angular;
   int32_t *errors, *topology_dirty;
   int32_t *fp_identity;    /* [1] 1 while every Fp is exactly I (no plastic flow, no host write) */
+  int32_t *update_pending; /* [1] 1 while the F update of the last substep waits for the next
+                            * P2G (bm_substep_deferred); null disables the deferral */
   /* batched instances (null for a single solver): the instance of each particle and its Lame
    * constants; tool_pos/vel/rot/omega, mobility, reaction, torque, board, work and the
    * interval sums are then per instance ([instances][...]) */
@@ -207,6 +209,15 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
 int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
                int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
                int32_t index, void* stream);
+
+/* As bm_substep (the same Solver._step_kernels, solver.py:920-940), but when no labels
+ * changed in this substep its F update (update_F_plastic, material.py:445-475) is left to the
+ * P2G of the next bm_substep or bm_substep_deferred on the same stream, which updates each
+ * particle and uses its new stress at once.  Nothing may read F, Fp or the stress in between;
+ * a sequence ends with bm_substep, which completes its own update. */
+int bm_substep_deferred(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
+                        int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
+                        int32_t index, void* stream);
 
 /* Held-blade grid constraint (blade_constraint.py:13-324): face contacts of the particles
  * beside the blade's broad faces, projected-gradient solve of W lambda >= rhs, lambda >= 0 on
