@@ -70,7 +70,7 @@ STATE_FIELDS = [
     ("tool_pos", c_f32p), ("tool_vel", c_f32p), ("tool_rot", c_f32p), ("tool_omega", c_f32p), ("mobility", c_f32p),
     ("reaction", c_f32p), ("torque", c_f32p), ("board", c_f32p), ("work", c_f32p), ("speed", c_f32p),
     ("table", c_f32p), ("interval_impulse", c_f32p), ("interval_angular", c_f32p),
-    ("errors", c_i32p), ("topology_dirty", c_i32p), ("fp_identity", c_i32p), ("update_pending", c_i32p), ("instance", c_i32p),
+    ("errors", c_i32p), ("topology_dirty", c_i32p), ("fp_identity", c_i32p), ("update_pending", c_i32p), ("update_done", c_i32p), ("instance", c_i32p),
     ("inst_mu", c_f32p), ("inst_lam", c_f32p), ("inst_sums", c_f32p),
     ("inst_buckets", c_i32p), ("inst_pair_builds", c_i32p), ("inst_pair_dirty", c_i32p),
     ("pair_keys", c_i64p), ("pair_history", c_f32p), ("pair_used", c_i32p), ("pair_count", c_i32p),

@@ -162,6 +162,7 @@ class Solver:
         T["topology_dirty"] = t((1,), i32, 1)
         T["fp_identity"] = t((1,), i32, 1)
         T["update_pending"] = t((1,), i32)
+        T["update_done"] = t((max(n, 1),), i32)
         # pair contacts
         P_ = self.pair_capacity
         self.list_buckets = _pow2_at_least(2 * n)

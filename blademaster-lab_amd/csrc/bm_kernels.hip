@@ -1250,7 +1250,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   int lane = int(threadIdx.x & 63);
   float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   // The last substep's F update, when it was deferred to this pass (bm_substep_deferred).
-  const bool pending = S.update_pending != nullptr && S.update_pending[0] != 0;
+  const int pending = S.update_pending != nullptr ? S.update_pending[0] : 0;
   if (__ballot(live) != 0) {
     f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
     m3 Pk = mzero();
@@ -1260,7 +1260,15 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       imp = ld3(S.impulses, p);
       lp = ld3(S.local, p);
       fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      Pk = pending ? update_particle(P, S, p, lp, fidx) : ld9(S.stress, p);
+      // pending 2: the particles marked in update_done were updated at the end of the last
+      // substep; their marks are cleared here.
+      const bool done = pending == 2 && S.update_done[p] != 0;
+      if (pending && !done) {
+        Pk = update_particle(P, S, p, lp, fidx);
+      } else {
+        Pk = ld9(S.stress, p);
+        if (done) S.update_done[p] = 0;
+      }
 #pragma unroll
       for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
     }
@@ -1799,16 +1807,39 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int6
   return stress;
 }
 
-// The F update of a substep, one lane per particle slot of a brick.  With `defer` and no label
-// change this substep (the field slots stay as they are), it is left to the next P2G, which
-// updates each particle and uses its stress at once: update_pending tells that P2G.
+// The F update of a substep, one lane per particle slot of a brick.  With `defer` it is left to
+// the next P2G, which updates each particle and uses its stress at once (update_pending tells
+// that P2G), for every particle whose field slots stay as they are: all of them without a
+// label change; after relabelling only, all but the particles of the 27 cells around each
+// relabelled one, which register again before that P2G and are updated here (once each,
+// marked in update_done).  A full re-registration updates every particle here.
 // The workgroups stride over the bricks (at most kUpdateBlocks of them, a multiple of 8, so a
 // workgroup keeps the bricks of its XCD): a deferred launch then costs little.
 constexpr int kUpdateBlocks = 1024;
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
-  if (defer && S.update_pending != nullptr && S.topology_dirty[0] == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) S.update_pending[0] = 1;
+  const int topo = S.topology_dirty[0];
+  if (defer && S.update_pending != nullptr && S.update_done != nullptr && (topo & kTopoFull) == 0) {
+    // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.update_pending[0] = (topo & kTopoChanged) ? 2 : 1;
+    if (topo & kTopoChanged) {
+      int64_t count = min(S.changed_count[0], int(P.changed_capacity));
+      int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
+      const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
+      const int64_t cpb = int64_t(BX) * BY * BZ;
+      int64_t stride = int64_t(gridDim.x) * blockDim.x;
+      for (int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; a < count * 27 * 8; a += stride) {
+        int64_t base = S.base_node[S.changed[a / (27 * 8)]];
+        int k = int((a >> 3) % 27);
+        int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
+          continue;
+        int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
+        int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
+        if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
+          update_particle(P, S, q, ld3(S.local, q), reinterpret_cast<const uint2*>(S.field_idx)[q]);
+      }
+    }
     return;
   }
   int CS = 8 * cells_per_brick(P);

@@ -85,6 +85,8 @@ typedef struct bm_state {
   int32_t *fp_identity;    /* [1] 1 while every Fp is exactly I (no plastic flow, no host write) */
   int32_t *update_pending; /* [1] 1 while the F update of the last substep waits for the next
                             * P2G (bm_substep_deferred); null disables the deferral */
+  int32_t *update_done;    /* [n] 1 for a particle whose deferred update already ran (it is
+                            * registered again before the next P2G); zero between substeps */
   /* batched instances (null for a single solver): the instance of each particle and its Lame
    * constants; tool_pos/vel/rot/omega, mobility, reaction, torque, board, work and the
    * interval sums are then per instance ([instances][...]) */
