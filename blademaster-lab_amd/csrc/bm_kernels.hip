@@ -1816,6 +1816,7 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int6
 // The workgroups stride over the bricks (at most kUpdateBlocks of them, a multiple of 8, so a
 // workgroup keeps the bricks of its XCD): a deferred launch then costs little.
 constexpr int kUpdateBlocks = 1024;
+constexpr int kDeferredUpdateBlocks = 128;
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
   const int topo = S.topology_dirty[0];
@@ -2415,7 +2416,10 @@ void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int6
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
-  hipLaunchKernelGGL(k_update_F, dim3(nb < kUpdateBlocks ? nb : kUpdateBlocks), dim3(tb), 0, st, *p, *s, D, defer, nb);
+  // A deferred launch does little (the early updates after relabelling, or all of them after a
+  // full registration): a small grid keeps its cost low.
+  int ub = defer ? kDeferredUpdateBlocks : kUpdateBlocks;
+  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer, nb);
 }
 }  // namespace
 
