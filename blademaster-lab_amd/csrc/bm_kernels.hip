@@ -8,10 +8,20 @@
 //    walks at most F slots of one node instead of a list.
 //  * The stencil is rebuilt in registers from the base node and three local fractions (16 B
 //    per particle) instead of loading 8 node ids, 8 weights and 24 gradient terms (128 B).
-//  * Scalar reductions onto one address (speed, reaction, torque, board, work) are reduced
-//    across the wavefront first, so one lane per wave issues the global atomic.
-//  * Pair contacts use a sort-based cell list (bucket counting sort + scan) rebuilt on the
-//    device when the list is stale, instead of per-bucket linked lists.
+//  * The contact sums of a step (speed, reaction, torque, board, work) are reduced per
+//    workgroup and stored as partials that one workgroup adds in order: no global atomics.
+//    Batched instances add per-workgroup sums into 16 slots per instance.
+//  * Particles are binned into bricks of cells; P2G and the MUSL re-scatter sum each cell's
+//    particles with DPP moves and write per-brick window partials that the node kernels
+//    gather in a fixed order, so the transfers need no global float atomics.
+//  * Pair candidates come from bucket chains whose heads carry the build count, so a stale
+//    head reads as an empty bucket and no clearing pass is needed; after relabelling only,
+//    just the lists around the relabelled particles are rebuilt.  Batched instances keep one
+//    list each.
+//  * After relabelling only, registration empties and refills just the nodes around the
+//    relabelled particles.
+//  * Inside a recorded interval the F update of a substep runs in the next substep's P2G,
+//    which uses the new stress at once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1210,7 +1220,9 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // and lane 8c + a stores the sums of corner a in a per-(cell, corner) LDS table with plain
 // stores; a particle on another field slot (a cut through the node) adds its values to an LDS
 // node window with atomics.  Each window node then gathers its cells' corner sums in fixed
-// order, adds the window, and the brick's partial goes out with plain stores.
+// order, adds the window, and the brick's partial goes out with plain stores.  When the last
+// substep's F update is pending (update_pending), each lane first updates its particle's F and
+// stress (update_F_plastic, material.py:445-475) and uses that stress.
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int64_t p, f3 lp, uint2 fidx);
 
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
@@ -1480,12 +1492,11 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
 
 __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp);
 
-// g2p_remap (material.py:384-401) for one brick: grid velocities of the window are staged in
-// LDS, each particle lane gathers its 8 corners, updates v and x, and re-scatters m w v
+// g2p_remap (material.py:384-401) for one brick: each particle lane reads the old and new
+// velocities of its 8 corners from the grid arrays, updates v and x, and re-scatters m w v
 // through the same cell-corner table as k_p2g into remap_partial.  In a substep the updated
 // particle is also labelled (assign, contact.py:996-1039) and checked against the pair
-// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.  The
-// particle loads are issued before the window staging so the two latencies overlap.
+// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.
 __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags,
                                               const Divisors D) {
   extern __shared__ float lds[];
@@ -2405,7 +2416,8 @@ void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
 }
 // P2G .. update_F with the brick kernels.  p2g_flags: reset topology_dirty (1) and
 // pair_dirty (2) once every reader of this step is done; g2p_flags: assign (1) and the
-// pair-list staleness check (2) on the moved particles.
+// pair-list staleness check (2) on the moved particles; defer: leave the F update to the
+// next P2G where the field slots allow it.
 void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
                       hipStream_t st, int defer = 0) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
