@@ -577,11 +577,14 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     for (int k = 0; k < kRigidSums; ++k) part[threadIdx.x >> 6][k] = acc[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < int(blockDim.x >> 6); ++w)
-#pragma unroll
-      for (int k = 0; k < kRigidSums; ++k) part[0][k] = (k == 10) ? fmaxf(part[0][k], part[w][k]) : part[0][k] + part[w][k];
+  // One thread per sum adds the waves' partials in wave order.
+  if (threadIdx.x < kRigidSums) {
+    int k = int(threadIdx.x);
+    float a = part[0][k];
+    for (int w = 1; w < int(blockDim.x >> 6); ++w) a = (k == 10) ? fmaxf(a, part[w][k]) : a + part[w][k];
+    part[0][k] = a;
   }
+  __syncthreads();
   if (S.instance != nullptr) {
     // Batched instances: each thread finishes the step of some instances (their sums were
     // accumulated by k_rigid) and counts the builds of their pair lists; thread 0 also keeps
