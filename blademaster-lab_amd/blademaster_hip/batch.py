@@ -278,7 +278,7 @@ class BatchPort:
         return self.collect(states, duration_s)
 
 
-def autotune_brick_z(configs, candidates=(8, 12, 16, 24, 32), substeps=20, device="cuda:0"):
+def autotune_brick_z(configs, candidates=(4, 8, 12, 16, 24), substeps=20, device="cuda:0"):
     """Brick height with the shortest substep for this batch, by timing a prescribed
     plunge of every knife over `substeps` substeps per candidate."""
     best, timings = None, {}

@@ -133,7 +133,9 @@ def compact_stencil(reference, origin, spacing, cells):
 # (bricks are at most as tall as the kernels' LDS windows allow, and at most 1024 particle
 # slots).
 BRICK_XY = 2
-BRICK_Z = 16
+# Measured on the 1M-particle block (MI355X): 4 cells per brick in z gives 120.5 us per
+# substep, 16 gives 126.7 us on the same device, 2 and 3 are slower again.
+BRICK_Z = 4
 LDS_BYTES = 65536
 
 
@@ -144,7 +146,7 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
     cells of a brick z-fastest, and the c-th cell in this order holds the particles
     slots[8c:8c + 8] in reference order, -1 for unused slots.  The sampling puts at most 8
     particles in a cell (2 per axis, jitter stays inside the sub-cell).
-    The brick kernels keep per (field slot, window node) 7 (P2G) or 9 (G2P) floats in LDS,
+    The brick kernels keep per (field slot, window node) 7 (P2G) or 3 (G2P) floats in LDS,
     plus 7 or 3 floats per (cell, corner); the larger must fit LDS_BYTES.
     """
     cells = np.asarray(cells, dtype=np.int64)
@@ -152,7 +154,7 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
     def fits(bx, by, bz):
         window = fields * (bx + 1) * (by + 1) * (bz + 1)
         corners = 8 * bx * by * bz
-        return max(7 * window + 7 * corners, 9 * window + 3 * corners) * 4 <= LDS_BYTES
+        return max(7 * window + 7 * corners, 3 * window + 3 * corners) * 4 <= LDS_BYTES
 
     bx = by = brick_xy
     bz = int(max(1, min(cells[2], brick_z, 1024 // (8 * bx * by))))

@@ -1827,9 +1827,10 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int6
 // label change; after relabelling only, all but the particles of the 27 cells around each
 // relabelled one, which register again before that P2G and are updated here (once each,
 // marked in update_done).  A full re-registration updates every particle here.
-// The workgroups stride over the bricks (at most kUpdateBlocks of them, a multiple of 8, so a
-// workgroup keeps the bricks of its XCD): a deferred launch then costs little.
-constexpr int kUpdateBlocks = 1024;
+// The workgroups stride over the bricks (at most kUpdateThreads threads in all, in a multiple
+// of 8 workgroups, so a workgroup keeps the bricks of its XCD): a deferred launch then costs
+// little.
+constexpr int kUpdateThreads = 1 << 20;
 constexpr int kDeferredUpdateBlocks = 128;
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
@@ -2349,7 +2350,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
 }
 
 namespace {
-bool pair_args_ok(const bm_params* p, const bm_state* s) {
+static bool pair_args_ok(const bm_params* p, const bm_state* s) {
   return state_ok(p, s) && s->pair_keys && s->pair_history && s->pair_used && s->pair_count && s->pair_dirty &&
          s->pair_builds && s->list_reach && s->built_x && s->built_codes && s->partners && s->partner_count &&
          s->list_heads && s->list_links && s->changed && s->changed_count && s->pair_mark && s->pair_epoch &&
@@ -2357,15 +2358,15 @@ bool pair_args_ok(const bm_params* p, const bm_state* s) {
          p->list_buckets > 0 && (s->instance != nullptr || (p->list_buckets & (p->list_buckets - 1)) == 0) &&
          p->list_buckets <= (int64_t(1) << 28) && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
 }
-int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
+static int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
 // Workgroups of a brick kernel: one per brick with particles.
-int64_t brick_launch(const bm_params* p) { return p->active_bricks > 0 ? p->active_bricks : brick_count(p); }
-int64_t node_launch(const bm_params* p) { return p->active_nodes > 0 ? p->active_nodes : p->node_count; }
-int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
-Div32 div32(int64_t d) { return Div32{d > 1 ? uint32_t(0xFFFFFFFFu / uint32_t(d)) + 1u : 0u, uint32_t(d)}; }
-Div64 div64(int64_t d) { return Div64{d > 1 ? ~uint64_t(0) / uint64_t(d) + 1u : 0u, uint32_t(d)}; }
+static int64_t brick_launch(const bm_params* p) { return p->active_bricks > 0 ? p->active_bricks : brick_count(p); }
+static int64_t node_launch(const bm_params* p) { return p->active_nodes > 0 ? p->active_nodes : p->node_count; }
+static int64_t window_nodes(const bm_params* p) { return (p->brick[0] + 1) * (p->brick[1] + 1) * (p->brick[2] + 1); }
+static Div32 div32(int64_t d) { return Div32{d > 1 ? uint32_t(0xFFFFFFFFu / uint32_t(d)) + 1u : 0u, uint32_t(d)}; }
+static Div64 div64(int64_t d) { return Div64{d > 1 ? ~uint64_t(0) / uint64_t(d) + 1u : 0u, uint32_t(d)}; }
 // Division constants of a launch (the index ranges they need are checked by brick_args_ok).
-Divisors divisors_of(const bm_params* p) {
+static Divisors divisors_of(const bm_params* p) {
   Divisors D;
   D.bx = div32(p->brick[0]);
   D.by = div32(p->brick[1]);
@@ -2381,12 +2382,12 @@ Divisors divisors_of(const bm_params* p) {
 }
 // LDS of a brick kernel: floats_per_fw floats per (field slot, window node) plus
 // floats_per_corner per (cell, corner) of the brick.
-size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
+static size_t lds_bytes(const bm_params* p, int floats_per_fw, int floats_per_corner) {
   int64_t cells = p->brick[0] * p->brick[1] * p->brick[2];
   return size_t(floats_per_fw) * size_t(p->fields) * size_t(window_nodes(p)) * sizeof(float) +
          size_t(floats_per_corner) * size_t(8 * cells) * sizeof(float);
 }
-bool brick_args_ok(const bm_params* p, const bm_state* s) {
+static bool brick_args_ok(const bm_params* p, const bm_state* s) {
   if (!state_ok(p, s) || !s->perm || !s->p2g_partial || !s->remap_partial || !s->field_idx || !s->node_nf) return false;
   if (p->brick[0] < 1 || p->brick[1] < 1 || p->brick[2] < 1 || brick_count(p) < 1) return false;
   if ((p->active_bricks > 0 && (!s->brick_list || p->active_bricks > brick_count(p))) ||
@@ -2404,15 +2405,17 @@ bool brick_args_ok(const bm_params* p, const bm_state* s) {
     return false;
   return lds_bytes(p, 7, 7) <= 65536 && lds_bytes(p, 3, 3) <= 65536;
 }
+// Workgroups of a running F update: kUpdateThreads threads, a multiple of 8 workgroups.
+static int update_blocks(int threads) { return (kUpdateThreads / threads) & ~7; }
 // Threads of a brick workgroup: one per particle slot (8 per cell) and at least one per window
 // node, in whole waves (at most 1024 by brick_args_ok and the LDS bound on the window).
-int brick_threads(const bm_params* p) {
+static int brick_threads(const bm_params* p) {
   int64_t slots = 8 * p->brick[0] * p->brick[1] * p->brick[2];
   int64_t t = slots > window_nodes(p) ? slots : window_nodes(p);
   t = ((t + 63) / 64) * 64;
   return int(t > 1024 ? 1024 : t);
 }
-void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
+static void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
   int64_t total = node_launch(p);
   hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
   if (p->n > 0) hipLaunchKernelGGL(k_topo_insert, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
@@ -2421,7 +2424,7 @@ void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
 // pair_dirty (2) once every reader of this step is done; g2p_flags: assign (1) and the
 // pair-list staleness check (2) on the moved particles; defer: leave the F update to the
 // next P2G where the field slots allow it.
-void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
+static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
                       hipStream_t st, int defer = 0) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
   const Divisors D = divisors_of(p);
@@ -2433,7 +2436,7 @@ void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int6
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
   // A deferred launch does little (the early updates after relabelling, or all of them after a
   // full registration): a small grid keeps its cost low.
-  int ub = defer ? kDeferredUpdateBlocks : kUpdateBlocks;
+  int ub = defer ? kDeferredUpdateBlocks : update_blocks(tb);
   hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer, nb);
 }
 }  // namespace
@@ -2492,8 +2495,8 @@ int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  int nb = int(brick_launch(p));
-  hipLaunchKernelGGL(k_update_F, dim3(nb < kUpdateBlocks ? nb : kUpdateBlocks), dim3(brick_threads(p)), 0,
+  int nb = int(brick_launch(p)), ub = update_blocks(brick_threads(p));
+  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(brick_threads(p)), 0,
                      (hipStream_t)stream, *p, *s, divisors_of(p), 0, nb);
   return launch_status();
 }
@@ -2536,7 +2539,7 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
 }
 
 namespace {
-int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit, int32_t parity,
+static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit, int32_t parity,
             int32_t accumulate, const float* targets, const float* rotations, int32_t index, void* stream, int defer) {
   if (!brick_args_ok(p, s) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
   if (p->rigid_blocks < blocks_for(p->n)) return -2;

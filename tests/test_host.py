@@ -261,7 +261,8 @@ def test_brick_bins_cover_every_particle_window(shape, size, spacing, fields):
     used = slots[slots >= 0]
     np.testing.assert_array_equal(np.sort(used), np.arange(len(ref)))
     assert np.all(bricks * brick >= cells)
-    assert 9 * fields * np.prod(brick + 1) * 4 <= 65536
+    window, corners = fields * np.prod(brick + 1), 8 * np.prod(brick)
+    assert max(7 * window + 7 * corners, 3 * window + 3 * corners) * 4 <= 65536
     per_brick = int(np.prod(brick))
     assert len(slots) == 8 * int(np.prod(bricks)) * per_brick
     cell_slots = slots.reshape(-1, 8)
