@@ -2436,7 +2436,7 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
   // A deferred launch does little (the early updates after relabelling, or all of them after a
   // full registration): a small grid keeps its cost low.
-  int ub = defer ? kDeferredUpdateBlocks : update_blocks(tb);
+  int ub = defer && s->update_pending && s->update_done ? kDeferredUpdateBlocks : update_blocks(tb);
   hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer, nb);
 }
 }  // namespace
