@@ -909,8 +909,10 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
           cc[j] = Cell{home.x + dx, home.y + dy, home.z + dz};
           cur[j] = (k < nyz && cull.near(dx, dy, dz)) ? bucket_head(S, base + bucket_of(cc[j], mask), tag) : -1;
         }
+        // A chain holds each particle at most once, so every walk ends within n steps; the
+        // bound keeps a damaged table from looping.
         bool more = true;
-        while (more) {
+        for (int64_t step = 0; more && step < P.n; ++step) {
           more = false;
 #pragma unroll
           for (int j = 0; j < kChains; ++j) {
@@ -959,7 +961,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
       for (int dz = -rings; dz <= rings; ++dz) {
         if (!cull.near(dx, dy, dz)) continue;
         Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-        for (int32_t q = bucket_head(S, base + bucket_of(c, mask), tag); q >= 0; q = S.list_links[q]) {
+        int64_t step = 0;
+        for (int32_t q = bucket_head(S, base + bucket_of(c, mask), tag); q >= 0 && step < P.n;
+             q = S.list_links[q], ++step) {
           if (q <= p) continue;
           Cell cq = cell_of(ld3(bx, q), inv);
           if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
@@ -994,7 +998,9 @@ __device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int3
     int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
     if (!cull.near(dx, dy, dz)) continue;
     Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
-    for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), tag); q >= 0; q = S.list_links[q]) {
+    int64_t step = 0;  // a chain holds each particle at most once
+    for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), tag); q >= 0 && step < P.n;
+         q = S.list_links[q], ++step) {
       f3 dd = ld3(S.built_x, q) - xd;
       if (dot(dd, dd) < reach2) S.pair_mark[q] = mark;
     }
@@ -1336,12 +1342,14 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   }
 }
 // Bricks along one axis whose window holds node coordinate c: c / B, and c / B - 1 when c
-// is on a brick boundary.
+// is on a brick boundary.  The stores use fixed indices (out[0] is c / B when that is a
+// brick, else c / B - 1) so that the caller's arrays stay in registers.
 __device__ __forceinline__ int bricks_at(int c, const Div32& B, int nb, int* out) {
-  int k = 0, b0 = quot(B, c);
-  if (b0 < nb) out[k++] = b0;
-  if (c == b0 * int(B.d) && b0 > 0) out[k++] = b0 - 1;
-  return k;
+  int b0 = quot(B, c);
+  bool inside = b0 < nb, below = c == b0 * int(B.d) && b0 > 0;
+  out[0] = inside ? b0 : b0 - 1;
+  out[1] = b0 - 1;
+  return int(inside) + int(below);
 }
 
 // Sum of the window entries of node (nx, ny, nz), field f, components [0, C) over its (at
