@@ -59,6 +59,10 @@ class Solver:
     """brick_z: height in cells of the particle bricks of the transfer kernels (a tuning knob;
     results do not depend on it beyond float summation order)."""
 
+    # Relabelled particles listed per substep for the incremental re-registration; more in one
+    # substep re-register every particle instead.
+    CHANGED_CAPACITY = 65536
+
     def __init__(self, config=Config(), device="cuda:0", brick_z=None, _layout=None):
         self.config = cfg = config
         self._brick_z = brick_z
@@ -191,7 +195,7 @@ class Solver:
         # reads as an empty bucket, so the lists need no clearing pass.
         T["list_heads"] = t((self.list_buckets,), i64)
         T["list_links"] = t((max(n, 1),), i32, -1)
-        self.changed_capacity = 65536
+        self.changed_capacity = int(self.CHANGED_CAPACITY)
         T["changed"] = t((self.changed_capacity,), i32)
         T["changed_count"] = t((1,), i32)
         T["pair_mark"] = t((max(n, 1),), i32)

@@ -1232,7 +1232,9 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // order, adds the window, and the brick's partial goes out with plain stores.  When the last
 // substep's F update is pending (update_pending), each lane first updates its particle's F and
 // stress (update_F_plastic, material.py:445-475) and uses that stress.
-__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int64_t p, f3 lp, uint2 fidx);
+template <bool kRemap>
+__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
+                                     uint2 fidx);
 
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
@@ -1285,7 +1287,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       // substep; their marks are cleared here.
       const bool done = pending == 2 && S.update_done[p] != 0;
       if (pending && !done) {
-        Pk = update_particle(P, S, p, lp, fidx);
+        Pk = update_particle<false>(P, S, D, p, lp, fidx);
       } else {
         Pk = ld9(S.stress, p);
         if (done) S.update_done[p] = 0;
@@ -1625,9 +1627,20 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     out[int64_t(i) * 3 + 2] = sum[2];
   }
 }
+template <bool kRemap>
+__device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks);
+
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
-__global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D) {
+// With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
+// update to the next P2G (deferred_update), so no separate F-update launch follows.
+__global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
+                                                            int update_blocks) {
+  const int node_blocks = int(gridDim.x) - update_blocks;
+  if (int(blockIdx.x) >= node_blocks) {
+    deferred_update<true>(P, S, D, int(blockIdx.x) - node_blocks, update_blocks);
+    return;
+  }
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
@@ -1715,6 +1728,36 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
   }
 }
 
+// The velocity k_couple_remapped stores for field a of a node, computed here from the
+// re-scattered partials with the same operations in the same order, for an F update that
+// runs beside that kernel.
+__device__ f3 remapped_velocity(const bm_params& P, const bm_state& S, const Divisors& D, int64_t node, int a) {
+  const int64_t N = P.node_count;
+  int nx, ny, nz;
+  node_coords(P, D, node, nx, ny, nz);
+  const int nf = S.node_nf[node];
+  float g[3];
+  gather_node<3>(P, D, S.remap_partial, nx, ny, nz, a, g);
+  const int64_t sa = int64_t(a) * N + node;
+  const float ma = S.grid_mass[sa];
+  const f3 xa = f3{g[0], g[1], g[2]};
+  if (nf <= 1) return ma > 0.0f ? xa / ma : f3{0.f, 0.f, 0.f};
+  if (ma <= 0.0f) return f3{0.f, 0.f, 0.f};
+  float total = 0.0f;
+  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  const int64_t ca = S.field_code[sa];
+  f3 ex = f3{0.f, 0.f, 0.f};
+  for (int b = 0; b < nf; ++b) {
+    if (b == a) continue;
+    const int64_t sb = int64_t(b) * N + node;
+    if (!compatible(ca, S.field_code[sb])) continue;
+    float gb[3];
+    gather_node<3>(P, D, S.remap_partial, nx, ny, nz, b, gb);
+    ex = ex + (ma * f3{gb[0], gb[1], gb[2]} - S.grid_mass[sb] * xa);
+  }
+  return (xa + ex / total) / ma;
+}
+
 // Lame constants of particle p: its instance's for batched instances, else the solver's.
 __device__ __forceinline__ void lame_of(const bm_params& P, const bm_state& S, int64_t p, float& mu, float& lam) {
   if (S.inst_mu != nullptr) {
@@ -1792,9 +1835,12 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 
 // update_F_plastic (material.py:445-475) for one particle: F += dt Σ_a v_a ⊗ ∇w_a over its
 // 8 corners, then the stress.  The corner velocities of its field come straight from
-// grid_vel (a node array small enough to stay in L2).  Stores F (and Fp after plastic flow)
-// and the stress, and returns the stress.
-__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int64_t p, f3 lp, uint2 fidx) {
+// grid_vel (a node array small enough to stay in L2), or with kRemap from the re-scattered
+// partials (remapped_velocity), for an update that runs beside k_couple_remapped.  Stores F
+// (and Fp after plastic flow) and the stress, and returns the stress.
+template <bool kRemap>
+__device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
+                                     uint2 fidx) {
   // While no plastic flow has happened and the host has not written Fp, every Fp is I.
   bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
   int32_t base = S.base_node[p];
@@ -1808,18 +1854,32 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int6
   int64_t N = P.node_count;
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
   float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
-  f3 vel[8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
   m3 rate = mzero();
+  if constexpr (kRemap) {
+    // One corner at a time: each velocity is a gather over bricks and fields.
+#pragma unroll 1
+    for (int a = 0; a < 8; ++a) {
+      f3 v = remapped_velocity(P, S, D, base + corner_offset(a, dyz, dz), field_of(fidx, a));
+      float wt;
+      f3 g;
+      stencil_corner(lp, inv_h, a, wt, g);
+      rate.a[0] += v.x * g.x; rate.a[1] += v.x * g.y; rate.a[2] += v.x * g.z;
+      rate.a[3] += v.y * g.x; rate.a[4] += v.y * g.y; rate.a[5] += v.y * g.z;
+      rate.a[6] += v.z * g.x; rate.a[7] += v.z * g.y; rate.a[8] += v.z * g.z;
+    }
+  } else {
+    f3 vel[8];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    float wt;
-    f3 g;
-    stencil_corner(lp, inv_h, a, wt, g);
-    rate.a[0] += vel[a].x * g.x; rate.a[1] += vel[a].x * g.y; rate.a[2] += vel[a].x * g.z;
-    rate.a[3] += vel[a].y * g.x; rate.a[4] += vel[a].y * g.y; rate.a[5] += vel[a].y * g.z;
-    rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
+    for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      float wt;
+      f3 g;
+      stencil_corner(lp, inv_h, a, wt, g);
+      rate.a[0] += vel[a].x * g.x; rate.a[1] += vel[a].x * g.y; rate.a[2] += vel[a].x * g.z;
+      rate.a[3] += vel[a].y * g.x; rate.a[4] += vel[a].y * g.y; rate.a[5] += vel[a].y * g.z;
+      rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
@@ -1829,41 +1889,54 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, int6
   return stress;
 }
 
-// The F update of a substep, one lane per particle slot of a brick.  With `defer` it is left to
-// the next P2G, which updates each particle and uses its stress at once (update_pending tells
-// that P2G), for every particle whose field slots stay as they are: all of them without a
-// label change; after relabelling only, all but the particles of the 27 cells around each
-// relabelled one, which register again before that P2G and are updated here (once each,
-// marked in update_done).  A full re-registration updates every particle here.
-// The workgroups stride over the bricks (at most kUpdateThreads threads in all, in a multiple
-// of 8 workgroups, so a workgroup keeps the bricks of its XCD): a deferred launch then costs
-// little.
+// The deferred F update of a substep, run by `blocks` workgroups (this is workgroup `block`
+// of them): with kRemap extra workgroups of k_couple_remapped, else a k_update_F launch after
+// it.  The update is left to the next P2G, which updates
+// each particle and uses its stress at once (update_pending tells that P2G), for every
+// particle whose field slots stay as they are: all of them without a label change; after
+// relabelling only, all but the particles of the 27 cells around each relabelled one, which
+// register again before that P2G and are updated here (once each, marked in update_done).
+// A full re-registration (rare: more relabels than the changed list holds) updates every
+// particle here.  Running beside the node kernel (kRemap), these updates read their corner
+// velocities through remapped_velocity.
+template <bool kRemap>
+__device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks) {
+  const int topo = S.topology_dirty[0];
+  const int64_t first = int64_t(block) * blockDim.x + threadIdx.x, stride = int64_t(blocks) * blockDim.x;
+  const uint2* fidx = reinterpret_cast<const uint2*>(S.field_idx);
+  if (topo & kTopoFull) {
+    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap>(P, S, D, p, ld3(S.local, p), fidx[p]);
+    return;
+  }
+  // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
+  if (first == 0) S.update_pending[0] = (topo & kTopoChanged) ? 2 : 1;
+  if ((topo & kTopoChanged) == 0) return;
+  int64_t count = min(S.changed_count[0], int(P.changed_capacity));
+  int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
+  const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
+  const int64_t cpb = int64_t(BX) * BY * BZ;
+  for (int64_t a = first; a < count * 27 * 8; a += stride) {
+    int64_t base = S.base_node[S.changed[a / (27 * 8)]];
+    int k = int((a >> 3) % 27);
+    int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
+      continue;
+    int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
+    int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
+    if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
+      update_particle<kRemap>(P, S, D, q, ld3(S.local, q), fidx[q]);
+  }
+}
+
+// The F update of a substep, one lane per particle slot of a brick; with `defer`, the
+// deferred update above.  The workgroups stride over the bricks (at most kUpdateThreads
+// threads in all, in a multiple of 8 workgroups, so a workgroup keeps the bricks of its XCD).
 constexpr int kUpdateThreads = 1 << 20;
 constexpr int kDeferredUpdateBlocks = 128;
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
-  const int topo = S.topology_dirty[0];
-  if (defer && S.update_pending != nullptr && S.update_done != nullptr && (topo & kTopoFull) == 0) {
-    // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
-    if (blockIdx.x == 0 && threadIdx.x == 0) S.update_pending[0] = (topo & kTopoChanged) ? 2 : 1;
-    if (topo & kTopoChanged) {
-      int64_t count = min(S.changed_count[0], int(P.changed_capacity));
-      int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
-      const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
-      const int64_t cpb = int64_t(BX) * BY * BZ;
-      int64_t stride = int64_t(gridDim.x) * blockDim.x;
-      for (int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; a < count * 27 * 8; a += stride) {
-        int64_t base = S.base_node[S.changed[a / (27 * 8)]];
-        int k = int((a >> 3) % 27);
-        int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
-          continue;
-        int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
-        int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
-        if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
-          update_particle(P, S, q, ld3(S.local, q), reinterpret_cast<const uint2*>(S.field_idx)[q]);
-      }
-    }
+  if (defer) {
+    deferred_update<false>(P, S, D, int(blockIdx.x), int(gridDim.x));
     return;
   }
   int CS = 8 * cells_per_brick(P);
@@ -1872,7 +1945,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
   for (int k = int(blockIdx.x); k < nbricks; k += int(gridDim.x)) {
     int b = active_brick(S, brick_of_block(k, nbricks));
     int64_t p = S.perm[int64_t(b) * CS + slot];
-    if (p >= 0) update_particle(P, S, p, ld3(S.local, p), reinterpret_cast<const uint2*>(S.field_idx)[p]);
+    if (p >= 0) update_particle<false>(P, S, D, p, ld3(S.local, p), reinterpret_cast<const uint2*>(S.field_idx)[p]);
   }
 }
 
@@ -2441,11 +2514,20 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
-  // A deferred launch does little (the early updates after relabelling, or all of them after a
-  // full registration): a small grid keeps its cost low.
-  int ub = defer && s->update_pending && s->update_done ? kDeferredUpdateBlocks : update_blocks(tb);
-  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer, nb);
+  const int node_blocks = int(blocks_for(node_launch(p)));
+  defer = defer && s->update_pending && s->update_done;
+  if (defer && s->instance == nullptr) {
+    // The F update is deferred to the next P2G: a few extra workgroups of the node kernel set
+    // that up (and make the early updates), and no F-update launch follows.
+    hipLaunchKernelGGL(k_couple_remapped, dim3(node_blocks + kDeferredUpdateBlocks), dim3(kBlock), 0, st, *p, *s, D,
+                       kDeferredUpdateBlocks);
+    return;
+  }
+  // Batched instances relabel in many instances each substep: their early updates read
+  // grid_vel after the node kernel, in a small deferred launch.
+  hipLaunchKernelGGL(k_couple_remapped, dim3(node_blocks), dim3(kBlock), 0, st, *p, *s, D, 0);
+  int ub = defer ? kDeferredUpdateBlocks : update_blocks(tb);
+  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer ? 1 : 0, nb);
 }
 }  // namespace
 
@@ -2497,7 +2579,7 @@ int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
-                     *s, divisors_of(p));
+                     *s, divisors_of(p), 0);
   return launch_status();
 }
 

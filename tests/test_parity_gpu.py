@@ -299,3 +299,36 @@ def test_deferred_update_matches_eager_across_relabels():
     for name in ("x", "v", "F"):
         assert rel(getattr(graph, name).numpy().ravel(), getattr(eager, name).numpy().ravel()) < 1e-6, name
     assert int(graph._t["update_pending"].item()) == 0 and not graph._t["update_done"].any()
+
+
+def test_deferred_update_matches_eager_after_overflowing_relabels():
+    """With a changed list of two entries the relabels of a substep overflow it, so the next
+    substep re-registers every particle: inside a recorded interval the deferred update then
+    updates every particle beside the node kernel, and the run still equals substep-by-substep
+    stepping."""
+
+    class Tiny(Solver):
+        CHANGED_CAPACITY = 2
+
+    cfg = small_cfg(yield_stress=2000.0, cutting_energy=0.05)
+    eager, graph, probe = Tiny(cfg), Tiny(cfg), Tiny(cfg)
+    for s in (eager, graph, probe):
+        blade_touching(s, 0.002)
+    edge = cfg.size[2] - 0.002
+    commands = np.column_stack([np.zeros(16), np.zeros(16), np.linspace(edge, edge - 0.0005, 16)])
+    # The same substeps one at a time: the relabel count of each (all of them are counted,
+    # also those beyond the list) shows that some substep overflows the list.
+    counts = []
+    for row in commands:
+        probe.advance(row[None, :], True, 0, graph=False)
+        counts.append(int(probe._t["changed_count"].item()))
+    assert max(counts) > Tiny.CHANGED_CAPACITY, counts
+    before = eager.codes.numpy().copy()
+    eager.advance(commands, True, 0, graph=False)
+    graph.advance(commands, True, 0, graph=True)
+    after = eager.codes.numpy()
+    assert np.count_nonzero(after != before) > 0
+    np.testing.assert_array_equal(graph.codes.numpy(), after)
+    for name in ("x", "v", "F"):
+        assert rel(getattr(graph, name).numpy().ravel(), getattr(eager, name).numpy().ravel()) < 1e-6, name
+    assert int(graph._t["update_pending"].item()) == 0 and not graph._t["update_done"].any()
