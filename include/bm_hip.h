@@ -212,11 +212,13 @@ int bm_substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t 
                int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
                int32_t index, void* stream);
 
-/* As bm_substep (the same Solver._step_kernels, solver.py:920-940), but when no labels
- * changed in this substep its F update (update_F_plastic, material.py:445-475) is left to the
- * P2G of the next bm_substep or bm_substep_deferred on the same stream, which updates each
- * particle and uses its new stress at once.  Nothing may read F, Fp or the stress in between;
- * a sequence ends with bm_substep, which completes its own update. */
+/* As bm_substep (the same Solver._step_kernels, solver.py:920-940), but its F update
+ * (update_F_plastic, material.py:445-475) is left to the P2G of the next bm_substep or
+ * bm_substep_deferred on the same stream, which updates each particle and uses its new stress
+ * at once.  Particles that register again before that P2G (those around relabelled particles;
+ * all of them after a full re-registration) are updated within this call.  Nothing may read
+ * F, Fp or the stress in between; a sequence ends with bm_substep, which completes its own
+ * update. */
 int bm_substep_deferred(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit,
                         int32_t parity, int32_t accumulate, const float* targets, const float* rotations,
                         int32_t index, void* stream);
