@@ -24,20 +24,50 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "blademaster-lab_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per particle-substep of the reference substep (x, v, F, Fp, stress,
-# codes, friction histories, stencil slots/weights/gradients), attributed to the kernel of
-# this build that first reads / last writes each array; the four shares sum to 444 B.
-# Algorithmic bytes per particle of each kernel: the per-substep figure of 444 B split by
-# charging every array read to the first kernel of the substep that reads it and every array
-# write to the last kernel that writes it (rigid: x, v read + friction histories read and
-# written; pairs: codes; p2g: stress + the stencil slots, weights and gradients of the
-# reference layout; g2p: x, v written; update_F: F read and written, Fp read, stress written).
-ALG_BYTES = {"k_rigid": 72, "k_pairs": 8, "k_p2g": 196, "k_g2p": 24, "k_update_F": 144}
-# Inside a recorded interval the F update of a substep runs in the next substep's P2G (one
-# launch, both shares: F, Fp, stress and the stencil).
-ALG_BYTES["k_p2g_update"] = ALG_BYTES["k_p2g"] + ALG_BYTES["k_update_F"]
+# Algorithmic bytes per particle of each kernel: the per-particle arrays of the substep state
+# (x, v, F, Fp, stress, codes, friction histories, the stencil) that the kernel must read or
+# write, each array counted once per kernel.  The stencil is this design's base node (int32)
+# and three local cell fractions (16 B): the kernels rebuild weights and gradients from it and
+# never read the reference's 160 B of stored slots, weights and gradients.  Fp is not read
+# while every Fp is the identity (no plastic flow; a device flag), which holds for this
+# workload.  Intermediates between kernels (contact impulses, grid partials) are not charged.
+#   k_rigid       x, v read 24; friction histories read 24 + written 24            = 72
+#   k_pairs       codes read 8                                                      = 8
+#   k_p2g         v 12, stress 36 and the stencil 16 read                           = 64
+#   k_g2p         x, v read 24 + written 24; stencil 16; codes read 8               = 72
+#   k_update_F    F read 36 + written 36; stress written 36; stencil 16             = 124
+#   k_p2g_update  k_p2g with the previous substep's F update inside it: v 12, stencil 16,
+#                 F read + written 72, stress written 36 (not read back)            = 136
+ALG_BYTES = {"k_rigid": 72, "k_pairs": 8, "k_p2g": 64, "k_g2p": 72, "k_update_F": 124, "k_p2g_update": 136}
+# Per particle-substep in the reference's layout (x, v, F, Fp, stress, codes, histories and the
+# stored stencil): the substep-level figure, hbm_peak_frac.
 SUBSTEP_BYTES = 444
 INTERVAL_SUBSTEPS = 40
+
+
+def interval_schedule(steps, warmup, per=INTERVAL_SUBSTEPS):
+    """Interval lengths, in substeps, of the untimed warm-up and of the timed region.
+
+    The timed region is exactly `steps` substeps: intervals of `per` substeps and one shorter
+    interval for the remainder.  Each interval length is one captured HIP graph, cached under
+    (length, step parity), so the warm-up runs every timed length at the parity it is timed
+    at, and runs at least `warmup` substeps: the remainder first (twice when it is odd, which
+    restores the parity), then intervals of `per`.  No capture falls in the timed region.
+    """
+    if steps < 1:
+        raise ValueError("steps must be at least 1")
+    if warmup < 0:
+        raise ValueError("warmup must be non-negative")
+    full, rest = divmod(steps, per)
+    timed = [per] * full + ([rest] if rest else [])
+    step = 2 if rest % 2 else 1  # remainder intervals in pairs when odd
+    warm = [rest] * step if rest else []
+    if full:
+        warm += [per] * max(1, -(-(warmup - sum(warm)) // per))
+    else:
+        while sum(warm) < warmup:
+            warm += [rest] * step
+    return warm, timed
 
 
 def workload(seed=0, n_side=(0.3, 0.15, 0.075)):
@@ -81,35 +111,31 @@ def run_gpu(args):
     cfg, q, edge_z0, hz, E = workload()
     s = Solver(cfg, brick_z=args.brick_z)
     port = ExternalToolPort(s)
-    interval = INTERVAL_SUBSTEPS * cfg.dt
     vel = np.array([0.0, 0.0, -0.1])
+    warm, timed = interval_schedule(args.steps, args.warmup)
+    clock = [0]  # substeps submitted so far: the start time of the next interval
 
-    def submit(k):
-        t = k * interval
-        state = ToolState(pose_at(t, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0])
-        port.submit(state, interval, cutting=True, cut_index=0)
+    def submit(length):
+        state = ToolState(pose_at(clock[0] * cfg.dt, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0])
+        port.submit(state, length * cfg.dt, cutting=True, cut_index=0)
+        clock[0] += length
         return state
 
-    total = args.steps + args.warmup
-    intervals = -(-total // INTERVAL_SUBSTEPS)
-    warm_intervals = -(-args.warmup // INTERVAL_SUBSTEPS)
-    timed_intervals = intervals - warm_intervals
-    for k in range(warm_intervals):
-        submit(k)
+    for length in warm:
+        submit(length)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
     start.record(stream)
-    last = None
-    for k in range(warm_intervals, intervals):
-        last = submit(k)
+    for length in timed:
+        last = submit(length)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     gpu_ms = start.elapsed_time(stop)
-    steps_timed = timed_intervals * INTERVAL_SUBSTEPS
-    reaction = port.collect(last, interval, 2)
+    steps_timed = sum(timed)
+    reaction = port.collect(last, timed[-1] * cfg.dt, 2)
     st = s.state()
     ms_per_step = gpu_ms / steps_timed
     rate = s.n * steps_timed / (gpu_ms * 1e-3)
@@ -152,10 +178,16 @@ def run_gpu(args):
     dominant = max(per_kernel_ms, key=per_kernel_ms.get)
     t_dom = per_kernel_ms[dominant] * 1e-3
     achieved = ALG_BYTES[dominant] * s.n / t_dom / 1e9
-    traffic = None
+    # HBM bytes per launch are hardware counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): they
+    # cannot be read inside this process, so they come from the committed counter profile of the
+    # same workload, named in traffic_source.
+    traffic = traffic_source = None
     if args.traffic_file and os.path.exists(args.traffic_file):
-        entry = json.load(open(args.traffic_file)).get("kernels", {}).get(dominant)
-        traffic = entry["total"] if entry else None
+        profile = json.load(open(args.traffic_file))
+        entry = profile.get("kernels", {}).get(dominant)
+        if entry:
+            traffic = entry["total"]
+            traffic_source = os.path.relpath(args.traffic_file, ROOT)
     return dict(
         solver=s,
         cfg=cfg,
@@ -165,10 +197,12 @@ def run_gpu(args):
         wall=wall,
         gpu_ms=gpu_ms,
         steps=steps_timed,
+        warmup=sum(warm),
         per_kernel_ms=per_kernel_ms,
         dominant=dominant,
         achieved=achieved,
         traffic=traffic,
+        traffic_source=traffic_source,
         reaction=reaction,
         codes_nonzero=int(np.count_nonzero(st["sides"])),
     )
@@ -205,24 +239,29 @@ def run_batch(args):
     brick_z, timings = autotune_brick_z(configs[:16], substeps=20) if args.brick_z is None else (args.brick_z, {})
     s = BatchSolver(configs, brick_z=brick_z, gap=gap)
     port = BatchPort(s)
-    interval = INTERVAL_SUBSTEPS * configs[0].dt
-    total = args.steps + args.warmup
-    intervals = -(-total // INTERVAL_SUBSTEPS)
-    warm = -(-args.warmup // INTERVAL_SUBSTEPS)
-    for k in range(warm):
-        port.submit([kn(k * interval) for kn in knives], interval)
+    dt = configs[0].dt
+    warm, timed = interval_schedule(args.steps, args.warmup)
+    clock = [0]
+
+    def submit(length):
+        states = [kn(clock[0] * dt) for kn in knives]
+        port.submit(states, length * dt)
+        clock[0] += length
+        return states
+
+    for length in warm:
+        submit(length)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(stream)
-    last = None
-    for k in range(warm, intervals):
-        last = port.submit([kn(k * interval) for kn in knives], interval)
+    for length in timed:
+        last = submit(length)
     b.record(stream)
     torch.cuda.synchronize()
-    steps = (intervals - warm) * INTERVAL_SUBSTEPS
+    steps = sum(timed)
     gpu_ms = a.elapsed_time(b)
-    reactions = port.collect(last[0], interval)
+    reactions = port.collect(last, timed[-1] * dt)
     rate = s.n * steps / (gpu_ms * 1e-3)
     cut = int(np.count_nonzero(s.codes.numpy()))
     return {
@@ -231,7 +270,7 @@ def run_batch(args):
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": sum(warm),
         "ms_per_step": gpu_ms / steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -254,11 +293,12 @@ def run_batch(args):
     }
 
 
-# MLS-MPM baseline (mlsmpm.py): x, v 24 r + 24 w; C, F 72 r + 72 w; damage and its
-# reference 8 r; friction histories 24 r + 24 w = 248 B per particle-substep, charged as above:
-# contact reads x, v and reads/writes the histories; P2G reads C, F and damage; G2P writes
-# x, v, C, F and reads the damage reference.
-MLS_ALG_BYTES = {"k_rigid": 72, "k_mls_p2g": 76, "k_mls_g2p": 100}
+# MLS-MPM baseline (mlsmpm.py), per kernel the per-particle arrays it must read or write, as
+# above: contact reads x, v and reads/writes the friction histories (72); P2G reads x, v, C, F
+# and the damage flag (100); G2P reads x, F and the damage reference, and writes x, v, C and F
+# (148).  Per particle-substep: x, v 24 r + 24 w; C, F 72 r + 72 w; damage and its reference
+# 8 r; friction histories 24 r + 24 w = 248 B.
+MLS_ALG_BYTES = {"k_rigid": 72, "k_mls_p2g": 100, "k_mls_g2p": 148}
 MLS_SUBSTEP_BYTES = 248
 # Node updates per particle of the MLS P2G: 27 nodes x (mass + 3 momentum components), summed
 # per workgroup in LDS before one global atomic per node and component.
@@ -275,31 +315,30 @@ def run_mls(args):
 
     cfg, q, edge_z0, hz, E = workload()
     s = MlsSolver(cfg)
-    interval = INTERVAL_SUBSTEPS * cfg.dt
     vel = np.array([0.0, 0.0, -0.1])
+    warm, timed = interval_schedule(args.steps, args.warmup)
+    clock = [0]
 
-    def advance(k, check=True):
-        t = k * interval
-        return s.advance(ToolState(pose_at(t, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0]), interval, check=check)
+    def advance(length, check=True):
+        state = ToolState(pose_at(clock[0] * cfg.dt, q, edge_z0, hz), q, vel, [0.0, 0.0, 0.0])
+        clock[0] += length
+        return s.advance(state, length * cfg.dt, check=check)
 
-    total = args.steps + args.warmup
-    intervals = -(-total // INTERVAL_SUBSTEPS)
-    warm = -(-args.warmup // INTERVAL_SUBSTEPS)
-    for k in range(warm):
-        advance(k)
+    for length in warm:
+        advance(length)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     gpu_ms = 0.0
-    for k in range(warm, intervals):
+    for length in timed:
         # Each interval ends with the reaction's readback: time the device work of each
         # interval and add them (the domain check runs once, after the timed intervals).
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        advance(k, check=False)
+        advance(length, check=False)
         b.record(stream)
         torch.cuda.synchronize()
         gpu_ms += a.elapsed_time(b)
-    steps = (intervals - warm) * INTERVAL_SUBSTEPS
+    steps = sum(timed)
     rate = s.n * steps / (gpu_ms * 1e-3)
     lost = s.diagnostics()["left_domain"]
     mp, ms = ctypes.byref(s._mls_params), ctypes.byref(s._mls_state)
@@ -345,7 +384,7 @@ def run_mls(args):
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": sum(warm),
         "ms_per_step": gpu_ms / steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -375,7 +414,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--cpu-substeps", type=int, default=8)
+    ap.add_argument("--cpu-substeps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_1M_block.json"))
     ap.add_argument("--brick-z", type=int, default=None, help="brick height of the transfer kernels (cells)")
@@ -427,7 +466,7 @@ def main():
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": r["steps"],
-        "warmup": args.warmup,
+        "warmup": r["warmup"],
         "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -454,6 +493,7 @@ def main():
             "unit": "GB/s",
             "frac": r["achieved"] / peak,
             "traffic": r["traffic"],
+            "traffic_source": r["traffic_source"],
             "alg_bytes_per_particle": ALG_BYTES[r["dominant"]],
         },
         "kernel_ms": {k: round(v, 5) for k, v in r["per_kernel_ms"].items()},

@@ -21,7 +21,7 @@ import torch
 
 from . import _lib
 from .config import Config, derived
-from .coupling import Reaction, ToolState
+from .coupling import Reaction, ToolState, register_port
 from .geometry import grid_cells, points
 from .solver import Solver
 
@@ -164,6 +164,7 @@ class BatchPort:
         self._staging = [torch.zeros(I, 49, dtype=torch.float32, pin_memory=True) for _ in range(4 * self.slots)]
         self._staged = [None] * len(self._staging)
         self._stage_next = 0
+        register_port(self, "batch")
 
     def _ptr(self, tensor):
         return ctypes.c_void_p(tensor.data_ptr())
@@ -219,7 +220,7 @@ class BatchPort:
 
         unused_targets = np.zeros((steps, 3), dtype=np.float32)
         _, _, graph = s.prepare(unused_targets, cutting, cut_index, before=[load], after=[finish],
-                                tag=("batch", id(self), slot))
+                                tag=self._owner + (slot,))
         graph.replay()
         s.step_count += steps
         return steps

@@ -10,9 +10,29 @@ tool.py:42-68, blade_constraint.py:208-210).  One interval is one HIP graph repl
 
 from dataclasses import dataclass
 import ctypes
+import itertools
 import time
+import weakref
 import numpy as np
 import torch
+
+# Ports are told apart in the solver's graph cache by a serial number that is never reused
+# (an id() can be reused by a later port once this one is collected).
+_port_serials = itertools.count()
+
+
+def _release_port_graphs(solver_ref, owner):
+    solver = solver_ref()
+    if solver is not None:
+        solver.release_graphs(owner)
+
+
+def register_port(port, kind):
+    """Give `port` its graph-cache tag prefix and drop its graphs from the solver's cache
+    when the port is collected."""
+    port._owner = (kind, next(_port_serials))
+    weakref.finalize(port, _release_port_graphs, weakref.ref(port.solver), port._owner)
+    return port._owner
 
 
 def vector(value, length, name):
@@ -94,6 +114,7 @@ class ExternalToolPort:
         self.trajectory_times = None
         self._results = {}
         self.timing_s = dict(submission=0.0, completion_and_readback=0.0)
+        register_port(self, "external")
 
     def _ptr(self, tensor):
         return ctypes.c_void_p(tensor.data_ptr())
@@ -166,7 +187,7 @@ class ExternalToolPort:
         if graph:
             unused_targets = np.zeros((steps, 3), dtype=np.float32)
             _, _, captured = s.prepare(
-                unused_targets, cutting, cut_index, before=before, after=[finish], tag=("external", id(self)),
+                unused_targets, cutting, cut_index, before=before, after=[finish], tag=self._owner,
                 after_step=sample,
             )
             captured.replay()

@@ -538,6 +538,13 @@ class Solver:
             self.graph_build_s += time.perf_counter() - start
         return self._batches[key]
 
+    def release_graphs(self, owner):
+        """Drop the cached graphs recorded for `owner` (a port's tag prefix, e.g.
+        ("external", serial)): their launches hold raw pointers into that port's buffers."""
+        owner = tuple(owner)
+        for key in [k for k in self._batches if isinstance(k[-1], tuple) and k[-1][: len(owner)] == owner]:
+            del self._batches[key]
+
     # ------------------------------------------------------------------ readback
     def state(self):
         """Host copies of the particle and blade state; raises on device errors (solver.py:1021-1051)."""

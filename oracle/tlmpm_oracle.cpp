@@ -1496,6 +1496,11 @@ int oracle_stage(void* handle, const char* stage, int arg0, int64_t arg1) {
     if (st == "pairs") { s->particle_contacts(); return 0; }
     if (st == "material") { s->material_step(); return 0; }
     if (st == "register") { s->register_topology(); return 0; }
+    if (st == "p2g") {
+      if (!s->c.cache_topology || s->topology_dirty) s->register_topology();
+      s->p2g();
+      return 0;
+    }
     if (st == "assign") { s->assign(arg1, arg0); return 0; }
     if (st == "finish_external") { s->finish_external(); return 0; }
     if (st == "refresh_stress") { for (int64_t p = 0; p < s->n; ++p) s->stress_at(p); return 0; }

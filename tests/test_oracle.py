@@ -23,7 +23,14 @@ wrong sign or index fails:
   tool.step        free-fall closed form (tests/test_multi_tool.py:68-79)
   predict mode     x = 0.01 t, q_z = sin(0.1 t) (tests/test_coupling.py:322-339)
   compatible       tests/test_topology.py:10-12
+  p2g              dense tent-function sums over every (particle, node) pair; total mass,
+                   momentum and impulse; first moment of the impulse = sum j X^T - dt V sum P
+  update_F         an affine velocity field v = A X + b gives F = I + dt A, v and x exact
+                   (a swapped outer product, a missing dt or a transposed stress fails)
   MLS transfers    APIC carries an affine velocity field exactly (v = A x + b + g dt, C = A)
+  MLS damage       J / J_ref thresholds (1 - eps_c)^m, (1 + eps_s)^m, off for J_ref = 0;
+                   singular-value clamp at 0.05; Eq. 7 softening by gamma (|dev e| - limit)
+                   with damage at zero yield stress, F returned to the yield surface
   blade constraint face particles leave a held-blade substep without velocity into the
                    faces beyond their gap (tests/test_blade_constraint.py:57-101)
 
@@ -433,3 +440,200 @@ def test_held_blade_face_particles_do_not_advance_into_the_faces():
     assert np.all(side[active] * vx >= -np.maximum(gap, 0) / cfg.dt - 1e-9)
     assert np.max(np.abs(vx)) < 0.5 * squeeze
     assert o.get("bc_max_violation")[0] <= 1e-6
+
+
+def _node_positions(origin, cells, h):
+    dims = np.asarray(cells) + 1
+    ix, iy, iz = np.meshgrid(*(np.arange(d) for d in dims), indexing="ij")
+    return np.asarray(origin) + h * np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1)
+
+
+def _hat_stencil(ref, node_x, h):
+    """Trilinear weights and reference gradients of every node at every particle, brute force
+    from the tent function N(u) = max(0, 1 - |u|) per axis (dense n x N)."""
+    u = (ref[:, None, :] - node_x[None, :, :]) / h
+    hat = np.clip(1.0 - np.abs(u), 0.0, None)
+    dhat = np.where(np.abs(u) < 1.0, -np.sign(u) / h, 0.0)
+    w = hat.prod(-1)
+    grad = np.stack([dhat[..., 0] * hat[..., 1] * hat[..., 2], hat[..., 0] * dhat[..., 1] * hat[..., 2],
+                     hat[..., 0] * hat[..., 1] * dhat[..., 2]], -1)
+    return w, grad
+
+
+def test_p2g_against_brute_force_tent_functions():
+    """P2G (material.py:127-152) in float64 against a dense sum over every (particle, node)
+    pair of tent-function weights: grid mass m w, momentum m w v, impulse w j - dt V P grad w.
+    Moment identities of the trilinear basis then pin the terms independently of the dense
+    sum: sum_a w_a = 1 gives the total mass and momentum, sum_a grad w_a = 0 leaves only the
+    contact impulses in the total impulse, and sum_a x_a grad w_a^T = I turns the first moment
+    of the impulse into sum_p j_p X_p^T - dt V sum_p P_p (a sign, a missing dt V or a
+    transposed stress fails)."""
+    cfg = Config(size=(0.036, 0.024, 0.018), grid_spacing=0.006, dt=2.0e-4, cache_topology=True)
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, weights, gradients = stencil(ref, origin, cfg.grid_spacing, cells)
+    N = int(np.prod(np.asarray(cells) + 1))
+    o = OracleSim(cfg, ref, nodes, weights, gradients, N, 64)
+    rng = np.random.default_rng(11)
+    n = len(ref)
+    v = rng.normal(0.0, 0.1, (n, 3))
+    imp = rng.normal(0.0, 1e-6, (n, 3))
+    P = rng.normal(0.0, 50.0, (n, 3, 3))  # not symmetric
+    o.set("v", v)
+    o.set("impulses", imp)
+    o.set("stress", P)
+    o.stage("p2g")
+    active = o.scalars()["active_count"]
+    keys = o.get("grid_keys", 8 * n)[:active]
+    assert np.array_equal(np.sort(keys), np.unique(nodes))  # one field per node, code 0
+    mass = o.get("grid_mass", 8 * n)[:active]
+    mom = o.get("momentum", 24 * n).reshape(-1, 3)[:active]
+    jmp = o.get("grid_impulse", 24 * n).reshape(-1, 3)[:active]
+    k = oracle.constants(cfg)
+    m, V, dt, h = k["mass"], k["volume"], cfg.dt, cfg.grid_spacing
+    node_x = _node_positions(origin, cells, h)
+    w, grad = _hat_stencil(ref, node_x, h)
+    want_mass = m * w.sum(0)
+    want_mom = m * w.T @ v
+    want_jmp = w.T @ imp - dt * V * np.einsum("pij,pnj->ni", P, grad)
+    np.testing.assert_allclose(mass, want_mass[keys], rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(mom, want_mom[keys], rtol=1e-10, atol=1e-15)
+    np.testing.assert_allclose(jmp, want_jmp[keys], rtol=1e-10, atol=1e-15)
+    # Moment identities.
+    xk = node_x[keys]
+    assert abs(mass.sum() - n * m) <= 1e-12 * n * m
+    np.testing.assert_allclose(mom.sum(0), m * v.sum(0), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(jmp.sum(0), imp.sum(0), rtol=0, atol=1e-12 * np.abs(P).max() * dt * V * n / h)
+    np.testing.assert_allclose(mom.T @ xk, m * v.T @ ref, rtol=1e-10, atol=1e-15)
+    first = jmp.T @ xk
+    np.testing.assert_allclose(first, imp.T @ ref - dt * V * P.sum(0), rtol=1e-9, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_affine_velocity_field_updates_F_by_dt_A(precision):
+    """One material step (material.py:127-475) of an affine velocity field v = A X + b from
+    F = I, zero stress and no contact.  A node whose eight cells are full of the regular
+    lattice gets the mass-weighted velocity A x + b, so a particle whose eight corners are
+    such nodes (cells 1 .. c-2) keeps v = A X + b and moves to X + dt (A X + b).  The F update
+    reads the re-scattered velocities, which are A x + b one layer further in, so a particle
+    in cells 2 .. c-3 gets F = I + dt A by sum_a v_a grad w_a^T = A (an outer product with its
+    arguments swapped gives I + dt A^T, a missing dt gives I + A)."""
+    cfg = Config(size=(0.048, 0.042, 0.036), grid_spacing=0.006, dt=2.0e-4, cache_topology=True)
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, weights, gradients = stencil(ref, origin, cfg.grid_spacing, cells)
+    N = int(np.prod(np.asarray(cells) + 1))
+    o = OracleSim(cfg, ref, nodes, weights, gradients, N, precision)
+    A = np.array([[0.3, -1.1, 0.4], [0.7, -0.2, 0.0], [-0.5, 0.9, 0.1]])
+    b = np.array([0.02, -0.01, 0.03])
+    v = ref @ A.T + b
+    o.set("v", v)
+    o.stage("material")
+    h = cfg.grid_spacing
+    cell = np.floor((ref - origin) / h).astype(int)
+    top = np.asarray(cells)
+    moved = np.all((cell >= 1) & (cell <= top - 2), axis=1)
+    inner = np.all((cell >= 2) & (cell <= top - 3), axis=1)
+    assert inner.sum() >= 100
+    tol = 1e-12 if precision == 64 else 2e-6
+    F = o.get("F").reshape(-1, 3, 3)[inner]
+    np.testing.assert_allclose(F, np.tile(np.eye(3) + cfg.dt * A, (inner.sum(), 1, 1)), rtol=0, atol=tol)
+    np.testing.assert_allclose(o.get("v").reshape(-1, 3)[moved], v[moved], rtol=0, atol=tol)
+    x = o.get("x").reshape(-1, 3)[moved]
+    np.testing.assert_allclose(x, ref[moved] + cfg.dt * v[moved], rtol=0, atol=tol * 0.1)
+
+
+def _mls_block(cfg, prm, lift=0.03):
+    ref, _, _ = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    ref = ref + np.array([0.0, 0.0, lift])  # clear of the board
+    dx = cfg.grid_spacing
+    low = ref.min(0) - prm.margin
+    high = ref.max(0) + prm.margin
+    low[2] = -2 * dx
+    high[2] = ref[:, 2].max() + prm.headroom
+    dims = np.ceil((high - low) / dx).astype(int) + 1
+    return ref, low, dims
+
+
+def test_mls_damage_threshold_and_singular_value_clamp():
+    """MLS-MPM g2p damage (mlsmpm.py:196-229) on a resting block of negligible stiffness, so
+    the substep leaves every F as it was: a particle is damaged exactly when
+    det F / J_ref <= (1 - eps_c)^m or >= (1 + eps_s)^m with J_ref > 0, never with J_ref = 0,
+    and a singular value below 0.05 is clamped to 0.05 (which then counts as compression)."""
+    from blademaster_hip.mlsmpm import TopoCutParams
+
+    cfg = Config(size=(0.036, 0.024, 0.018), grid_spacing=0.006, dt=1.0e-4, young_modulus=1.0e-6)
+    prm = TopoCutParams(compression=0.025, stretch=0.01, sensitivity=2.0)
+    ref, low, dims = _mls_block(cfg, prm)
+    n = len(ref)
+    o = oracle.OracleMls(cfg, prm, ref, low, dims, precision=64)
+    rng = np.random.default_rng(5)
+    J = rng.uniform(0.9, 1.05, n)
+    J_ref = rng.choice([0.0, 1.0, 1.02], n)
+    squashed = rng.random(n) < 0.05
+    F = np.zeros((n, 3, 3))
+    for p in range(n):
+        R = Rotation.from_rotvec(rng.normal(0, 0.3, 3)).as_matrix()
+        s = np.array([J[p] ** (1 / 3)] * 3)
+        if squashed[p]:
+            s = np.array([0.01, 1.0, 1.0])
+        F[p] = R @ np.diag(s) @ R.T
+    o.set("F", F)
+    o.set("damage_reference", J_ref)
+    o.set("tool_pos", [0.0, 0.0, 1.0])
+    o.substep(1)
+    out = o.get("F").reshape(-1, 3, 3)
+    sv = np.linalg.svd(out, compute_uv=False)
+    np.testing.assert_allclose(sv[squashed].min(1), 0.05, rtol=1e-9)
+    np.testing.assert_allclose(out[~squashed], F[~squashed], rtol=0, atol=1e-9)
+    det = np.where(squashed, 0.05, J)
+    lo, hi = (1 - prm.compression) ** prm.sensitivity, (1 + prm.stretch) ** prm.sensitivity
+    ratio = np.divide(det, J_ref, out=np.ones(n), where=J_ref > 0)
+    # stay clear of the thresholds by more than the substep's change of F
+    margin = np.minimum(np.abs(ratio - lo), np.abs(ratio - hi)) > 1e-7
+    want = (J_ref > 0) & ((ratio <= lo) | (ratio >= hi))
+    got = o.get("damage") > 0
+    assert np.array_equal(got[margin], want[margin])
+    assert want.sum() > 20 and (~want & (J_ref > 0)).sum() > 20 and squashed.sum() > 0
+
+
+def test_mls_softening_follows_the_plastic_increment():
+    """Hencky return map with Eq. 7 softening (mlsmpm.py:206-223): a trial strain with
+    deviatoric norm |dev e| above the yield limit sigma_y / (2 mu) sqrt(2/3) is returned to the
+    limit, and the yield stress drops by gamma (|dev e| - limit), floored at 0, where the
+    particle is damaged; a particle below the limit keeps its F and yield stress."""
+    from blademaster_hip.mlsmpm import TopoCutParams
+
+    E, nu = 1.0e-6, 0.3
+    mu = E / (2 * (1 + nu))
+    cfg = Config(size=(0.036, 0.024, 0.018), grid_spacing=0.006, dt=1.0e-4, young_modulus=E, poisson_ratio=nu,
+                 yield_stress=2.0e-8)
+    gamma = 1.0e-6
+    prm = TopoCutParams(softening=gamma, compression=0.5, stretch=0.5)
+    ref, low, dims = _mls_block(cfg, prm)
+    n = len(ref)
+    o = oracle.OracleMls(cfg, prm, ref, low, dims, precision=64)
+    rng = np.random.default_rng(9)
+    stretch = rng.uniform(0.0, 0.08, n)
+    F = np.zeros((n, 3, 3))
+    for p in range(n):
+        R = Rotation.from_rotvec(rng.normal(0, 0.3, 3)).as_matrix()
+        s = np.exp(np.array([stretch[p], -stretch[p] / 2, -stretch[p] / 2]))  # volume preserving
+        F[p] = R @ np.diag(s) @ R.T
+    o.set("F", F)
+    o.set("tool_pos", [0.0, 0.0, 1.0])
+    o.substep(1)
+    e = np.stack([stretch, -stretch / 2, -stretch / 2], 1)
+    norm = np.linalg.norm(e - e.mean(1, keepdims=True), axis=1)
+    limit = cfg.yield_stress / (2 * mu) * np.sqrt(2 / 3)
+    clear = np.abs(norm - limit) > 1e-6
+    yielded = norm > limit
+    ys = o.get("yield_stress")
+    want_ys = np.where(yielded, np.maximum(0.0, cfg.yield_stress - gamma * (norm - limit)), cfg.yield_stress)
+    np.testing.assert_allclose(ys[clear], want_ys[clear], rtol=1e-6, atol=1e-14)
+    out = o.get("F").reshape(-1, 3, 3)
+    sv = np.linalg.svd(out, compute_uv=False)
+    dev_out = np.log(sv) - np.log(sv).mean(1, keepdims=True)
+    np.testing.assert_allclose(np.linalg.norm(dev_out, axis=1)[yielded & clear], limit, rtol=1e-6)
+    np.testing.assert_allclose(out[~yielded & clear], F[~yielded & clear], rtol=0, atol=1e-9)
+    dmg = o.get("damage") > 0
+    assert np.array_equal(dmg[clear], (want_ys <= 0)[clear])
+    assert yielded.sum() > 50 and (~yielded).sum() > 10 and (want_ys <= 0).sum() > 5
