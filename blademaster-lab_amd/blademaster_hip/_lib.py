@@ -187,6 +187,7 @@ SIGNATURES = {
     "bm_mls_g2p": [MP, MS, VP],
     "bm_mls_substep": [P, S, MP, MS, VP],
     "bm_piece_labels": [P, S, ctypes.POINTER(Labels), VP],
+    "bm_piece_labels_static": [P, S, I64, VP, VP, VP, VP, VP],
     "bm_load_external_batch": [P, S, VP, VP],
     "bm_store_interval_batch": [P, S, VP, VP],
     "bm_mlp_sizes": [I32, I32, I32, c_i64p, c_i64p, c_i64p],
@@ -215,6 +216,8 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    lib.bm_piece_labels_static_bytes.argtypes = [I64, I64, I64]
+    lib.bm_piece_labels_static_bytes.restype = ctypes.c_int64
     _lib = lib
     return lib
 

@@ -16,10 +16,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "bm_device.h"
 #include "bm_hip.h"
 
 using namespace bm;
+
+// Scratch buffers of bm_piece_labels_static: one file-scope array on every GPU the library
+// runs on, held until the process ends.  On the host its name is only a stand-in; host code
+// uses the device address that hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = size_t(BM_LABELS_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -64,6 +72,8 @@ __device__ void unite(int32_t* parent, int32_t a, int32_t b, int32_t* errors) {
   }
   atomicAdd(errors, 1);
 }
+
+__global__ void k_lbl_clear_errors(int32_t* errors) { errors[0] = 0; }
 
 __global__ void k_lbl_init(bm_labels L, int64_t n, int64_t FN, int64_t N) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -201,17 +211,9 @@ int launch_status() {
   return e == hipSuccess ? 0 : int(e);
 }
 
-}  // namespace
-
-extern "C" int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream) {
-  if (!p || !s || !l || !s->codes || !s->base_node || !l->corner_mask || !l->field_code || !l->node_nf ||
-      !l->field_idx || !l->parent || !l->root || !l->block_sum || !l->labels || !l->count || !l->errors)
-    return -1;
-  if (p->n < 0 || l->fields < 1 || l->fields > 255) return -1;
+// The launch sequence shared by both entry points; l holds every buffer.
+int launch_labels(const bm_params* p, const bm_state* s, const bm_labels* l, hipStream_t st) {
   int64_t FN = l->fields * p->node_count;
-  if (p->n + FN >= (int64_t(1) << 31)) return -2;
-  if (p->n == 0) return 0;
-  hipStream_t st = (hipStream_t)stream;
   int64_t init = p->n + FN;
   hipLaunchKernelGGL(k_lbl_init, dim3(blocks_for(init)), dim3(kBlock), 0, st, *l, p->n, FN, p->node_count);
   hipLaunchKernelGGL(k_lbl_register, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, *l);
@@ -223,4 +225,81 @@ extern "C" int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_l
   hipLaunchKernelGGL(k_lbl_rank, dim3(nb), dim3(kScanBlock), 0, st, *p, *l);
   hipLaunchKernelGGL(k_lbl_label, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *l);
   return launch_status();
+}
+
+// ------------------------------------------------------------------ no-allocation variant
+constexpr int kMaxDevices = 64;
+unsigned char* scratch_address[kMaxDevices] = {};
+std::mutex scratch_lock;
+
+// Device address of the current device's scratch array, looked up on its first use there.
+int scratch_of_current_device(unsigned char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -4;
+  std::lock_guard<std::mutex> guard(scratch_lock);
+  if (scratch_address[dev] == nullptr) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) return -4;
+    scratch_address[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = scratch_address[dev];
+  return 0;
+}
+
+inline int64_t aligned(int64_t bytes) { return (bytes + 255) & ~int64_t(255); }
+
+}  // namespace
+
+extern "C" int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream) {
+  if (!p || !s || !l || !s->codes || !s->base_node || !l->corner_mask || !l->field_code || !l->node_nf ||
+      !l->field_idx || !l->parent || !l->root || !l->block_sum || !l->labels || !l->count || !l->errors)
+    return -1;
+  if (p->n < 0 || l->fields < 1 || l->fields > 255) return -1;
+  int64_t FN = l->fields * p->node_count;
+  if (p->n + FN >= (int64_t(1) << 31)) return -2;
+  if (p->n == 0) return 0;
+  return launch_labels(p, s, l, (hipStream_t)stream);
+}
+
+extern "C" int64_t bm_piece_labels_static_bytes(int64_t n, int64_t node_count, int64_t fields) {
+  if (n < 0 || node_count < 0 || fields < 1 || fields > 255) return -1;
+  int64_t FN = fields * node_count;
+  return aligned(FN * 8) + aligned(node_count * 4) + aligned(n * 8) + aligned((n + FN) * 4) + aligned(n * 4) +
+         aligned(((n + kScanBlock - 1) / kScanBlock) * 4);
+}
+
+extern "C" int bm_piece_labels_static(const bm_params* p, const bm_state* s, int64_t fields,
+                                      const uint8_t* corner_mask, int32_t* labels, int32_t* count, int32_t* errors,
+                                      void* stream) {
+  if (!p || !s || !s->codes || !s->base_node || !corner_mask || !labels || !count || !errors) return -1;
+  if (p->n < 0 || p->node_count < 0 || fields < 1 || fields > 255) return -1;
+  int64_t FN = fields * p->node_count;
+  if (p->n + FN >= (int64_t(1) << 31)) return -2;
+  if (bm_piece_labels_static_bytes(p->n, p->node_count, fields) > int64_t(kScratchBytes)) return -3;
+  if (p->n == 0) return 0;
+  unsigned char* base = nullptr;
+  int code = scratch_of_current_device(&base);
+  if (code) return code;
+  bm_labels l;
+  l.fields = fields;
+  l.corner_mask = corner_mask;
+  int64_t at = 0;
+  l.field_code = reinterpret_cast<int64_t*>(base + at);
+  at += aligned(FN * 8);
+  l.node_nf = reinterpret_cast<int32_t*>(base + at);
+  at += aligned(p->node_count * 4);
+  l.field_idx = base + at;
+  at += aligned(p->n * 8);
+  l.parent = reinterpret_cast<int32_t*>(base + at);
+  at += aligned((p->n + FN) * 4);
+  l.root = reinterpret_cast<int32_t*>(base + at);
+  at += aligned(p->n * 4);
+  l.block_sum = reinterpret_cast<int32_t*>(base + at);
+  l.labels = labels;
+  l.count = count;
+  l.errors = errors;
+  // The error count is an output that starts at zero: cleared by a kernel (no memset node
+  // when the caller records this call into a graph).
+  hipLaunchKernelGGL(k_lbl_clear_errors, dim3(1), dim3(1), 0, (hipStream_t)stream, errors);
+  return launch_labels(p, s, &l, (hipStream_t)stream);
 }

@@ -291,6 +291,21 @@ int bm_store_interval_batch(const bm_params* p, const bm_state* s, float* out, v
 /* Replaces topology.components (topology.py:17-57) for a solver's current codes. */
 int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream);
 
+/* No counterpart in the reference.  bm_piece_labels without allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops): the
+ * scratch buffers (field table of `fields` slots per node, union-find forest, roots, block
+ * sums) live in a static device array of BM_LABELS_SCRATCH_BYTES on each GPU, kept until the
+ * process ends.  The caller passes the inputs (solver codes and stencil through p and s, the
+ * positive-weight corner mask [n]) and the outputs labels [n], count [1] (pieces) and
+ * errors [1] (nonzero when a node needed more than `fields` codes: call again with more).
+ * Returns -3 when bm_piece_labels_static_bytes(n, N, fields) exceeds the array, -4 when the
+ * array's device address cannot be looked up.  Calls on one GPU share its array and keep
+ * the last call's contents; calls that overlap on one GPU (two streams) race. */
+#define BM_LABELS_SCRATCH_BYTES (64ll << 20)
+int64_t bm_piece_labels_static_bytes(int64_t n, int64_t node_count, int64_t fields);
+int bm_piece_labels_static(const bm_params* p, const bm_state* s, int64_t fields, const uint8_t* corner_mask,
+                           int32_t* labels, int32_t* count, int32_t* errors, void* stream);
+
 /* ---------------------------------------------------------------------------------
  * MLS-MPM cutting baseline (blademaster_warp/mlsmpm.py:42-229, MlsSolver._substep
  * mlsmpm.py:293-393).  Dense grid of dims nodes with planes [3][nodes] for momentum and

