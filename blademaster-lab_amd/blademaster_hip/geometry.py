@@ -139,13 +139,26 @@ BRICK_Z = 4
 LDS_BYTES = 65536
 
 
-def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
+def lattice_slots(local):
+    """Cell slot 4 bx + 2 by + bz of each particle from its local cell coordinates when every
+    particle sits on the regular lattice (1/4 or 3/4 of the cell per axis, exactly in
+    float32), else None (jittered sampling)."""
+    local = np.asarray(local, dtype=np.float32)
+    if not np.all((local == np.float32(0.25)) | (local == np.float32(0.75))):
+        return None
+    bits = (local > 0.5).astype(np.int64)
+    return 4 * bits[:, 0] + 2 * bits[:, 1] + bits[:, 2]
+
+
+def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z, lattice=None):
     """Particle bins of the brick kernels.
 
     Returns (brick cells per axis, bricks per axis, slots): bricks are numbered x-major, the
-    cells of a brick z-fastest, and the c-th cell in this order holds the particles
-    slots[8c:8c + 8] in reference order, -1 for unused slots.  The sampling puts at most 8
-    particles in a cell (2 per axis, jitter stays inside the sub-cell).
+    cells of a brick z-fastest, and the c-th cell in this order holds its particles in
+    slots[8c:8c + 8], -1 for unused slots.  With `lattice` (from `lattice_slots`) a particle
+    takes the slot of its lattice position; otherwise the cell's particles fill its first
+    slots in reference order.  The sampling puts at most 8 particles in a cell (2 per axis,
+    jitter stays inside the sub-cell).
     The brick kernels keep per (field slot, window node) 7 (P2G) or 3 (G2P) floats in LDS,
     plus 7 or 3 floats per (cell, corner); the larger must fit LDS_BYTES.
     """
@@ -174,10 +187,16 @@ def bin_bricks(cells, base_cell, fields, brick_xy=BRICK_XY, brick_z=BRICK_Z):
     perm = np.argsort(key, kind="stable")
     nb = int(np.prod(bricks))
     sorted_key = key[perm]
+    slots = np.full(8 * nb * per_brick, -1, dtype=np.int32)
+    if lattice is not None:
+        index = 8 * key + np.asarray(lattice, dtype=np.int64)
+        if len(np.unique(index)) != len(index):
+            raise ValueError("two particles on one lattice position")
+        slots[index] = np.arange(len(key), dtype=np.int32)
+        return brick, bricks, slots
     start = np.searchsorted(sorted_key, sorted_key)
     rank = np.arange(len(perm)) - start
     if len(rank) and rank.max() >= 8:
         raise ValueError("more than 8 particles in a grid cell")
-    slots = np.full(8 * nb * per_brick, -1, dtype=np.int32)
     slots[8 * sorted_key + rank] = perm
     return brick, bricks, slots

@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .arrays import DArray
 from .config import Config, derived, max_cut_index
-from .geometry import points, stencil, compact_stencil, bin_bricks
+from .geometry import points, stencil, compact_stencil, bin_bricks, lattice_slots
 
 # Listed pair partners per particle before the bucket fallback (solver.py:152-154).
 PAIR_LIST_LANES = 16
@@ -118,6 +118,9 @@ class Solver:
             return DArray(tensor, hook)
 
         base_node, local, base_cell = compact_stencil(self.reference, self.origin, cfg.grid_spacing, cells)
+        # Regular lattice: the transfer kernels use the separable stencil.
+        self._lattice = lattice_slots(local) if n else None
+        self.regular = self._lattice is not None
         self._bin_bricks(base_cell)
         eye = torch.eye(3, dtype=f32, device=dev).expand(n, 3, 3).contiguous()
         self._t = {}
@@ -323,6 +326,7 @@ class Solver:
         p.instances = getattr(self, "instances", 1)
         p.active_bricks = self.active_bricks
         p.active_nodes = self.active_nodes
+        p.regular = int(self.regular)
         return p
 
     def _make_state(self):
@@ -361,7 +365,7 @@ class Solver:
     def _bin_bricks(self, base_cell):
         kw = {} if self._brick_z is None else dict(brick_z=int(self._brick_z))
         self.brick, self.bricks, self.brick_slots = bin_bricks(self.cells, base_cell, int(self.config.fields_per_node),
-                                                               **kw)
+                                                               lattice=self._lattice, **kw)
 
     def launch(self, name, *args):
         """Launch one C-ABI stage by name with the solver's params/state (cf. solver.py:331)."""

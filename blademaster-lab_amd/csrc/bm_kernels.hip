@@ -1185,20 +1185,85 @@ __device__ __forceinline__ int cell_window_base(const bm_params& P, const Window
 }
 __device__ __forceinline__ int cells_per_brick(const bm_params& P) { return int(P.brick[0] * P.brick[1] * P.brick[2]); }
 
-// Sum over the 8 lanes of a cell (lanes 8c .. 8c+7 of the wavefront) with DPP lane moves:
-// swap neighbours, swap pairs, then mirror the two quads; every lane of the group ends with
-// the total.  Call with the whole wavefront active.
+// Lanes of a brick workgroup and the particle slots of its cells.  Slot s of a cell (0..7)
+// is 4 bx + 2 by + bz; on a regular lattice (bm_params.regular) the particle in slot s sits
+// at local cell coordinates (bx, by, bz) / 2 + 1/4.  Two cells share each row of 16 lanes so
+// that the slot's position bits z, y, x are lane bits 0, 1, 3: lane l holds slot
+// (l & 3) | (l >> 3 & 1) << 2 of cell 2 (l >> 4) + (l >> 2 & 1), and lanes differing in one
+// position bit are one DPP move apart (xor 1 and 2 by quad_perm, xor 8 by row_ror:8).
+__device__ __forceinline__ int lane_cell(int l) { return ((l >> 4) << 1) | ((l >> 2) & 1); }
+__device__ __forceinline__ int lane_slot(int l) { return (l & 3) | (((l >> 3) & 1) << 2); }
+
+// The lane's value from its partner across slot position bit Axis (0 x, 1 y, 2 z).  Call
+// with the whole wavefront active.
+template <int Axis>
+__device__ __forceinline__ float partner(float v) {
+  if constexpr (Axis == 0) return dpp_move<0x128>(v);  // row_ror:8
+  else if constexpr (Axis == 1) return dpp_move<0x4E>(v);  // quad_perm [2,3,0,1]
+  else return dpp_move<0xB1>(v);  // quad_perm [1,0,3,2]
+}
+
+// Sum over the 8 lanes of a cell: every lane of the cell ends with the total.
 __device__ __forceinline__ float cell_sum(float v) {
-  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_move<0x141>(v);  // row_half_mirror: lane i <- 7 - i within 8
+  v += partner<2>(v);
+  v += partner<1>(v);
+  v += partner<0>(v);
   return v;
 }
-// Any particle in this lane's cell, from a wavefront ballot of `live`.
-__device__ __forceinline__ bool cell_occupied(bool live) {
-  uint64_t mask = __ballot(live);
-  return ((mask >> (threadIdx.x & 56)) & 0xffull) != 0;
+
+// Separable trilinear stencil of a regular lattice.  Along one axis the weights of corner bit
+// b at the particles of position bit p are phi_b(1/4 + p/2): 3/4 when b = p, 1/4 otherwise,
+// and their derivatives are (2 b - 1) / h whatever p.  So the 8 x 8 (corner, particle) weight
+// matrix of a cell is a tensor product of 2 x 2 factors, applied one axis at a time by lane
+// pairs:
+//   phi:  4 x the weighted pair   3 q + q'      (weights; symmetric, so scatter = gather)
+//   both: the pair sum            q + q'        (scatter with the derivative: x (2b-1)/h)
+//   diff: the pair difference     q - q'        (gather with the derivative:  x (2p-1)/h)
+// where q' is the partner's value.  Scattering particle values with phi along all three
+// axes leaves 64 x the cell's corner sum of w q in the lane of that corner; gathering corner
+// values the same way leaves 64 x sum_a w_a q_a in the lane of each particle.
+template <int Axis>
+__device__ __forceinline__ float phi(float q) { return 3.0f * q + partner<Axis>(q); }
+template <int Axis>
+__device__ __forceinline__ float both(float q) { return q + partner<Axis>(q); }
+template <int Axis>
+__device__ __forceinline__ float diff(float q) { return q - partner<Axis>(q); }
+__device__ __forceinline__ float phi3(float q) { return phi<0>(phi<1>(phi<2>(q))); }
+// Signs of the derivative at the lane's position bits, over 16 h: the scale of a stage
+// sequence with one both/diff and two phi stages.
+__device__ __forceinline__ f3 gradient_signs(int lane, float inv_h) {
+  float g = inv_h * (1.0f / 16.0f);
+  return f3{(lane >> 3) & 1 ? g : -g, (lane >> 1) & 1 ? g : -g, lane & 1 ? g : -g};
 }
+// Any live particle in the lane's cell, from a wavefront ballot of live lanes.
+__device__ __forceinline__ bool cell_live(uint64_t live_mask, int lane) {
+  return ((live_mask >> (lane & 48)) & (0x0F0Full << (4 * ((lane >> 2) & 1)))) != 0;
+}
+// Grid node of corner s of the c-th cell of a brick (the cell must hold a particle, so the
+// node exists).
+__device__ __forceinline__ int64_t cell_corner_node(const bm_params& P, const Window& w, const Divisors& D, int c,
+                                                   int s) {
+  int bz = int(P.brick[2]), by = int(P.brick[1]);
+  int qz = quot(D.bz, c), lx = quot(D.bybz, c);
+  int lz = c - qz * bz, ly = qz - lx * by;
+  int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
+  return int64_t(w.x0 + lx) * dyz + int64_t(w.y0 + ly) * dz + (w.z0 + lz) + corner_offset(s, int(dyz), int(dz));
+}
+// Velocity gradient sum_a v_a grad w_a^T at the lane's particle from the lane's corner
+// velocity v (corner = the lane's slot), by the separable stencil; sg from gradient_signs.
+__device__ __forceinline__ m3 separable_rate(f3 v, f3 sg) {
+  m3 r;
+  float vi[3] = {v.x, v.y, v.z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float a = phi<2>(vi[i]);
+    r.a[3 * i] = sg.x * diff<0>(phi<1>(a));
+    r.a[3 * i + 1] = sg.y * phi<0>(diff<1>(a));
+    r.a[3 * i + 2] = sg.z * phi<0>(phi<1>(diff<2>(vi[i])));
+  }
+  return r;
+}
+
 // Sum over the (up to 8) cells of the brick around window node wi of their corner entries
 // cc[k * CS + 8 c + a], in fixed corner order.
 template <int K>
@@ -1224,14 +1289,24 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 }
 
 // p2g (material.py:127-152) for one brick: mass, momentum and stress + contact impulse.
-// One lane per particle slot, 8 slots per cell (at most 1024 slots per brick).  Per stencil
-// corner the cell's lanes sum the 7 values of its particles on field slot 0 (DPP reduction)
-// and lane 8c + a stores the sums of corner a in a per-(cell, corner) LDS table with plain
-// stores; a particle on another field slot (a cut through the node) adds its values to an LDS
-// node window with atomics.  Each window node then gathers its cells' corner sums in fixed
-// order, adds the window, and the brick's partial goes out with plain stores.  When the last
-// substep's F update is pending (update_pending), each lane first updates its particle's F and
-// stress (update_F_plastic, material.py:445-475) and uses that stress.
+// One lane per particle slot, 8 slots per cell (at most 1024 slots per brick; lane_cell and
+// lane_slot give a lane's cell and slot).  The lane of slot a of a cell ends with the cell's
+// sums at corner a of the 7 values of its particles on field slot 0, and stores them in a
+// per-(cell, corner) LDS table with plain stores; a particle on another field slot (a cut
+// through the node) adds its values to an LDS node window with atomics.  Each window node
+// then gathers its cells' corner sums in fixed order, adds the window, and the brick's
+// partial goes out with plain stores.  The cell sums come from one of two paths, chosen per
+// wavefront:
+//  - separable: on a regular lattice, when none of the wavefront's particles has a field
+//    slot other than 0, the corner sums are the separable stencil's lane-pair stages
+//    (phi/both), about 100 VALU per lane;
+//  - per corner: otherwise each lane forms its 7 values per stencil corner and the cell's
+//    lanes sum them (cell_sum).
+// When the last substep's F update is pending (update_pending), each lane first updates its
+// particle's F and stress (update_F_plastic, material.py:445-475) and uses that stress; on
+// the separable path the corner velocities of that update are one gather per lane (the lane's
+// corner) and the stencil's gather stages.
+__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate);
 template <bool kRemap>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx);
@@ -1258,9 +1333,9 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   int CS = 8 * cpb;
   float* win = lds;          // [7][F][W]
   float* cc = lds + 7 * FW;  // [7][CS]
-  int slot = int(threadIdx.x);
-  int c = slot >> 3;
-  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+  int lane = int(threadIdx.x & 63);
+  int c = lane_cell(int(threadIdx.x)), a_mine = lane_slot(int(threadIdx.x));
+  int64_t p = c < cpb ? int64_t(S.perm[int64_t(b) * CS + 8 * c + a_mine]) : -1;
   bool live = p >= 0;
   int64_t my_node = int(threadIdx.x) < w.W ? window_node(P, w, D, int(threadIdx.x)) : -1;
   int my_nf = my_node >= 0 ? S.node_nf[my_node] : 0;
@@ -1270,14 +1345,15 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
     __syncthreads();
   }
   float mass = float(P.mass), sv = float(P.dt) * float(P.volume), inv_h = 1.0f / float(P.spacing);
-  int lane = int(threadIdx.x & 63);
   float mine[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   // The last substep's F update, when it was deferred to this pass (bm_substep_deferred).
   const int pending = S.update_pending != nullptr ? S.update_pending[0] : 0;
-  if (__ballot(live) != 0) {
+  const uint64_t live_mask = __ballot(live);
+  if (live_mask != 0) {
     f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
     m3 Pk = mzero();
     uint2 fidx = make_uint2(0u, 0u);
+    bool update = false;
     if (live) {
       vp = ld3(S.v, p);
       imp = ld3(S.impulses, p);
@@ -1286,42 +1362,73 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       // pending 2: the particles marked in update_done were updated at the end of the last
       // substep; their marks are cleared here.
       const bool done = pending == 2 && S.update_done[p] != 0;
-      if (pending && !done) {
-        Pk = update_particle<false>(P, S, D, p, lp, fidx);
-      } else {
-        Pk = ld9(S.stress, p);
-        if (done) S.update_done[p] = 0;
+      update = pending && !done;
+      if (done) S.update_done[p] = 0;
+    }
+    const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0;
+    const f3 sg = gradient_signs(lane, inv_h);
+    if (separable) {
+      if (__ballot(update) != 0) {
+        // Corner velocity of the lane's slot in its cell; every lane of an occupied cell
+        // gathers, whether or not its own slot holds a particle to update.
+        f3 vc = f3{0.f, 0.f, 0.f};
+        if (cell_live(live_mask, lane)) vc = ld3(S.grid_vel, cell_corner_node(P, w, D, c, a_mine));
+        m3 rate = separable_rate(vc, sg);
+        if (update) Pk = finish_update(P, S, p, rate);
       }
+    } else if (update) {
+      Pk = update_particle<false>(P, S, D, p, lp, fidx);
+    }
+    if (live) {
+      if (!update) Pk = ld9(S.stress, p);
 #pragma unroll
       for (int k = 0; k < 9; ++k) Pk.a[k] *= sv;
     }
-    int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
-#pragma unroll 2
-    for (int a = 0; a < 8; ++a) {
-      int f = field_of(fidx, a);
-      float wt;
-      f3 ga;
-      stencil_corner(lp, inv_h, a, wt, ga);
-      if (!live) wt = 0.0f;
-      float mw = mass * wt;
-      f3 jm = wt * imp - mvec(Pk, ga);
-      float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
-      bool own = live && f != 0;
-      if (own) {
-        int idx = __mul24(f, w.W) + wb + window_corner(w, a);
+    if (separable) {
+      // mass, momentum: 64 x the corner sums of m w and m w v; impulse: 64 x sum w j minus
+      // sum_j (2 b_j - 1) / (16 h) x the stages with the derivative along j of the stress.
+      float m64 = mass * (1.0f / 64.0f);
+      mine[0] = m64 * phi3(live ? 1.0f : 0.0f);
+      mine[1] = m64 * phi3(vp.x);
+      mine[2] = m64 * phi3(vp.y);
+      mine[3] = m64 * phi3(vp.z);
+      float ji[3] = {imp.x, imp.y, imp.z};
 #pragma unroll
-        for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
+      for (int i = 0; i < 3; ++i) {
+        float gx = both<0>(phi<1>(phi<2>(Pk.a[3 * i])));
+        float gy = phi<0>(both<1>(phi<2>(Pk.a[3 * i + 1])));
+        float gz = phi<0>(phi<1>(both<2>(Pk.a[3 * i + 2])));
+        mine[4 + i] = phi3(ji[i]) * (1.0f / 64.0f) - (sg.x * gx + sg.y * gy + sg.z * gz);
       }
+    } else {
+      int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
+#pragma unroll 2
+      for (int a = 0; a < 8; ++a) {
+        int f = field_of(fidx, a);
+        float wt;
+        f3 ga;
+        stencil_corner(lp, inv_h, a, wt, ga);
+        if (!live) wt = 0.0f;
+        float mw = mass * wt;
+        f3 jm = wt * imp - mvec(Pk, ga);
+        float cv[7] = {mw, mw * vp.x, mw * vp.y, mw * vp.z, jm.x, jm.y, jm.z};
+        bool own = live && f != 0;
+        if (own) {
+          int idx = __mul24(f, w.W) + wb + window_corner(w, a);
 #pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        float t = cell_sum(own ? 0.0f : cv[k]);
-        if ((lane & 7) == a) mine[k] = t;
+          for (int k = 0; k < 7; ++k) atomicAdd(win + k * FW + idx, cv[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          float t = cell_sum(own ? 0.0f : cv[k]);
+          if (a_mine == a) mine[k] = t;
+        }
       }
     }
   }
-  if (slot < CS) {
+  if (c < cpb) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) cc[k * CS + slot] = mine[k];
+    for (int k = 0; k < 7; ++k) cc[k * CS + 8 * c + a_mine] = mine[k];
   }
   wave_max_store(my_nf, wave_nf);
   __syncthreads();
@@ -1505,11 +1612,14 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
 
 __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp);
 
-// g2p_remap (material.py:384-401) for one brick: each particle lane reads the old and new
-// velocities of its 8 corners from the grid arrays, updates v and x, and re-scatters m w v
-// through the same cell-corner table as k_p2g into remap_partial.  In a substep the updated
-// particle is also labelled (assign, contact.py:996-1039) and checked against the pair
-// list's skin (check_pair_list, contact.py:716-735) while x and v are in registers.
+// g2p_remap (material.py:384-401) for one brick: each particle lane gathers the old and new
+// velocities of its 8 corners, updates v and x, and re-scatters m w v through the same
+// cell-corner table as k_p2g into remap_partial.  As in k_p2g, a wavefront whose particles
+// are all on field slot 0 of a regular lattice gathers one corner per lane and applies the
+// separable stencil's stages (phi) for the gather and the re-scatter; otherwise each lane
+// reads its 8 corners and the re-scatter sums per corner (cell_sum).  In a substep the updated
+// particle is also labelled (assign, contact.py:996-1039) and checked against the pair list's
+// skin (check_pair_list, contact.py:716-735) while x and v are in registers.
 __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags,
                                               const Divisors D) {
   extern __shared__ float lds[];
@@ -1523,9 +1633,9 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   float* win = lds;             // [3][F][W]
   float* cc = lds + 3 * FW;     // [3][CS]
   int64_t N = P.node_count;
-  int slot = int(threadIdx.x);
-  int c = slot >> 3;
-  int64_t p = slot < CS ? int64_t(S.perm[int64_t(b) * CS + slot]) : -1;
+  int lane = int(threadIdx.x & 63);
+  int c = lane_cell(int(threadIdx.x)), a_mine = lane_slot(int(threadIdx.x));
+  int64_t p = c < cpb ? int64_t(S.perm[int64_t(b) * CS + 8 * c + a_mine]) : -1;
   bool live = p >= 0;
   f3 lp = f3{0.f, 0.f, 0.f}, xp = lp, vold = lp;
   uint2 fidx = make_uint2(0u, 0u);
@@ -1536,6 +1646,15 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     xp = ld3(S.x, p);
     vold = ld3(S.v, p);
     base = S.base_node[p];
+  }
+  const uint64_t live_mask = __ballot(live);
+  const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0;
+  // The separable path's corner velocities: the lane's corner of its cell, field slot 0.
+  f3 o_c = f3{0.f, 0.f, 0.f}, n_c = o_c;
+  if (separable && cell_live(live_mask, lane)) {
+    int64_t node = cell_corner_node(P, w, D, c, a_mine);
+    o_c = ld3(S.old_v, node);
+    n_c = ld3(S.new_v, node);
   }
   // The window's field count and the cleared cross-field accumulator; the barrier that
   // orders the clearing before the accumulation comes after the velocity gather below, so
@@ -1553,16 +1672,22 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   }
   wave_max_store(my_nf, wave_nf);
   float flip = float(P.flip), dt = float(P.dt), mass = float(P.mass);
-  int lane = int(threadIdx.x & 63);
   float mine[3] = {0.f, 0.f, 0.f};
   f3 vp = f3{0.f, 0.f, 0.f};
-  if (live) {
+  f3 pic = vp, dv = vp;
+  if (separable) {
+    if (live_mask != 0) {
+      const float s64 = 1.0f / 64.0f;
+      pic = s64 * f3{phi3(n_c.x), phi3(n_c.y), phi3(n_c.z)};
+      f3 d = n_c - o_c;
+      dv = s64 * f3{phi3(d.x), phi3(d.y), phi3(d.z)};
+    }
+  } else if (live) {
     // The corner velocities come straight from old_v / new_v (node arrays small enough to
     // stay in L2).
     int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
     float wt[8];
     stencil_weights(lp, wt);
-    f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       int64_t s = int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz));
@@ -1570,6 +1695,8 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
       pic = pic + wt[a] * nv;
       dv = dv + wt[a] * (nv - o);
     }
+  }
+  if (live) {
     vp = (1.0f - flip) * pic + flip * (vold + dv);
     xp = xp + dt * pic;
     st3(S.v, p, vp);
@@ -1583,35 +1710,42 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     }
   }
   __syncthreads();
-  if (__ballot(live) != 0) {
-    int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
+  if (live_mask != 0) {
+    if (separable) {
+      float m64 = mass * (1.0f / 64.0f);
+      mine[0] = m64 * phi3(vp.x);
+      mine[1] = m64 * phi3(vp.y);
+      mine[2] = m64 * phi3(vp.z);
+    } else {
+      int wb = cell_window_base(P, w, D, c < cpb ? c : 0);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      int f = field_of(fidx, a);
-      float wt;
-      f3 ga;
-      stencil_corner(lp, 0.0f, a, wt, ga);
-      if (!live) wt = 0.0f;
-      f3 r = (mass * wt) * vp;
-      bool own = live && f != 0;
-      if (own) {
-        int idx = __mul24(f, w.W) + wb + window_corner(w, a);
-        atomicAdd(win + idx, r.x);
-        atomicAdd(win + FW + idx, r.y);
-        atomicAdd(win + 2 * FW + idx, r.z);
-      }
-      float tx = cell_sum(own ? 0.0f : r.x), ty = cell_sum(own ? 0.0f : r.y), tz = cell_sum(own ? 0.0f : r.z);
-      if ((lane & 7) == a) {
-        mine[0] = tx;
-        mine[1] = ty;
-        mine[2] = tz;
+      for (int a = 0; a < 8; ++a) {
+        int f = field_of(fidx, a);
+        float wt;
+        f3 ga;
+        stencil_corner(lp, 0.0f, a, wt, ga);
+        if (!live) wt = 0.0f;
+        f3 r = (mass * wt) * vp;
+        bool own = live && f != 0;
+        if (own) {
+          int idx = __mul24(f, w.W) + wb + window_corner(w, a);
+          atomicAdd(win + idx, r.x);
+          atomicAdd(win + FW + idx, r.y);
+          atomicAdd(win + 2 * FW + idx, r.z);
+        }
+        float tx = cell_sum(own ? 0.0f : r.x), ty = cell_sum(own ? 0.0f : r.y), tz = cell_sum(own ? 0.0f : r.z);
+        if (a_mine == a) {
+          mine[0] = tx;
+          mine[1] = ty;
+          mine[2] = tz;
+        }
       }
     }
   }
-  if (slot < CS) {
-    cc[slot] = mine[0];
-    cc[CS + slot] = mine[1];
-    cc[2 * CS + slot] = mine[2];
+  if (c < cpb) {
+    cc[8 * c + a_mine] = mine[0];
+    cc[CS + 8 * c + a_mine] = mine[1];
+    cc[2 * CS + 8 * c + a_mine] = mine[2];
   }
   __syncthreads();
   int Fb = window_max(wave_nf);
@@ -1833,6 +1967,27 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp, mu, lam);
 }
 
+// F += dt rate for particle p, then the stress; stores F (and Fp after plastic flow) and the
+// stress, and returns the stress.
+__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate) {
+  // While no plastic flow has happened and the host has not written Fp, every Fp is I.
+  bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
+  m3 Fm = ld9(S.F, p), Fp;
+  if (fp_identity) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fp.a[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+  } else {
+    Fp = ld9(S.Fp, p);
+  }
+  float dt = float(P.dt);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
+  st9(S.F, p, Fm);
+  m3 stress = stress_of(P, S, p, Fm, Fp);
+  st9(S.stress, p, stress);
+  return stress;
+}
+
 // update_F_plastic (material.py:445-475) for one particle: F += dt Σ_a v_a ⊗ ∇w_a over its
 // 8 corners, then the stress.  The corner velocities of its field come straight from
 // grid_vel (a node array small enough to stay in L2), or with kRemap from the re-scattered
@@ -1841,19 +1996,10 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
 template <bool kRemap>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx) {
-  // While no plastic flow has happened and the host has not written Fp, every Fp is I.
-  bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
   int32_t base = S.base_node[p];
-  m3 Fm = ld9(S.F, p), Fp;
-  if (fp_identity) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Fp.a[k] = (k % 4 == 0) ? 1.0f : 0.0f;
-  } else {
-    Fp = ld9(S.Fp, p);
-  }
   int64_t N = P.node_count;
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
-  float dt = float(P.dt), inv_h = 1.0f / float(P.spacing);
+  float inv_h = 1.0f / float(P.spacing);
   m3 rate = mzero();
   if constexpr (kRemap) {
     // One corner at a time: each velocity is a gather over bricks and fields.
@@ -1881,12 +2027,7 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, cons
       rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
     }
   }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
-  st9(S.F, p, Fm);
-  m3 stress = stress_of(P, S, p, Fm, Fp);
-  st9(S.stress, p, stress);
-  return stress;
+  return finish_update(P, S, p, rate);
 }
 
 // The deferred F update of a substep, run by `blocks` workgroups (this is workgroup `block`
@@ -1939,13 +2080,27 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
     deferred_update<false>(P, S, D, int(blockIdx.x), int(gridDim.x));
     return;
   }
-  int CS = 8 * cells_per_brick(P);
-  int slot = int(threadIdx.x);
-  if (slot >= CS) return;
+  int cpb = cells_per_brick(P), CS = 8 * cpb;
+  int lane = int(threadIdx.x & 63);
+  int c = lane_cell(int(threadIdx.x)), a_mine = lane_slot(int(threadIdx.x));
+  const f3 sg = gradient_signs(lane, 1.0f / float(P.spacing));
+  // Every lane stays for the whole loop: the separable path's lane-pair stages need the
+  // whole wavefront.
   for (int k = int(blockIdx.x); k < nbricks; k += int(gridDim.x)) {
     int b = active_brick(S, brick_of_block(k, nbricks));
-    int64_t p = S.perm[int64_t(b) * CS + slot];
-    if (p >= 0) update_particle<false>(P, S, D, p, ld3(S.local, p), reinterpret_cast<const uint2*>(S.field_idx)[p]);
+    int64_t p = c < cpb ? int64_t(S.perm[int64_t(b) * CS + 8 * c + a_mine]) : -1;
+    bool live = p >= 0;
+    uint2 fidx = live ? reinterpret_cast<const uint2*>(S.field_idx)[p] : make_uint2(0u, 0u);
+    const uint64_t live_mask = __ballot(live);
+    if (live_mask == 0) continue;
+    if (P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0) {
+      f3 vc = f3{0.f, 0.f, 0.f};
+      if (cell_live(live_mask, lane)) vc = ld3(S.grid_vel, cell_corner_node(P, window_of(P, D, b), D, c, a_mine));
+      m3 rate = separable_rate(vc, sg);
+      if (live) finish_update(P, S, p, rate);
+    } else if (live) {
+      update_particle<false>(P, S, D, p, ld3(S.local, p), fidx);
+    }
   }
 }
 
@@ -2491,7 +2646,8 @@ static int update_blocks(int threads) { return (kUpdateThreads / threads) & ~7; 
 // Threads of a brick workgroup: one per particle slot (8 per cell) and at least one per window
 // node, in whole waves (at most 1024 by brick_args_ok and the LDS bound on the window).
 static int brick_threads(const bm_params* p) {
-  int64_t slots = 8 * p->brick[0] * p->brick[1] * p->brick[2];
+  // Lanes map to slots two cells per row of 16 (lane_cell, lane_slot): whole rows.
+  int64_t slots = 16 * ((p->brick[0] * p->brick[1] * p->brick[2] + 1) / 2);
   int64_t t = slots > window_nodes(p) ? slots : window_nodes(p);
   t = ((t + 63) / 64) * 64;
   return int(t > 1024 ? 1024 : t);
