@@ -25,20 +25,20 @@ sys.path.insert(0, os.path.join(ROOT, "blademaster-lab_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per particle of each kernel: the per-particle arrays of the substep state
-# (x, v, F, Fp, stress, codes, friction histories, the stencil) that the kernel must read or
-# write, each array counted once per kernel.  The stencil is this design's base node (int32)
-# and three local cell fractions (16 B): the kernels rebuild weights and gradients from it and
-# never read the reference's 160 B of stored slots, weights and gradients.  Fp is not read
-# while every Fp is the identity (no plastic flow; a device flag), which holds for this
-# workload.  Intermediates between kernels (contact impulses, grid partials) are not charged.
+# (x, v, F, Fp, stress, codes, friction histories) that the kernel must read or write, each
+# array counted once per kernel.  On this regular lattice the stencil is implied by a
+# particle's cell slot, so no kernel reads stencil arrays; Fp is not read while every Fp is the
+# identity (no plastic flow; a device flag).  Intermediates between kernels (contact
+# impulses, grid partials, the particle bins and field slots) are not charged.
 #   k_rigid       x, v read 24; friction histories read 24 + written 24            = 72
 #   k_pairs       codes read 8                                                      = 8
-#   k_p2g         v 12, stress 36 and the stencil 16 read                           = 64
-#   k_g2p         x, v read 24 + written 24; stencil 16; codes read 8               = 72
-#   k_update_F    F read 36 + written 36; stress written 36; stencil 16             = 124
-#   k_p2g_update  k_p2g with the previous substep's F update inside it: v 12, stencil 16,
-#                 F read + written 72, stress written 36 (not read back)            = 136
-ALG_BYTES = {"k_rigid": 72, "k_pairs": 8, "k_p2g": 64, "k_g2p": 72, "k_update_F": 124, "k_p2g_update": 136}
+#   k_p2g         v read 12; stress read 36                                         = 48
+#   k_g2p         x, v read 24 + written 24; codes read 8                           = 56
+#   k_update_F    F read 36 + written 36; stress written 36                         = 108
+#   k_p2g_update  k_p2g with the previous substep's F update inside it: v read 12, F read 36 +
+#                 written 36; the stress it computes is used at once, neither read nor
+#                 written                                                           = 84
+ALG_BYTES = {"k_rigid": 72, "k_pairs": 8, "k_p2g": 48, "k_g2p": 56, "k_update_F": 108, "k_p2g_update": 84}
 # Per particle-substep in the reference's layout (x, v, F, Fp, stress, codes, histories and the
 # stored stencil): the substep-level figure, hbm_peak_frac.
 SUBSTEP_BYTES = 444

@@ -1306,10 +1306,11 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // particle's F and stress (update_F_plastic, material.py:445-475) and uses that stress; on
 // the separable path the corner velocities of that update are one gather per lane (the lane's
 // corner) and the stencil's gather stages.
-__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate);
+__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate,
+                                   bool store_stress);
 template <bool kRemap>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
-                                     uint2 fidx);
+                                     uint2 fidx, bool store_stress);
 
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
@@ -1357,7 +1358,6 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
     if (live) {
       vp = ld3(S.v, p);
       imp = ld3(S.impulses, p);
-      lp = ld3(S.local, p);
       fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
       // pending 2: the particles marked in update_done were updated at the end of the last
       // substep; their marks are cleared here.
@@ -1365,7 +1365,9 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
       update = pending && !done;
       if (done) S.update_done[p] = 0;
     }
+    // The separable path needs no local coordinates: the slot gives them.
     const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0;
+    if (live && !separable) lp = ld3(S.local, p);
     const f3 sg = gradient_signs(lane, inv_h);
     if (separable) {
       if (__ballot(update) != 0) {
@@ -1374,10 +1376,10 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
         f3 vc = f3{0.f, 0.f, 0.f};
         if (cell_live(live_mask, lane)) vc = ld3(S.grid_vel, cell_corner_node(P, w, D, c, a_mine));
         m3 rate = separable_rate(vc, sg);
-        if (update) Pk = finish_update(P, S, p, rate);
+        if (update) Pk = finish_update(P, S, p, rate, false);
       }
     } else if (update) {
-      Pk = update_particle<false>(P, S, D, p, lp, fidx);
+      Pk = update_particle<false>(P, S, D, p, lp, fidx, false);
     }
     if (live) {
       if (!update) Pk = ld9(S.stress, p);
@@ -1641,14 +1643,16 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
   uint2 fidx = make_uint2(0u, 0u);
   int32_t base = 0;
   if (live) {
-    lp = ld3(S.local, p);
     fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
     xp = ld3(S.x, p);
     vold = ld3(S.v, p);
-    base = S.base_node[p];
   }
   const uint64_t live_mask = __ballot(live);
   const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0;
+  if (live && !separable) {
+    lp = ld3(S.local, p);
+    base = S.base_node[p];
+  }
   // The separable path's corner velocities: the lane's corner of its cell, field slot 0.
   f3 o_c = f3{0.f, 0.f, 0.f}, n_c = o_c;
   if (separable && cell_live(live_mask, lane)) {
@@ -1967,9 +1971,13 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp, mu, lam);
 }
 
-// F += dt rate for particle p, then the stress; stores F (and Fp after plastic flow) and the
-// stress, and returns the stress.
-__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate) {
+// F += dt rate for particle p, then the stress; stores F (and Fp after plastic flow) and,
+// with store_stress, the stress, and returns the stress.  P2G uses the stress of the particles
+// it updates at once and does not store it: the stored stress is read only by a P2G with no
+// update pending, and every path to such a P2G (the F-update kernel, early deferred
+// updates, refresh_stress) stores it.
+__device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate,
+                                   bool store_stress = true) {
   // While no plastic flow has happened and the host has not written Fp, every Fp is I.
   bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
   m3 Fm = ld9(S.F, p), Fp;
@@ -1984,7 +1992,7 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
   st9(S.F, p, Fm);
   m3 stress = stress_of(P, S, p, Fm, Fp);
-  st9(S.stress, p, stress);
+  if (store_stress) st9(S.stress, p, stress);
   return stress;
 }
 
@@ -1995,7 +2003,7 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
 // (and Fp after plastic flow) and the stress, and returns the stress.
 template <bool kRemap>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
-                                     uint2 fidx) {
+                                     uint2 fidx, bool store_stress) {
   int32_t base = S.base_node[p];
   int64_t N = P.node_count;
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
@@ -2027,7 +2035,7 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, cons
       rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
     }
   }
-  return finish_update(P, S, p, rate);
+  return finish_update(P, S, p, rate, store_stress);
 }
 
 // The deferred F update of a substep, run by `blocks` workgroups (this is workgroup `block`
@@ -2046,7 +2054,7 @@ __device__ void deferred_update(const bm_params& P, const bm_state& S, const Div
   const int64_t first = int64_t(block) * blockDim.x + threadIdx.x, stride = int64_t(blocks) * blockDim.x;
   const uint2* fidx = reinterpret_cast<const uint2*>(S.field_idx);
   if (topo & kTopoFull) {
-    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap>(P, S, D, p, ld3(S.local, p), fidx[p]);
+    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap>(P, S, D, p, ld3(S.local, p), fidx[p], true);
     return;
   }
   // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
@@ -2065,7 +2073,7 @@ __device__ void deferred_update(const bm_params& P, const bm_state& S, const Div
     int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
     int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
     if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
-      update_particle<kRemap>(P, S, D, q, ld3(S.local, q), fidx[q]);
+      update_particle<kRemap>(P, S, D, q, ld3(S.local, q), fidx[q], true);
   }
 }
 
@@ -2099,7 +2107,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
       m3 rate = separable_rate(vc, sg);
       if (live) finish_update(P, S, p, rate);
     } else if (live) {
-      update_particle<false>(P, S, D, p, ld3(S.local, p), fidx);
+      update_particle<false>(P, S, D, p, ld3(S.local, p), fidx, true);
     }
   }
 }
