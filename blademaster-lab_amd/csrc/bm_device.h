@@ -302,9 +302,10 @@ __device__ __forceinline__ m3 cofactor(const m3& m) {
 // U diag(2 mu (s - 1) + lam (J - 1) J / s) V^T = 2 mu (f - R) + lam (J - 1) cof(f), where
 // R = U V^T is the rotation of the polar decomposition f = R S and cof(f) = J f^-T.  R comes
 // from Newton's iteration X <- (X + X^-T) / 2 from X = f (Higham 1986), which converges
-// quadratically; it stops once an update changes X by less than 1e-6 (Frobenius), after
-// which the iterate is orthogonal to float resolution.  Returns false when that has not
-// happened within 12 iterations (a nearly singular f): the caller then takes the SVD path.
+// quadratically: an update that changes X by c leaves an error of about c^2 / 2 in the new
+// iterate, so the iteration stops once c <= 1e-4 (Frobenius), when that error is below float
+// resolution.  Returns false when that has not happened within 12 iterations (a nearly
+// singular f): the caller then takes the SVD path.
 __device__ inline bool corotated_polar(const m3& f, float J, float mu, float lam, m3& P) {
   m3 C = cofactor(f);
   m3 X = f, Cx = C;
@@ -321,7 +322,7 @@ __device__ inline bool corotated_polar(const m3& f, float J, float mu, float lam
       change += d * d;
       X.a[k] = y;
     }
-    if (change <= 1.0e-12f) {
+    if (change <= 1.0e-8f) {
       converged = true;
       break;
     }

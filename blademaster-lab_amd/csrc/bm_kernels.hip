@@ -1959,6 +1959,22 @@ __device__ inline m3 elastic_stress(const bm_params& P, const bm_state& S, const
   return corotated(U, sg, V, pinv, mu, lam);
 }
 
+// stress_at with Fp = I (no plastic flow): the corotated stress of F itself.
+__device__ inline m3 elastic_stress_plain(const bm_state& S, const m3& F, float mu, float lam) {
+  float J = mdet(F);
+  if (!(J > 0.0f) || !isfinite(J)) {
+    atomicAdd(S.errors + 1, 1);
+    return mzero();
+  }
+  m3 Pk;
+  if (corotated_polar(F, J, mu, lam, Pk)) return Pk;
+  m3 U, V, I3 = mzero();
+  I3.a[0] = I3.a[4] = I3.a[8] = 1.0f;
+  f3 sg;
+  svd3(F, U, sg, V);
+  return corotated(U, sg, V, I3, mu, lam);
+}
+
 __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p, const m3& F, m3 Fp) {
   float mu, lam;
   lame_of(P, S, p, mu, lam);
@@ -1991,7 +2007,14 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
   st9(S.F, p, Fm);
-  m3 stress = stress_of(P, S, p, Fm, Fp);
+  m3 stress;
+  if (fp_identity && !P.plastic) {
+    float mu, lam;
+    lame_of(P, S, p, mu, lam);
+    stress = elastic_stress_plain(S, Fm, mu, lam);
+  } else {
+    stress = stress_of(P, S, p, Fm, Fp);
+  }
   if (store_stress) st9(S.stress, p, stress);
   return stress;
 }
@@ -2343,8 +2366,8 @@ __global__ void k_accumulate_reaction(bm_state S) { accumulate_reaction_dev(S); 
 // assign (contact.py:996-1039): side digit for a particle behind the edge band or pinched
 // on the board under the edge.
 __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp) {
-  int64_t code = S.codes[p];
-  if ((code / digit) % 3 != 0) return;
+  // The band test below needs only the particle's x and v and the knife, so it comes first;
+  // the code (an int64 load and division) is read only for particles inside the band.
   int inst = instance_of(S, p);
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
   quat q = ldq(S.tool_rot + 4 * inst);
@@ -2363,6 +2386,8 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   f3 edge_here = edge + rotate(q, f3{0.f, local.y, 0.f});
   bool pinched = on_board && edge_here.z <= r && xp.z <= edge_here.z + r;
   if (fabsf(local.x) <= 6.0f * spacing && fabsf(local.y) <= hf.y && (behind || pinched)) {
+    int64_t code = S.codes[p];
+    if ((code / digit) % 3 != 0) return;
     int64_t side = local.x < 0.0f ? 1 : 2;
     S.codes[p] = code + digit * side;
     // A new side code changes the grid fields and the pair candidates: the fields and lists
