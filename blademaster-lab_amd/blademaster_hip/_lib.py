@@ -83,7 +83,7 @@ STATE_FIELDS = [
     ("bc_side", c_f32p), ("bc_normal", c_f32p), ("bc_lever", c_f32p), ("bc_rhs", c_f32p), ("bc_bound", c_f32p),
     ("bc_impulse", c_f32p), ("bc_extrapolated", c_f32p), ("bc_active", c_i32p), ("bc_nodes", c_i32p),
     ("bc_node_mark", c_i32p), ("bc_grid", c_f32p), ("bc_coupled", c_f32p), ("bc_counts", c_i32p),
-    ("bc_sums", c_f32p),
+    ("bc_sums", c_f32p), ("history_mask", ctypes.POINTER(ctypes.c_uint64)),
 ]
 
 

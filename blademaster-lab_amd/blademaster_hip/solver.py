@@ -139,6 +139,8 @@ class Solver:
         T["field_idx"] = t((n, 8), torch.uint8)
         T["impulses"] = t((n, 3))
         T["histories"] = t((2, n, 3))
+        # Rows of 64-bit words, one bit per particle: its friction history may be nonzero.
+        T["history_mask"] = t((2, max(1, -(-n // 64))), i64)
         F_, N_ = self.fields, self.node_count
         T["field_code"] = t((F_, N_), i64, -1)
         T["node_nf"] = t((N_,), i32)
@@ -252,7 +254,9 @@ class Solver:
         self.single_field_codes = arr(T["single_codes"], mark_topology)
         self.damage = arr(T["damage"])
         self.tool_damage = [self.damage]
-        self.impulses, self.histories = arr(T["impulses"]), arr(T["histories"])
+        self.impulses = arr(T["impulses"])
+        # Host writes to the histories: every particle's history may now be nonzero.
+        self.histories = arr(T["histories"], lambda: T["history_mask"].fill_(-1))
         self.tool_pos, self.tool_vel = arr(T["tool_pos"]), arr(T["tool_vel"])
         self.tool_rot, self.tool_omega = arr(T["tool_rot"]), arr(T["tool_omega"])
         self.tool_mobility = arr(T["tool_mobility"])

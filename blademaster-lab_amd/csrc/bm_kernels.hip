@@ -350,6 +350,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
       S.node_nf[node] = 0;
     }
   }
+  f3 total = f3{0.f, 0.f, 0.f};
+  // Friction histories that may be nonzero (bit p & 63 of the wavefront's word per row): a zero
+  // history is written only over one of them.
+  const int64_t mask_words = (P.n + 63) >> 6;
+  uint64_t may0 = ~0ull, may1 = ~0ull;
+  if (S.history_mask != nullptr && live) {
+    may0 = S.history_mask[p >> 6];
+    may1 = S.history_mask[mask_words + (p >> 6)];
+  }
+  const bool nonzero0 = (may0 >> (p & 63)) & 1ull, nonzero1 = (may1 >> (p & 63)) & 1ull;
+  bool board_contact = false, blade_contact = false;
   if (live) {
     f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
     spd = len(vp);
@@ -357,27 +368,47 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
       ListRegion L = list_region(P, S, inst);
       if (L.build) bin_particle(P, S, p, xp, L);
     }
-    f3 total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
-    // board: plane z = 0 inside its rectangle
+    total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
+    // board: plane z = 0 inside its rectangle; penalty_impulse is zero (history zero) unless
+    // the predicted gap closes (contact.py:123-147)
     float gap = xp.z - float(P.radius);
     if (P.board_half[0] > 0.0 && fabsf(xp.x) > float(P.board_half[0])) gap = 1.0e30f;
     if (P.board_half[1] > 0.0 && fabsf(xp.y) > float(P.board_half[1])) gap = 1.0e30f;
-    f3 jb, hb;
-    penalty(c, gap, vp, f3{0.f, 0.f, 1.f}, float(P.mass), ld3(S.histories, p), float(P.board_sn), jb, hb);
-    st3(S.histories, p, hb);
-    if (dot(jb, jb) > 0.0f) {
-      total = total + jb;
-      board_j = jb;
+    if (gap + dt * fminf(vp.z, 0.0f) < 0.0f) {
+      board_contact = true;
+      f3 jb, hb;
+      penalty(c, gap, vp, f3{0.f, 0.f, 1.f}, float(P.mass), ld3(S.histories, p), float(P.board_sn), jb, hb);
+      st3(S.histories, p, hb);
+      if (dot(jb, jb) > 0.0f) {
+        total = total + jb;
+        board_j = jb;
+      }
+    } else if (nonzero0) {
+      st3(S.histories, p, f3{0.f, 0.f, 0.f});
     }
     // blade (blade_at, contact.py:235-329)
     float blade_radius = len(hf);
     float max_closing = spd + len(tvel) + len(tom) * blade_radius;
     float offset = float(P.blade_offset);
     int64_t hrow = P.n + p;
-    if (len(xp - origin) - blade_radius - offset > dt * max_closing) {
-      st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
+    f3 local = f3{0.f, 0.f, 0.f};
+    bool near = len(xp - origin) - blade_radius - offset <= dt * max_closing;
+    if (near) {
+      // The distance to the box without the bevel bounds the blade's from below (the bevel
+      // only removes material), and |relative . n| <= max_closing: a particle whose box
+      // distance stays beyond the offset over the step cannot touch the blade.  The column
+      // under the edge of a held cut keeps the exact query (its distance is the edge's).
+      local = rotate_inv(q, xp - origin);
+      f3 out = f3{fmaxf(fabsf(local.x) - hf.x, 0.f), fmaxf(fabsf(local.y) - hf.y, 0.f),
+                  fmaxf(fabsf(local.z) - hf.z, 0.f)};
+      float box = len(out);
+      bool hold_column = (cut_bits & BM_CUT_HOLD) && fabsf(local.x) <= hf.x + offset && fabsf(local.y) <= hf.y &&
+                         -hf.z - local.z >= 0.0f;
+      if (!hold_column && box - offset > dt * max_closing + 1.0e-6f * (box + blade_radius)) near = false;
+    }
+    if (!near) {
+      if (nonzero1) st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
     } else {
-      f3 local = rotate_inv(q, xp - origin);
       f3 closest, normal;
       float distance;
       blade_query(local, hf, float(P.bevel), closest, normal, distance);
@@ -393,8 +424,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
       f3 n = rotate(q, normal);
       f3 relative = vp - tvel - cross(tom, lever);
       if (distance - offset + dt * fminf(dot(relative, n), 0.0f) >= 0.0f) {
-        st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
+        if (nonzero1) st3(S.histories, hrow, f3{0.f, 0.f, 0.f});
       } else {
+        blade_contact = true;
         float me = effective_mass(P, S.mobility + 36 * inst, lever, n, q);
         f3 j, h;
         penalty(c, distance - offset, relative, n, me, ld3(S.histories, hrow), float(P.sn), j, h);
@@ -428,8 +460,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
         }
       }
     }
-    st3(S.impulses, p, total);
   }
+  // This step's contacts are the histories that may be nonzero next step.
+  const uint64_t now0 = __ballot(board_contact), now1 = __ballot(blade_contact);
+  if (S.history_mask != nullptr && lane0() && p < P.n) {
+    S.history_mask[p >> 6] = now0;
+    S.history_mask[mask_words + (p >> 6)] = now1;
+  }
+  if (live) st3(S.impulses, p, total);
   // Batched instances: the step sums go to each instance's own slots.  The block's first
   // particle names the instance of the block reduction below; a wave with a live particle of
   // another instance (at most the waves where instances meet, since they are stored one after
@@ -2373,12 +2411,17 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   quat q = ldq(S.tool_rot + 4 * inst);
   f3 pos = ld3(S.tool_pos, inst);
   f3 edge = pos + rotate(q, f3{0.f, 0.f, -hf.z});
+  float spacing = float(P.spacing);
+  // The band is a slab |local.x| <= 6 h about the knife plane: its x coordinate alone, from
+  // the knife's x axis, sends most particles away.  The margin leaves every particle near the
+  // slab faces to the exact test below.
+  if (fabsf(dot(xp - edge, rotate(q, f3{1.f, 0.f, 0.f}))) > 6.0f * spacing * 1.0001f + 1.0e-6f) return;
   f3 local = rotate_inv(q, xp - edge);
   float front = -local.z;
   f3 edge_v = ld3(S.tool_vel, inst) + cross(ld3(S.tool_omega, inst), edge - pos);
   f3 d = rotate(q, f3{0.f, 0.f, -1.f});
   float advance = dot(edge_v - vp, d);
-  float spacing = float(P.spacing), r = float(P.radius), dt = float(P.dt);
+  float r = float(P.radius), dt = float(P.dt);
   float back = 2.0f * spacing + fmaxf(advance * dt, 0.0f);
   bool behind = front >= -back && front <= 0.0f;
   bool on_board = (P.board_half[0] == 0.0 || fabsf(xp.x) <= float(P.board_half[0])) &&

@@ -148,6 +148,10 @@ typedef struct bm_state {
   float *bc_coupled;       /* [F][N][3] */
   int32_t *bc_counts;      /* [4] active particles, active nodes, needs_solve, batches */
   float *bc_sums;          /* [8] linear 3, angular 3, max residual of the run, last residual */
+  /* [2][ceil(n / 64)]: bit (p & 63) of word p >> 6 of row r is set when friction history row
+   * r of particle p may be nonzero.  The contact kernel writes a zero history only over a
+   * set bit; host writes to the histories must set every bit.  Null: always write. */
+  uint64_t *history_mask;
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */

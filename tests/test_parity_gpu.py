@@ -173,6 +173,40 @@ def test_short_run_tracks_oracle(mode):
     assert worst["v"] < 1e-4, worst
 
 
+def test_friction_histories_track_contact_on_and_off():
+    """Friction history rows (board, blade) against the oracle while board contacts begin and
+    blade contacts end (the knife is lifted away after four substeps): a particle without
+    contact holds an exactly zero history, including one whose contact just ended (the contact
+    kernel skips rewriting histories known to be zero)."""
+    cfg = small_cfg(cutting_energy=0.05)
+    s = Solver(cfg)
+    blade_touching(s, 0.0015)
+    s.v.assign(np.tile([0.0, 0.0, -0.02], (s.n, 1)).astype(np.float32))
+    o = oracle_for(s)
+    lifted = [0.0, 0.0, cfg.size[2] + cfg.blade_half_size[2] + 0.05]
+    began = ended = 0
+    before = None
+    for step in range(8):
+        if step == 4:
+            s.tool_pos.assign([lifted])
+            o.set("tool_pos", lifted)
+        target = (0.0, 0.0, cfg.size[2] - 0.002) if step < 4 else (0.0, 0.0, cfg.size[2] + 0.05)
+        s.step(target, cutting=False)
+        o.substep(target, cutting=False)
+        h, ho = s.histories.numpy().reshape(2, -1, 3), o.get("histories").reshape(2, -1, 3)
+        zero = np.all(ho == 0.0, axis=2)
+        assert np.all(h[zero] == 0.0), ("history left nonzero", step)
+        for row in range(2):
+            scale = max(np.abs(ho[row]).max(), 1e-30)
+            assert np.abs(h[row] - ho[row]).max() <= 1e-4 * scale, (row, step)
+        if before is not None:
+            began += int((before & ~zero).sum())
+            ended += int((~before & zero).sum())
+        before = zero
+        sync_to_oracle(s, o)
+    assert began > 0 and ended > 0
+
+
 def test_graph_replay_matches_eager():
     cfg = small_cfg(yield_stress=2000.0)
     eager, graph = Solver(cfg), Solver(cfg)
