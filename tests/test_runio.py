@@ -66,3 +66,39 @@ def test_edge_height_closed_forms():
     expected = 0.1 - half[2] * math.cos(a) - half[1] * math.sin(a)
     assert abs(low[0, 0] - expected) < 1e-12
     assert abs(tilt[0, 0] - 25.0) < 1e-9
+
+
+def test_drop_scenes_release_pose_and_command_values():
+    """The knife drops of blademaster_hip.freefall: the release height of the tilted knife is
+    the reference's closed form (paper_freefall.py:64-76: h_z cos a + h_y sin a about the
+    thickness axis, h_z cos a + h_x sin a about the length axis, above the clearance); the
+    8960-particle scene has its particle count; --set values parse as numbers and tuples."""
+    from dataclasses import replace
+
+    from blademaster_hip import freefall
+    from blademaster_hip.geometry import points
+
+    fig3 = freefall.SCENES["fig3"]
+    hx, hy, hz = fig3.knife_half_size
+    a = math.radians(25.0)
+    q, h = freefall.release_pose(fig3, 0.08)
+    np.testing.assert_allclose(q, [math.sin(a / 2), 0, 0, math.cos(a / 2)], atol=1e-15)
+    assert abs(h - (0.08 + 0.015 + hz * math.cos(a) + hy * math.sin(a))) < 1e-15
+    q, h = freefall.release_pose(replace(fig3, tilt_axis="length"), 0.08)
+    np.testing.assert_allclose(q, [0, math.sin(a / 2), 0, math.cos(a / 2)], atol=1e-15)
+    assert abs(h - (0.08 + 0.015 + hz * math.cos(a) + hx * math.sin(a))) < 1e-15
+    with pytest.raises(ValueError):
+        freefall.release_pose(replace(fig3, tilt_axis="edge"), 0.08)
+    knife = freefall.SCENES["gravity-knife"]
+    cfg = knife.config()
+    ref, _, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    assert len(ref) == 8960 and tuple(cells + 1) == (29, 11, 5)
+    assert knife.steps() == 8000 and cfg.free_blade
+    q, h = freefall.release_pose(knife, cfg.size[2])
+    np.testing.assert_allclose(q, [0, 0, 0, 1])
+    assert abs(h - (cfg.size[2] + 0.002 + cfg.blade_half_size[2])) < 1e-15
+    assert freefall._value("0.2") == 0.2 and freefall._value("3") == 3
+    assert freefall._value("0.06,0.04,0.03") == (0.06, 0.04, 0.03)
+    assert freefall._value("true") is True and freefall._value("length") == "length"
+    with pytest.raises(ValueError):
+        replace(knife, duration=0.00001).steps()
