@@ -97,34 +97,62 @@ def batch_cut_traces(spacing, seeds, x_offset_fractions=(0.0,), blade_contact_ra
     return runs
 
 
+def _mean_std(values):
+    """Mean and sample standard deviation (0 for a single value)."""
+    values = np.asarray(values, dtype=float)
+    return dict(mean=float(values.mean()), std=float(values.std(ddof=1)) if values.size > 1 else 0.0)
+
+
+class _ForceTable:
+    """Vertical knife force of the runs of one spacing, one row per run and one column per
+    coupling interval (force at the end of interval k + 1, t = (k + 1) INTERVAL)."""
+
+    def __init__(self, runs):
+        self.runs = runs
+        self.fz = np.stack([np.asarray(r["force_n"], dtype=float)[:, 2] for r in runs])
+        self.t = INTERVAL * np.arange(1, self.fz.shape[1] + 1)
+
+    def window_mean(self, start, stop):
+        """Per run: the mean force over start < t <= stop."""
+        return self.fz[:, (self.t > start) & (self.t <= stop)].mean(axis=1)
+
+    def impulse_until(self, stop):
+        """Per run: the impulse of the intervals ending at t <= stop."""
+        return INTERVAL * self.fz[:, self.t <= stop].sum(axis=1)
+
+    def bins(self, width_s):
+        """Per run: the force averaged over whole bins of width_s (a partial last bin dropped)."""
+        per = int(round(width_s / INTERVAL))
+        whole = self.fz.shape[1] // per
+        return self.fz[:, : whole * per].reshape(len(self.runs), whole, per).mean(axis=2)
+
+
 def summarize(runs, window=(0.4, 1.4), bin_s=0.02):
-    """Per spacing: mean/std over sampling seeds and offsets of force and impulse
-    (force_convergence.py:132-164)."""
-    out = {}
-    for spacing in sorted({r["spacing"] for r in runs}, reverse=True):
-        group = [r for r in runs if r["spacing"] == spacing]
-        fz = np.array([np.asarray(r["force_n"])[:, 2] for r in group])
-        per_bin = round(bin_s / INTERVAL)
-        binned = fz[:, : fz.shape[1] // per_bin * per_bin].reshape(len(group), -1, per_bin).mean(2)
-        t = (np.arange(fz.shape[1]) + 1) * INTERVAL
-        inside = (t > window[0]) & (t <= window[1])
-        steady = fz[:, inside].mean(1)
-        impulse = fz[:, t <= DESCENT_S].sum(1) * INTERVAL
-        out[f"{spacing * 1000:g} mm"] = dict(
-            particles=group[0]["particles"],
-            dt_us=group[0]["dt"] * 1.0e6,
-            seeds=len({r["seed"] for r in group}),
-            samples=len(group),
-            x_offsets_m=sorted({r.get("x_offset_m", 0.0) for r in group}),
-            steady_force_n=dict(mean=float(steady.mean()), std=float(steady.std(ddof=1)) if len(group) > 1 else 0.0),
-            descent_impulse_ns=dict(mean=float(impulse.mean()),
-                                    std=float(impulse.std(ddof=1)) if len(group) > 1 else 0.0),
-            peak_single_run_n=dict(mean=float(fz.max(1).mean()), max=float(fz.max())),
-            peak_of_mean_trace_n=float(binned.mean(0).max()),
-            severed_runs=int(sum(len(r["pieces"]) >= 2 for r in group)),
-            wall_s_per_run=float(np.mean([r["wall_s"] for r in group])),
-            gpu_errors=int(sum(sum(r["gpu_errors"]) for r in group)),
-            mean_trace_n=binned.mean(0).tolist(),
-            std_trace_n=binned.std(0, ddof=1).tolist() if len(group) > 1 else [0.0] * binned.shape[1],
-        )
-    return dict(window_s=list(window), bin_s=bin_s, by_spacing=out)
+    """Statistics over sampling seeds and lateral offsets of each spacing's cut traces: the
+    fields of scripts/force_convergence.py:132-164, with the same meaning."""
+    groups = {}
+    for r in runs:
+        groups.setdefault(r["spacing"], []).append(r)
+    by_spacing = {}
+    for spacing in sorted(groups, reverse=True):
+        rows = groups[spacing]
+        table = _ForceTable(rows)
+        binned = table.bins(bin_s)
+        trace_std = binned.std(axis=0, ddof=1) if len(rows) > 1 else np.zeros(binned.shape[1])
+        by_spacing[f"{spacing * 1000:g} mm"] = {
+            "particles": rows[0]["particles"],
+            "dt_us": rows[0]["dt"] * 1.0e6,
+            "seeds": len({r["seed"] for r in rows}),
+            "samples": len(rows),
+            "x_offsets_m": sorted({r.get("x_offset_m", 0.0) for r in rows}),
+            "steady_force_n": _mean_std(table.window_mean(*window)),
+            "descent_impulse_ns": _mean_std(table.impulse_until(DESCENT_S)),
+            "peak_single_run_n": {"mean": float(table.fz.max(axis=1).mean()), "max": float(table.fz.max())},
+            "peak_of_mean_trace_n": float(binned.mean(axis=0).max()),
+            "severed_runs": sum(1 for r in rows if len(r["pieces"]) > 1),
+            "wall_s_per_run": float(np.mean([r["wall_s"] for r in rows])),
+            "gpu_errors": int(sum(np.sum(r["gpu_errors"]) for r in rows)),
+            "mean_trace_n": binned.mean(axis=0).tolist(),
+            "std_trace_n": trace_std.tolist(),
+        }
+    return {"window_s": list(window), "bin_s": bin_s, "by_spacing": by_spacing}

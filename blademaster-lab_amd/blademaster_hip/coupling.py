@@ -44,11 +44,16 @@ def vector(value, length, name):
 
 @dataclass(frozen=True)
 class ToolState:
-    position: object  # Tool origin; must match the collision geometry origin.
+    # The knife frame's origin, world metres: the centre of the blade box the contact kernel
+    # tests against, not a point elsewhere on the tool.
+    position: object
     orientation_xyzw: object
-    linear_velocity: object  # Velocity at that same origin.
+    # Velocity of that frame origin, and the angular velocity, world frame.
+    linear_velocity: object
     angular_velocity: object
-    mobility: object = None  # World [linear, angular] at the tool origin, 6x6.
+    # 6 x 6 world-frame mobility at the frame origin, rows and columns (linear, angular);
+    # None for a prescribed (infinitely heavy) knife.
+    mobility: object = None
 
     def checked(self):
         """Validated copy (coupling.py:31-50)."""
@@ -179,7 +184,7 @@ class ExternalToolPort:
         sample = None
         if self.record_trajectory:
             def sample(i):
-                # In hold mode every intermediate pose is identical.
+                # A held knife does not move inside the interval: its start and end poses only.
                 if not held_pose or i == steps - 1:
                     record(1 if held_pose else i + 1)
 

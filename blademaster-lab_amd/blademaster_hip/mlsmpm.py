@@ -209,11 +209,17 @@ class MlsSolver:
         self.damage_reference.assign(np.linalg.det(self.F.numpy()))
 
     def diagnostics(self):
-        """Particles non-finite or outside the grid, and clamped inversions (mlsmpm.py:432-437)."""
-        local = (self.x.numpy() - self.origin) / self.config.grid_spacing
-        base = np.floor(local - 0.5)
-        outside = ~np.all(np.isfinite(base) & (base >= 0) & (base + 2 < self.dims), axis=1)
-        return dict(left_domain=int(outside.sum()), clamped_inversions=int(self.inversions.numpy()[0]))
+        """left_domain: particles with a non-finite position or a quadratic stencil (nodes
+        floor(u - 1/2) .. floor(u - 1/2) + 2 per axis, u the position in cells) not inside the
+        grid; clamped_inversions: singular values clamped so far (the counts of
+        mlsmpm.py:432-437)."""
+        u = (self.x.numpy().astype(np.float64) - self.origin) / self.config.grid_spacing
+        with np.errstate(invalid="ignore"):
+            first = np.floor(u - 0.5)
+            inside = (first >= 0) & (first + 2 <= np.asarray(self.dims) - 1)
+        valid = np.isfinite(u).all(axis=1) & inside.all(axis=1)
+        return {"left_domain": int(np.count_nonzero(~valid)),
+                "clamped_inversions": int(self.inversions.numpy()[0])}
 
     def state(self):
         return dict(
