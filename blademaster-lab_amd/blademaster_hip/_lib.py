@@ -180,6 +180,9 @@ SIGNATURES = {
     "bm_probe_blade_query": [VP, ctypes.POINTER(ctypes.c_float), F32, VP, I32, VP],
     "bm_probe_friction": [VP, F32, F32, F32, F32, F32, VP, I32, VP],
     "bm_probe_svd3": [VP, VP, VP, VP, I32, VP],
+    "bm_probe_resist_cut": [VP, VP, VP, VP, VP, I32, VP, VP, I32, VP],
+    "bm_couple": [VP, VP, VP, VP, VP, VP, I64, I64, VP],
+    "bm_g2p": [VP, VP, VP, VP, VP, VP, F32, F32, I64, I64, VP],
     "bm_probe_stress": [VP, F32, F32, VP, VP, I32, VP],
     "bm_blade_constraint": [P, S, VP],
     "bm_store_residual": [P, S, VP, I32, VP],

@@ -305,6 +305,33 @@ constexpr int kTopoFull = 1;
 constexpr int kTopoChanged = 2;
 constexpr int kInstSlots = 16;  // slots per instance of the batched contact sums (inst_sums)
 
+// resist_cut (contact.py:182-232): cutting resistance of a particle (position xp, velocity vp)
+// just ahead of the knife edge (knife origin, rotation q, half sizes hf, velocity tvel, angular
+// velocity tom).  Returns the blade impulse j, less its push along the cutting direction once
+// the particle is fully damaged.  While it is intact and the edge advances on it, the push
+// does work that adds to its damage; the new damage and the work done come back in `damage`
+// and `work` (unchanged when the particle is not ahead of the edge).  radius: the blade
+// contact offset; threshold: the work that damages one particle fully.
+__device__ inline f3 resist_cut(f3 j, f3 xp, f3 vp, f3 origin, f3 tvel, quat q, f3 tom, f3 hf, float radius,
+                                float spacing, float threshold, int cut_bits, float& damage, float& work) {
+  f3 edge = origin + rotate(q, f3{0.f, 0.f, -hf.z});
+  f3 down = rotate(q, f3{0.f, 0.f, -1.f});
+  f3 edge_velocity = tvel + cross(tom, edge - origin);
+  float front = dot(xp - edge, down);
+  float advance = (cut_bits & BM_CUT_ADVANCE) ? dot(edge_velocity, down) : dot(edge_velocity - vp, down);
+  f3 le = rotate_inv(q, xp - edge);
+  bool ahead = front >= 0.0f && front <= 2.0f * radius && fabsf(le.x) <= 6.0f * spacing && fabsf(le.y) <= hf.y;
+  if (!ahead) return j;
+  float push = fmaxf(dot(j, down), 0.0f);
+  if (damage >= 1.0f) return j - push * down;
+  if (advance > 0.0f) {
+    float delta = push * advance;
+    work = delta;
+    damage = fminf(1.0f, damage + delta / threshold);
+  }
+  return j;
+}
+
 // rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
 // Per-block sums go to rigid_partial; k_prep reduces them.  In a substep the same pass also
 // empties the pair slots of the new table, bins particles for a pair-list rebuild and clears
@@ -433,26 +460,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
         st3(S.histories, hrow, h);
         if (dot(j, j) > 0.0f) {
           if (cut_bits != 0) {
-            // resist_cut (contact.py:181-232)
-            f3 edge = origin + rotate(q, f3{0.f, 0.f, -hf.z});
-            f3 down = rotate(q, f3{0.f, 0.f, -1.f});
-            f3 edge_velocity = tvel + cross(tom, edge - origin);
-            float fr = dot(xp - edge, down);
-            float advance = dot(edge_velocity - vp, down);
-            if (cut_bits & BM_CUT_ADVANCE) advance = dot(edge_velocity, down);
-            f3 le = rotate_inv(q, xp - edge);
-            bool ahead = fr >= 0.0f && fr <= 2.0f * offset && fabsf(le.x) <= 6.0f * float(P.spacing) &&
-                         fabsf(le.y) <= hf.y;
-            if (ahead) {
-              float push = fmaxf(dot(j, down), 0.0f);
-              if (dmg >= 1.0f) {
-                j = j - push * down;
-              } else if (advance > 0.0f) {
-                float delta = push * advance;
-                work = delta;
-                S.damage[p] = fminf(1.0f, dmg + delta / float(P.threshold));
-              }
-            }
+            float d = dmg;
+            j = resist_cut(j, xp, vp, origin, tvel, q, tom, hf, offset, float(P.spacing), float(P.threshold), cut_bits,
+                           d, work);
+            if (d != dmg) S.damage[p] = d;
           }
           total = total + j;
           react = -j;
@@ -2632,6 +2643,72 @@ bool state_ok(const bm_params* p, const bm_state* s) {
 }  // namespace
 
 // ======================================================================== C-ABI
+// Test probe of resist_cut: lane i is particle (x, v, damage)[i] under the knife edge moving
+// at tool_velocity[i]; c holds the impulse j (3), knife origin (3), rotation xyzw (4),
+// angular velocity (3), half sizes (3), contact radius, grid spacing and the damage work.
+struct CutProbe {
+  float a[19];
+};
+__global__ void k_probe_resist_cut(const float* x, const float* v, const float* tool_velocity, float* damage,
+                                   CutProbe c, int cut_bits, float* out, float* work, int count) {
+  int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= count) return;
+  const float* a = c.a;
+  float d = damage[i], w = 0.0f;
+  f3 r = resist_cut(f3{a[0], a[1], a[2]}, ld3(x, i), ld3(v, i), f3{a[3], a[4], a[5]}, ld3(tool_velocity, i),
+                    quat{a[6], a[7], a[8], a[9]}, f3{a[10], a[11], a[12]}, f3{a[13], a[14], a[15]}, a[16], a[17],
+                    a[18], cut_bits, d, w);
+  damage[i] = d;
+  st3(out, i, r);
+  if (w != 0.0f) atomicAdd(work, w);
+}
+
+// couple (material.py:155-209) on caller tables: field a has key code * node_count + node
+// (-1 unused), mass and a state vector; heads[node] starts the node's linked list of fields
+// (links[field], -1 ends it).  result[a] = state[a] + sum over compatible b != a of
+// (m_a state_b - m_b state_a) / (sum of every field's mass at the node); fields with a negative
+// key or no mass are left as they are in result.  A list longer than count is cut there.
+__global__ void k_couple_keys(const int64_t* keys, const int32_t* heads, const int32_t* links, const float* mass,
+                              const float* state, float* result, int64_t count, int64_t node_count) {
+  int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= count) return;
+  const int64_t key = keys[a];
+  const float ma = mass[a];
+  if (key < 0 || !(ma > 0.0f)) return;
+  const int64_t node = key % node_count, code = key / node_count;
+  const f3 sa = ld3(state, a);
+  float total = 0.0f;
+  f3 exchange = f3{0.f, 0.f, 0.f};
+  int64_t walked = 0;
+  for (int32_t b = heads[node]; b >= 0 && b < count && walked < count; b = links[b], ++walked) {
+    const float mb = mass[b];
+    total += mb;
+    if (b != a && compatible(code, keys[b] / node_count)) exchange = exchange + (ma * ld3(state, b) - mb * sa);
+  }
+  st3(result, a, sa + exchange / total);
+}
+
+// g2p (material.py:343-384) on caller tables: particle p reads its 8 corners' old and new
+// velocities at slots[8 p + a] with weights[8 p + a]; v = (1 - flip) pic + flip (v + sum w
+// (new - old)) and x += dt pic (TLMPM Eq. 13: positions move with the grid velocity).  A slot
+// outside [0, grid_count) contributes nothing.
+__global__ void k_g2p_slots(float* x, float* v, const int32_t* slots, const float* weights, const float* old_v,
+                            const float* new_v, float dt, float flip, int64_t n, int64_t grid_count) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  f3 pic = f3{0.f, 0.f, 0.f}, dv = pic;
+  for (int a = 0; a < 8; ++a) {
+    int32_t s = slots[8 * p + a];
+    if (s < 0 || s >= grid_count) continue;
+    float w = weights[8 * p + a];
+    f3 nv = ld3(new_v, s);
+    pic = pic + w * nv;
+    dv = dv + w * (nv - ld3(old_v, s));
+  }
+  st3(v, p, (1.0f - flip) * pic + flip * (ld3(v, p) + dv));
+  st3(x, p, ld3(x, p) + dt * pic);
+}
+
 extern "C" {
 
 int bm_version(void) { return 27; }
@@ -2998,6 +3075,37 @@ int bm_probe_stress(const float* F, float mu, float lam, float* out, int32_t* sv
   if (count == 0) return 0;
   hipLaunchKernelGGL(k_probe_stress, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, F, mu, lam, out,
                      svd_path, count);
+  return launch_status();
+}
+
+int bm_probe_resist_cut(const float* x, const float* v, const float* tool_velocity, float* damage,
+                        const float* params19, int32_t cut_bits, float* out, float* work, int32_t count, void* stream) {
+  if (count < 0 || (count > 0 && (!x || !v || !tool_velocity || !damage || !params19 || !out || !work))) return -1;
+  if (count == 0) return 0;
+  CutProbe c;
+  for (int k = 0; k < 19; ++k) c.a[k] = params19[k];
+  hipLaunchKernelGGL(k_probe_resist_cut, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, x, v,
+                     tool_velocity, damage, c, int(cut_bits), out, work, int(count));
+  return launch_status();
+}
+
+int bm_couple(const int64_t* keys, const int32_t* heads, const int32_t* links, const float* mass, const float* state,
+              float* result, int64_t count, int64_t node_count, void* stream) {
+  if (count < 0 || node_count < 1 || count >= (int64_t(1) << 31)) return -1;
+  if (count == 0) return 0;
+  if (!keys || !heads || !links || !mass || !state || !result) return -1;
+  hipLaunchKernelGGL(k_couple_keys, dim3(blocks_for(count)), dim3(kBlock), 0, (hipStream_t)stream, keys, heads, links,
+                     mass, state, result, count, node_count);
+  return launch_status();
+}
+
+int bm_g2p(float* x, float* v, const int32_t* slots, const float* weights, const float* old_v, const float* new_v,
+           float dt, float flip, int64_t n, int64_t grid_count, void* stream) {
+  if (n < 0 || grid_count < 0) return -1;
+  if (n == 0) return 0;
+  if (!x || !v || !slots || !weights || !old_v || !new_v) return -1;
+  hipLaunchKernelGGL(k_g2p_slots, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, x, v, slots, weights,
+                     old_v, new_v, dt, flip, n, grid_count);
   return launch_status();
 }
 

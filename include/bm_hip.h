@@ -263,6 +263,14 @@ int bm_probe_blade_query(const float* local, const float* half, float bevel, flo
  * dt, mus, mud, gk, gc; out[6*count] = impulse 3, history 3. */
 int bm_probe_friction(const float* in, float dt, float mus, float mud, float gk, float gc, float* out,
                       int32_t count, void* stream);
+/* Test probe of the contact kernel's cutting resistance (resist_cut, contact.py:182-232;
+ * the reference's test tests/test_cut_resistance.py:15-42 launches it the same way): lane i
+ * is a particle x[i], v[i] with damage[i] (updated in place) under the knife edge moving at
+ * tool_velocity[i].  params19 = impulse j (3), knife origin (3), rotation xyzw (4), angular
+ * velocity (3), half sizes (3), contact radius, grid spacing, damage work threshold.  out:
+ * the returned impulse per lane; work[0] accumulates the work done (atomically). */
+int bm_probe_resist_cut(const float* x, const float* v, const float* tool_velocity, float* damage,
+                        const float* params19, int32_t cut_bits, float* out, float* work, int32_t count, void* stream);
 /* 3x3 SVD of A[9*count] -> U[9], s[3], V[9] per lane. */
 int bm_probe_svd3(const float* A, float* U, float* s, float* V, int32_t count, void* stream);
 /* Test probe: fixed-corotated first Piola stress of each F (Fp = I) as the F update computes
@@ -294,6 +302,19 @@ typedef struct bm_labels {
  * and angular impulse).  No reference counterpart: the batched driver of force_convergence. */
 int bm_load_external_batch(const bm_params* p, const bm_state* s, const float* packed, void* stream);
 int bm_store_interval_batch(const bm_params* p, const bm_state* s, float* out, void* stream);
+
+/* Replaces the kernel couple (material.py:155-209) on caller tables: keys [count] (code *
+ * node_count + node, -1 unused), heads [node_count] and links [count] (each node's fields as
+ * a linked list, -1 ends), mass [count], state [count][3] -> result [count][3]; fields with a
+ * negative key or no mass are not written.  The reference's test launches it on its own
+ * tables (tests/test_mechanics.py:80-105). */
+int bm_couple(const int64_t* keys, const int32_t* heads, const int32_t* links, const float* mass, const float* state,
+              float* result, int64_t count, int64_t node_count, void* stream);
+/* Replaces the kernel g2p (material.py:343-384) on caller tables: particles x, v [n][3]
+ * (updated in place), slots and weights [n][8] into the grid arrays old_v, new_v
+ * [grid_count][3] (tests/test_particle_transport.py:10-45 launches it this way). */
+int bm_g2p(float* x, float* v, const int32_t* slots, const float* weights, const float* old_v, const float* new_v,
+           float dt, float flip, int64_t n, int64_t grid_count, void* stream);
 
 /* Replaces topology.components (topology.py:17-57) for a solver's current codes. */
 int bm_piece_labels(const bm_params* p, const bm_state* s, const bm_labels* l, void* stream);
