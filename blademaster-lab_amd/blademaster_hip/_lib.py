@@ -39,6 +39,7 @@ class Params(ctypes.Structure):
         ("instances", ctypes.c_int64),
         ("active_bricks", ctypes.c_int64),
         ("active_nodes", ctypes.c_int64),
+        ("pair_mode", ctypes.c_int64),
         ("regular", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)

@@ -21,6 +21,12 @@ from .geometry import points, stencil, compact_stencil, bin_bricks, lattice_slot
 
 # Listed pair partners per particle before the bucket fallback (solver.py:152-154).
 PAIR_LIST_LANES = 16
+# Pair search (solver.py:260-263): Verlet partner lists ("list", "auto"), partner lists rebuilt
+# from the cell buckets every step ("hash"), or every pair tested ("all", O(n^2)).
+PAIR_MODES = {"auto": 0, "list": 0, "hash": 1, "all": 2}
+# "all" is a check of the other searches on small solvers: above this many particles its n^2
+# pair tests per step are refused.
+ALL_PAIRS_LIMIT = 100_000
 CUT_ON, CUT_ADVANCE, CUT_HOLD = 1, 2, 4
 
 
@@ -82,6 +88,8 @@ class Solver:
             self.reference, self.origin, cells = _layout["reference"], _layout["origin"], _layout["cells"]
         self.cells = np.asarray(cells)
         self.n = n = len(self.reference)
+        if cfg.pair_search == "all" and n > ALL_PAIRS_LIMIT:
+            raise ValueError(f"pair_search='all' tests every pair each step: at most {ALL_PAIRS_LIMIT} particles")
         self.node_numpy, self.weight_numpy, self._gradient_numpy = stencil(
             self.reference, self.origin, cfg.grid_spacing, cells
         )
@@ -331,6 +339,7 @@ class Solver:
         p.active_bricks = self.active_bricks
         p.active_nodes = self.active_nodes
         p.regular = int(self.regular)
+        p.pair_mode = PAIR_MODES[cfg.pair_search]
         return p
 
     def _make_state(self):

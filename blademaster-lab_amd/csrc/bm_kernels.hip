@@ -287,10 +287,15 @@ struct ListRegion {
   int base, mask, tag;
   bool build;
 };
+constexpr int kPairList = 0, kPairHash = 1, kPairAll = 2;  // bm_params.pair_mode
+// The search lists of the particles of instance inst; `build` when they are rebuilt this step
+// (always in the hash mode, which queries the cell buckets every step).
 __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_state& S, int inst) {
-  if (S.instance == nullptr) return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0], S.pair_dirty[0] != 0};
+  const bool every_step = P.pair_mode == kPairHash;
+  if (S.instance == nullptr)
+    return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0], every_step || S.pair_dirty[0] != 0};
   return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, S.inst_pair_builds[inst],
-                    S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0};
+                    every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0};
 }
 
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
@@ -595,7 +600,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
   }
   if (blockIdx.x > 0) {
     // Relabelled particles with a current pair list: mark the lists to rebuild this step.
-    if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0) return;
+    if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
     int count = min(S.changed_count[0], int(P.changed_capacity));
     int first = int(blockIdx.x) - 1 - insert_blocks, blocks = int(gridDim.x) - 1 - insert_blocks;
     constexpr int kWaves = kPrepBlock / 64;
@@ -640,7 +645,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     // the shared pair-list scalars.
     int built = 0;
     for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
-      if ((mode & kPairPrep) != 0 && (S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0)) {
+      if ((mode & kPairPrep) != 0 && (P.pair_mode == kPairHash || S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0)) {
         S.inst_pair_builds[i] += 1;
         built = 1;
       }
@@ -697,7 +702,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     speed = fmaxf(speed, S.speed[0]);
   }
   float work = S.work[0] + t[9];
-  int dirty = (mode & kPairPrep) ? S.pair_dirty[0] : 0;
+  int dirty = (mode & kPairPrep) ? (P.pair_mode == kPairHash || S.pair_dirty[0] != 0) : 0;
   int builds = (mode & kPairPrep) ? S.pair_builds[0] : 0;
   bool finish = (mode & kFinishExternal) != 0;
   ExternalTool ext;
@@ -846,9 +851,10 @@ __global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
   if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
   S.list_reach[0] = value;
   S.pair_count[parity_new] = 0;
-  bool built = S.pair_dirty[0] != 0;
+  const bool every_step = P.pair_mode == kPairHash;
+  bool built = every_step || S.pair_dirty[0] != 0;
   for (int i = 0; S.instance != nullptr && i < int(P.instances); ++i) {
-    if (S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0) {
+    if (every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0) {
       S.inst_pair_builds[i] += 1;
       built = true;
     }
@@ -917,6 +923,15 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   const int32_t listed = S.partner_count[p];
   // Registration, second half (the anchors claimed their slots in k_prep).
   if (topology && (S.topology_dirty[0] & kTopoFull) != 0) insert_particle(P, S, p);
+  if (P.pair_mode == kPairAll) {
+    // pairs_all (contact.py:929-993): every pair q > p, no lists.
+    int parity_new = 1 - parity_old;
+    int64_t cap = P.pair_capacity;
+    PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
+              S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, S.pair_count + parity_new};
+    for (int64_t q = p + 1; q < P.n; ++q) pair_at(P, S, T, p, q);
+    return;
+  }
   ListRegion L = list_region(P, S, instance_of(S, p));
   int tag = L.tag;
   float reach = S.list_reach[0];

@@ -59,6 +59,9 @@ typedef struct bm_params {
   int64_t instances;      /* independent cutting instances packed in this state (1: a single solver) */
   int64_t active_bricks;  /* bricks holding particles (entries of brick_list; 0: every brick) */
   int64_t active_nodes;   /* nodes some particle's stencil touches (entries of node_list; 0: all) */
+  int64_t pair_mode;      /* pair search (solver.py:260-263): 0 Verlet partner lists (list, auto),
+                             1 partner lists rebuilt from the cell buckets every step (hash),
+                             2 every pair q > p tested (all; O(n^2)) */
   int64_t regular;        /* every particle at local cell coordinates 1/4 or 3/4 per axis, in the
                              cell slot of its position bits (4 bx + 2 by + bz): the transfer
                              kernels may use the separable stencil */
