@@ -378,12 +378,20 @@ def evaluate_predictors(net, X, Y):
 
 
 # ---------------------------------------------------------------------------- plain coupling
-def plain_run(config, task, steps=None, arm_kw=None, device="cuda:0"):
+def plain_run(config, task, steps=None, arm_kw=None, device="cuda:0", record=None, frame_stride=10):
     """The reference's coupling loop for one knife (`franka_run._exchange_wrench`,
     franka_run.py:394-422, with `Franka.step`): per robot step `arm.state()`,
     `ExternalToolPort.advance` over one interval on a plain `Solver`, then the arm step with the
-    interval's wrench.  Returns the summary with sim/wall and the force trace."""
+    interval's wrench.  Returns the summary with sim/wall and the force trace.
+
+    record: a new directory for the run's files in the reference's formats (report.json,
+    trajectory.npz, columns.json, energy_columns.json; franka_run.py:600-690).  The arm
+    stand-in has no joints: its log and frames carry the end-effector pose (position, xyzw) as
+    qpos, its twist as qvel, and NaN in the joint-torque columns."""
+    from pathlib import Path
     from .coupling import ExternalToolPort
+    from .diagnostics import Metrics
+    from .runio import TrajectoryRecorder, edge_height, run_report, write
     from .solver import Solver
 
     solver = Solver(config, device)
@@ -392,14 +400,45 @@ def plain_run(config, task, steps=None, arm_kw=None, device="cuda:0"):
     arm = ImpedanceArm(origin[:1], orientation[:1], **(arm_kw or {}))
     total = int(round(task.duration / INTERVAL)) if steps is None else int(steps)
     forces = np.empty((total, 3))
+    out = None
+    if record is not None:
+        out = Path(record)
+        out.mkdir(parents=True, exist_ok=False)
+        recorder = TrajectoryRecorder(nq=7, nv=6, interval=INTERVAL, frame_stride=frame_stride)
+        metrics = Metrics(solver)
+    recording_s = 0.0
     begin = time.perf_counter()
-    for j in range(total):
-        origin, orientation, cutting, cut = task.at(j * INTERVAL)
-        reaction = port.advance(arm.state(0), INTERVAL, cutting=cutting, cut_index=cut)
-        wrench = np.concatenate([reaction.force_n, reaction.torque_nm])
-        arm.step(origin[:1], orientation[:1], wrench[None])
-        forces[j] = reaction.force_n
-    wall = time.perf_counter() - begin
+    status, error = "completed", None
+    try:
+        for j in range(total):
+            origin, orientation, cutting, cut = task.at(j * INTERVAL)
+            reaction = port.advance(arm.state(0), INTERVAL, cutting=cutting, cut_index=cut)
+            wrench = np.concatenate([reaction.force_n, reaction.torque_nm])
+            arm.step(origin[:1], orientation[:1], wrench[None])
+            forces[j] = reaction.force_n
+            if out is not None:
+                t_rec = time.perf_counter()
+                pose = np.concatenate([arm.p[0], arm.q[0]])
+                edge = origin[0] - rotation_matrix(orientation)[0, :, 2] * task.half_z
+                nan7 = np.full(7, np.nan)
+                if recorder.step((j + 1) * INTERVAL, edge, arm.p[0], wrench, pose, np.concatenate([arm.v[0], arm.w[0]]),
+                                 nan7, nan7, wrench):
+                    lowest, _ = edge_height([arm.p[:1]], [arm.q[:1]], config.blade_half_size)
+                    energy = np.concatenate([[(j + 1) * INTERVAL], metrics.read(), [float(solver.work.numpy()[0])]])
+                    recorder.frame(pose, solver, energy, np.concatenate([origin[0], orientation[0]]), lowest[0, 0])
+                recording_s += time.perf_counter() - t_rec
+    except Exception as exc:
+        status, error = "failed", repr(exc)
+        raise
+    finally:
+        wall = time.perf_counter() - begin - recording_s
+        if out is not None:
+            recorder.save(out, solver.origin, solver.radius, ["time_s"] + Metrics.columns +
+                          ["directional_contact_work_J"])
+            write(out / "report.json", run_report(status, error, recorder.steps * INTERVAL, wall,
+                                                  {"material": port.timing_s["submission"] +
+                                                   port.timing_s["completion_and_readback"],
+                                                   "recording": recording_s}, port.timing_s, True))
     return dict(instances=1, robot_steps=total, sim_s=total * INTERVAL, wall_s=wall,
                 sim_per_wall=total * INTERVAL / wall, mean_force_n=float(np.linalg.norm(forces, axis=1).mean())), forces
 

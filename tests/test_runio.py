@@ -102,3 +102,50 @@ def test_drop_scenes_release_pose_and_command_values():
     assert freefall._value("true") is True and freefall._value("length") == "length"
     with pytest.raises(ValueError):
         replace(knife, duration=0.00001).steps()
+
+
+TRAJECTORY_KEYS = {"log", "qpos", "particles", "codes", "origin", "deformation", "plastic_deformation", "velocity",
+                   "energy", "control_tcp", "cycle_trace", "edge_heights", "radius", "interval", "frame_stride"}
+REPORT_KEYS = {"status", "exception", "sim_s", "wall_s", "realtime_ratio", "warmup_s", "recording", "display",
+               "pause_wall_s", "live_setup_s", "timing_scope", "time_s", "material_present", "warnings",
+               "surface_wall_s", "surface_phase_s", "material_interval_phase_s"}
+
+
+class _Arrays:
+    def __init__(self, a):
+        self.a = a
+
+    def numpy(self):
+        return self.a
+
+
+def test_trajectory_and_report_formats(tmp_path):
+    """trajectory.npz, columns.json and report.json carry the keys of franka_run.py:600-690;
+    the log columns tile the row (nq = 7 pose values and nv = 6 twist values for the arm)."""
+    from blademaster_hip.runio import TrajectoryRecorder, log_columns, run_report
+
+    n = 5
+    solver = type("S", (), {})()
+    solver.x, solver.v = _Arrays(np.zeros((n, 3))), _Arrays(np.ones((n, 3)))
+    solver.codes = _Arrays(np.zeros(n, np.int64))
+    solver.F = solver.Fp = _Arrays(np.tile(np.eye(3), (n, 1, 1)))
+    rec = TrajectoryRecorder(nq=7, nv=6, interval=0.001, frame_stride=2)
+    frames = 0
+    for k in range(4):
+        if rec.step((k + 1) * 0.001, np.zeros(3), np.ones(3), np.arange(6), np.arange(7), np.zeros(6),
+                    np.full(7, np.nan), np.full(7, np.nan), np.arange(6)):
+            rec.frame(np.arange(7), solver, np.zeros(8), np.zeros(7), 0.01)
+            frames += 1
+    with pytest.raises(ValueError):
+        rec.step(0.0, np.zeros(3), np.zeros(3), np.zeros(6), np.zeros(6), np.zeros(6), np.zeros(7), np.zeros(7),
+                 np.zeros(6))
+    rec.save(tmp_path, np.zeros(3), 0.0015, ["time_s", "a", "b"])
+    with np.load(tmp_path / "trajectory.npz", allow_pickle=False) as z:
+        assert set(z.files) == TRAJECTORY_KEYS
+        assert z["log"].shape == (4, log_columns(7, 6)["material_generalized_force"][1])
+        assert z["particles"].shape == (frames, n, 3) and frames == 2
+    cols = json.loads((tmp_path / "columns.json").read_text())
+    spans = sorted([v] if isinstance(v, int) else list(range(*v)) for v in cols.values())
+    assert sorted(i for s in spans for i in s) == list(range(13 + 7 + 6 + 14 + 6))
+    report = run_report("completed", None, 0.004, 0.002, {"material": 0.001}, {"submission": 0.0}, True)
+    assert set(report) == REPORT_KEYS and report["realtime_ratio"] == 2.0

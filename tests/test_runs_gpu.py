@@ -33,3 +33,21 @@ def test_freefall_writes_states_manifest_and_result(tmp_path):
     assert manifest["status"] == "complete" and manifest["completed_steps"] == 20
     assert manifest["config"]["free_blade"] is True
     assert sum(result["component_sizes"]) == result["particle_count"]
+
+
+def test_coupled_run_records_trajectory_and_report(tmp_path):
+    """The single-knife coupled loop writes the reference's run files (franka_run.py:600-690)."""
+    from blademaster_hip import surrogate
+    from blademaster_hip.presets import banana_config
+
+    task = surrogate.ThreeCutTask(1, seed=0)
+    cfg = surrogate.task_configs(1)[0]
+    summary, forces = surrogate.plain_run(cfg, task, steps=30, record=tmp_path / "run", frame_stride=10)
+    assert summary["robot_steps"] == 30 and np.isfinite(forces).all()
+    with np.load(tmp_path / "run" / "trajectory.npz", allow_pickle=False) as z:
+        assert z["log"].shape[0] == 30 and z["particles"].shape[0] == 3
+        assert z["particles"].shape[1] == z["codes"].shape[1]
+        np.testing.assert_allclose(z["log"][:, 0], 0.001 * np.arange(1, 31), atol=1e-12)
+    report = json.loads((tmp_path / "run" / "report.json").read_text())
+    assert report["status"] == "completed" and report["recording"] is True
+    assert abs(report["sim_s"] - 0.03) < 1e-12
