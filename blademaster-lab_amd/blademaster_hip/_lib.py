@@ -41,6 +41,7 @@ class Params(ctypes.Structure):
         ("active_nodes", ctypes.c_int64),
         ("pair_mode", ctypes.c_int64),
         ("regular", ctypes.c_int64),
+        ("maxwell", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)
         for name in (
@@ -59,6 +60,9 @@ class Params(ctypes.Structure):
         ("rot_k", ctypes.c_double),
         ("rot_c", ctypes.c_double),
         ("pair_skin", ctypes.c_double),
+        ("viscosity", ctypes.c_double),
+        ("mu_maxwell", ctypes.c_double),
+        ("maxwell_relax", ctypes.c_double),
     ]
 
 
@@ -85,6 +89,7 @@ STATE_FIELDS = [
     ("bc_impulse", c_f32p), ("bc_extrapolated", c_f32p), ("bc_active", c_i32p), ("bc_nodes", c_i32p),
     ("bc_node_mark", c_i32p), ("bc_grid", c_f32p), ("bc_coupled", c_f32p), ("bc_counts", c_i32p),
     ("bc_sums", c_f32p), ("history_mask", ctypes.POINTER(ctypes.c_uint64)),
+    ("viscous", ctypes.POINTER(ctypes.c_double)), ("viscous_impulse", c_f32p),
 ]
 
 

@@ -111,10 +111,9 @@ class Solver:
         self.board_scale = d["board_scale"]
         self.inertia_inv = d["inertia_inv"]
         self.maxwell = d["maxwell"]
-        if self.maxwell:
-            raise ValueError("Maxwell viscoelasticity is not provided by this build")
-        if cfg.dynamic_viscosity:
-            raise ValueError("dynamic_viscosity > 0 is not provided by this build")
+        self.mu_maxwell, self.maxwell_relax = d["mu_maxwell"], d["maxwell_relax"]
+        if _layout is not None and (self.maxwell or cfg.dynamic_viscosity):
+            raise ValueError("Batched instances take neither the Maxwell branch nor dynamic_viscosity")
         self.compact = cfg.grid_storage == "compact"
 
         f32, i64, i32 = torch.float32, torch.int64, torch.int32
@@ -175,6 +174,11 @@ class Solver:
             T["inst_mu"] = torch.tensor(np.asarray(_layout["mu"], dtype=np.float32), device=dev)
             T["inst_lam"] = torch.tensor(np.asarray(_layout["lam"], dtype=np.float32), device=dev)
             T["inst_sums"] = t((I, 16, 10))
+        if self.maxwell:
+            # Fv - I of the Maxwell branch in float64 (material.py:485-535).
+            T["viscous"] = t((n, 3, 3), torch.float64)
+        if cfg.dynamic_viscosity:
+            T["viscous_impulse"] = t((F_, N_, 3))
         T["errors"] = t((3,), i32)
         T["topology_dirty"] = t((1,), i32, 1)
         T["fp_identity"] = t((1,), i32, 1)
@@ -256,6 +260,7 @@ class Solver:
         self.x, self.v = arr(T["x"], mark_pairs), arr(T["v"], mark_pairs)
         self.ref = arr(T["ref"])
         self.F, self.stress = arr(T["F"]), arr(T["stress"])
+        self.viscous = arr(T["viscous"]) if self.maxwell else None
         # Host writes to Fp end the device's identity shortcut for Fp.
         self.Fp = arr(T["Fp"], lambda: T["fp_identity"].fill_(0))
         self.codes = arr(T["codes"], mark_topology)
@@ -340,6 +345,9 @@ class Solver:
         p.active_nodes = self.active_nodes
         p.regular = int(self.regular)
         p.pair_mode = PAIR_MODES[cfg.pair_search]
+        p.maxwell = int(self.maxwell)
+        p.mu_maxwell, p.maxwell_relax = self.mu_maxwell, self.maxwell_relax
+        p.viscosity = cfg.dynamic_viscosity
         return p
 
     def _make_state(self):
@@ -386,7 +394,7 @@ class Solver:
 
     # ------------------------------------------------------------------ stages
     def refresh_stress(self):
-        """Recompute the stored stress after host writes to F or Fp (solver.py:318-329)."""
+        """Recompute the stored stress after host writes to F, Fp or viscous (solver.py:318-329)."""
         self._call("bm_refresh_stress")
 
     def material_step(self):
@@ -582,6 +590,8 @@ class Solver:
             "blade_orientation": self.tool_rot.numpy(),
             "blade_omega": self.tool_omega.numpy(),
         }
+        if self.maxwell:
+            state["viscous_deformation"] = np.eye(3) + self.viscous.numpy()
         state["time"] = self.step_count * self.config.dt
         if any(not np.isfinite(v).all() for v in state.values()):
             raise RuntimeError("Nonfinite state; run is invalid")

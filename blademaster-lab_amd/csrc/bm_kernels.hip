@@ -1370,12 +1370,16 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 // particle's F and stress (update_F_plastic, material.py:445-475) and uses that stress; on
 // the separable path the corner velocities of that update are one gather per lane (the lane's
 // corner) and the stencil's gather stages.
+// kMx: the Maxwell branch is on (a separate instantiation, so that the elastic kernels keep
+// their registers).
+template <bool kMx>
 __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate,
                                    bool store_stress);
-template <bool kRemap>
+template <bool kRemap, bool kMx>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx, bool store_stress);
 
+template <bool kMx>
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
@@ -1440,10 +1444,10 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
         f3 vc = f3{0.f, 0.f, 0.f};
         if (cell_live(live_mask, lane)) vc = ld3(S.grid_vel, cell_corner_node(P, w, D, c, a_mine));
         m3 rate = separable_rate(vc, sg);
-        if (update) Pk = finish_update(P, S, p, rate, false);
+        if (update) Pk = finish_update<kMx>(P, S, p, rate, false);
       }
     } else if (update) {
-      Pk = update_particle<false>(P, S, D, p, lp, fidx, false);
+      Pk = update_particle<false, kMx>(P, S, D, p, lp, fidx, false);
     }
     if (live) {
       if (!update) Pk = ld9(S.stress, p);
@@ -1676,6 +1680,74 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
   }
 }
 
+// viscous_impulse (material.py:321-340) of one particle: with the coupled old velocities of
+// its 8 corners, L = (sum_a old_v_a (x) grad w_a) F^-1 and the first Piola stress of a
+// Newtonian fluid eta J (L + L^T) F^-T; -dt V0 stress grad w_a goes to each corner's field.
+__global__ void __launch_bounds__(kBlock) k_viscous_impulse(const bm_params P, bm_state S) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  int32_t base = S.base_node[p];
+  int64_t N = P.node_count;
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  float inv_h = 1.0f / float(P.spacing);
+  uint2 fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  f3 lp = ld3(S.local, p);
+  int64_t slot[8];
+  f3 g[8];
+  m3 rate = mzero();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    slot[a] = int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz));
+    float wt;
+    stencil_corner(lp, inv_h, a, wt, g[a]);
+    f3 v = ld3(S.old_v, slot[a]);
+    rate.a[0] += v.x * g[a].x; rate.a[1] += v.x * g[a].y; rate.a[2] += v.x * g[a].z;
+    rate.a[3] += v.y * g[a].x; rate.a[4] += v.y * g[a].y; rate.a[5] += v.y * g[a].z;
+    rate.a[6] += v.z * g[a].x; rate.a[7] += v.z * g[a].y; rate.a[8] += v.z * g[a].z;
+  }
+  m3 F = ld9(S.F, p);
+  m3 inv = minv(F);
+  m3 L = mmul(rate, inv);
+  m3 sym;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sym.a[3 * i + j] = L.a[3 * i + j] + L.a[3 * j + i];
+  m3 stress = mmul_t(sym, inv);
+  // -dt V0 eta J folded into one factor.
+  float c = -float(P.dt) * float(P.volume) * (float(P.viscosity) * mdet(F));
+#pragma unroll
+  for (int a = 0; a < 8; ++a) atomic_add3(S.viscous_impulse, slot[a], c * mvec(stress, g[a]));
+}
+
+// Couples the viscous impulses of each node's fields (Eq. 8, as the node kernel does for the
+// P2G impulse; the exchange is linear, so coupling the two impulses apart equals coupling
+// their sum) and adds them to new_v: new_v_a += couple(j_visc)_a / m_a.  Clears the impulses
+// for the next substep.
+__global__ void __launch_bounds__(kBlock) k_viscous_couple(const bm_params P, bm_state S) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= active_node_count(P)) return;
+  int64_t node = active_node(S, i), N = P.node_count;
+  int nf = S.node_nf[node];
+  float total = 0.0f;
+  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
+  for (int a = 0; a < nf; ++a) {
+    int64_t sa = int64_t(a) * N + node;
+    float ma = S.grid_mass[sa];
+    if (!(ma > 0.0f)) continue;
+    int64_t ca = S.field_code[sa];
+    f3 ja = ld3(S.viscous_impulse, sa), di = f3{0.f, 0.f, 0.f};
+    for (int b = 0; b < nf; ++b) {
+      if (b == a) continue;
+      int64_t sb = int64_t(b) * N + node;
+      if (!compatible(ca, S.field_code[sb])) continue;
+      di = di + (ma * ld3(S.viscous_impulse, sb) - S.grid_mass[sb] * ja);
+    }
+    st3(S.new_v, sa, ld3(S.new_v, sa) + (ja + di / total) / ma);
+  }
+  for (int f = 0; f < nf; ++f) st3(S.viscous_impulse, int64_t(f) * N + node, f3{0.f, 0.f, 0.f});
+}
+
 __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp);
 
 // g2p_remap (material.py:384-401) for one brick: each particle lane gathers the old and new
@@ -1829,18 +1901,19 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     out[int64_t(i) * 3 + 2] = sum[2];
   }
 }
-template <bool kRemap>
+template <bool kRemap, bool kMx>
 __device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks);
 
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
 // update to the next P2G (deferred_update), so no separate F-update launch follows.
+template <bool kMx>
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
                                                             int update_blocks) {
   const int node_blocks = int(gridDim.x) - update_blocks;
   if (int(blockIdx.x) >= node_blocks) {
-    deferred_update<true>(P, S, D, int(blockIdx.x) - node_blocks, update_blocks);
+    deferred_update<true, kMx>(P, S, D, int(blockIdx.x) - node_blocks, update_blocks);
     return;
   }
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2051,13 +2124,114 @@ __device__ inline m3 stress_of(const bm_params& P, const bm_state& S, int64_t p,
   return elastic_stress(P, S, F, Fp, mu, lam);
 }
 
+// Generalized standard linear solid (maxwell_at, material.py:485-535).  The elastic part
+// f = U diag(s) V^T = F Fp^-1 loads two branches in parallel:
+//   equilibrium: fixed corotated with mu - mu_m and lam + 2/3 mu_m (so that the instantaneous
+//     small-strain shear and bulk moduli are those of mu and lam);
+//   Maxwell: f_m = f Fv^-1, Fv = I + viscous[p], Kirchhoff stress 2 mu_m dev(log V_m), with
+//     f_m = U_m diag(s_m) V_m^T.
+// relax > 0 moves a fraction relax of the deviatoric Hencky strain of f_m into Fv:
+// f_m <- f_m W with W = V_m diag(exp(-relax dev)) V_m^T, i.e. Fv <- W^-1 Fv, stored as
+// Fv - I += delta + delta (Fv - I), delta = V_m diag(expm1(relax dev)) V_m^T.  The per-step
+// change of Fv is far below float32 resolution for long relaxation times, so Fv - I is kept
+// in float64.  The first Piola stress is equilibrium + tau_m F^-T.
+__device__ inline m3 maxwell_stress(const bm_params& P, const bm_state& S, int64_t p, const m3& F, const m3& Fp,
+                                    const m3& U, f3 sg, const m3& V,
+                                                       float mu, float lam, float relax) {
+  double* Dp = S.viscous + 9 * p;
+  double D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = Dp[k];
+  m3 Fv, US;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fv.a[k] = float(D[k]) + ((k % 4 == 0) ? 1.0f : 0.0f);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    US.a[3 * i] = U.a[3 * i] * sg.x;
+    US.a[3 * i + 1] = U.a[3 * i + 1] * sg.y;
+    US.a[3 * i + 2] = U.a[3 * i + 2] * sg.z;
+  }
+  m3 fm = mmul(mmul_t(US, V), minv(Fv));
+  m3 Um, Vm;
+  f3 sm;
+  svd3(fm, Um, sm, Vm);
+  if (fminf(fminf(sm.x, sm.y), sm.z) <= 0.0f) {
+    atomicAdd(S.errors + 1, 1);
+    return mzero();
+  }
+  f3 e = f3{logf(sm.x), logf(sm.y), logf(sm.z)};
+  float mean = (e.x + e.y + e.z) / 3.0f;
+  f3 dev = e - f3{mean, mean, mean};
+  if (relax > 0.0f) {
+    f3 step = relax * dev;
+    dev = dev - step;
+    f3 growth = f3{expm1f(step.x), expm1f(step.y), expm1f(step.z)};
+    m3 VG;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      VG.a[3 * i] = Vm.a[3 * i] * growth.x;
+      VG.a[3 * i + 1] = Vm.a[3 * i + 1] * growth.y;
+      VG.a[3 * i + 2] = Vm.a[3 * i + 2] * growth.z;
+    }
+    m3 delta = mmul_t(VG, Vm);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double dd = double(delta.a[3 * i]) * D[j] + double(delta.a[3 * i + 1]) * D[3 + j] +
+                    double(delta.a[3 * i + 2]) * D[6 + j];
+        Dp[3 * i + j] = D[3 * i + j] + double(delta.a[3 * i + j]) + dd;
+      }
+  }
+  float mu_m = float(P.mu_maxwell);
+  f3 k2 = (2.0f * mu_m) * dev;
+  m3 UK;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    UK.a[3 * i] = Um.a[3 * i] * k2.x;
+    UK.a[3 * i + 1] = Um.a[3 * i + 1] * k2.y;
+    UK.a[3 * i + 2] = Um.a[3 * i + 2] * k2.z;
+  }
+  m3 kirchhoff = mmul_t(UK, Um);
+  m3 eq = corotated(U, sg, V, minv(Fp), mu - mu_m, lam + (2.0f / 3.0f) * mu_m);
+  m3 tf = mmul_t(kirchhoff, minv(F));
+#pragma unroll
+  for (int k = 0; k < 9; ++k) eq.a[k] += tf.a[k];
+  return eq;
+}
+
+// Stress of particle p with the Maxwell branch (maxwell_stress_at, material.py:538-566):
+// with `project` and plasticity, the return map at the instantaneous shear modulus first
+// (it stores Fp on flow); relax as in maxwell_stress (0 only evaluates the stress, as
+// compute_stress_maxwell, material.py:569-587).
+__device__ inline m3 maxwell_update(const bm_params& P, const bm_state& S, int64_t p,
+                                                       const m3& F, m3 Fp, bool project, float relax) {
+  float mu, lam;
+  lame_of(P, S, p, mu, lam);
+  m3 U, V;
+  f3 sg;
+  if (project && P.plastic) {
+    if (!plastic_at(P, S, p, F, Fp, U, sg, V, mu)) return mzero();
+  } else {
+    m3 f = mmul(F, minv(Fp));
+    float J = mdet(f);
+    if (!(J > 0.0f) || !isfinite(J)) {
+      atomicAdd(S.errors + 1, 1);
+      return mzero();
+    }
+    svd3(f, U, sg, V);
+  }
+  return maxwell_stress(P, S, p, F, Fp, U, sg, V, mu, lam, relax);
+}
+
 // F += dt rate for particle p, then the stress; stores F (and Fp after plastic flow) and,
 // with store_stress, the stress, and returns the stress.  P2G uses the stress of the particles
 // it updates at once and does not store it: the stored stress is read only by a P2G with no
 // update pending, and every path to such a P2G (the F-update kernel, early deferred
 // updates, refresh_stress) stores it.
+template <bool kMx>
 __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate,
-                                   bool store_stress = true) {
+                                   bool store_stress) {
   // While no plastic flow has happened and the host has not written Fp, every Fp is I.
   bool fp_identity = S.fp_identity != nullptr && S.fp_identity[0] != 0;
   m3 Fm = ld9(S.F, p), Fp;
@@ -2072,7 +2246,9 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
   for (int k = 0; k < 9; ++k) Fm.a[k] += dt * rate.a[k];
   st9(S.F, p, Fm);
   m3 stress;
-  if (fp_identity && !P.plastic) {
+  if constexpr (kMx) {
+    stress = maxwell_update(P, S, p, Fm, Fp, true, float(P.maxwell_relax));
+  } else if (fp_identity && !P.plastic) {
     float mu, lam;
     lame_of(P, S, p, mu, lam);
     stress = elastic_stress_plain(S, Fm, mu, lam);
@@ -2088,7 +2264,7 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
 // grid_vel (a node array small enough to stay in L2), or with kRemap from the re-scattered
 // partials (remapped_velocity), for an update that runs beside k_couple_remapped.  Stores F
 // (and Fp after plastic flow) and the stress, and returns the stress.
-template <bool kRemap>
+template <bool kRemap, bool kMx>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx, bool store_stress) {
   int32_t base = S.base_node[p];
@@ -2122,7 +2298,7 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, cons
       rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
     }
   }
-  return finish_update(P, S, p, rate, store_stress);
+  return finish_update<kMx>(P, S, p, rate, store_stress);
 }
 
 // The deferred F update of a substep, run by `blocks` workgroups (this is workgroup `block`
@@ -2135,13 +2311,13 @@ __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, cons
 // A full re-registration (rare: more relabels than the changed list holds) updates every
 // particle here.  Running beside the node kernel (kRemap), these updates read their corner
 // velocities through remapped_velocity.
-template <bool kRemap>
+template <bool kRemap, bool kMx>
 __device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks) {
   const int topo = S.topology_dirty[0];
   const int64_t first = int64_t(block) * blockDim.x + threadIdx.x, stride = int64_t(blocks) * blockDim.x;
   const uint2* fidx = reinterpret_cast<const uint2*>(S.field_idx);
   if (topo & kTopoFull) {
-    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap>(P, S, D, p, ld3(S.local, p), fidx[p], true);
+    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap, kMx>(P, S, D, p, ld3(S.local, p), fidx[p], true);
     return;
   }
   // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
@@ -2160,7 +2336,7 @@ __device__ void deferred_update(const bm_params& P, const bm_state& S, const Div
     int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
     int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
     if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
-      update_particle<kRemap>(P, S, D, q, ld3(S.local, q), fidx[q], true);
+      update_particle<kRemap, kMx>(P, S, D, q, ld3(S.local, q), fidx[q], true);
   }
 }
 
@@ -2169,10 +2345,11 @@ __device__ void deferred_update(const bm_params& P, const bm_state& S, const Div
 // threads in all, in a multiple of 8 workgroups, so a workgroup keeps the bricks of its XCD).
 constexpr int kUpdateThreads = 1 << 20;
 constexpr int kDeferredUpdateBlocks = 128;
+template <bool kMx>
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
   if (defer) {
-    deferred_update<false>(P, S, D, int(blockIdx.x), int(gridDim.x));
+    deferred_update<false, kMx>(P, S, D, int(blockIdx.x), int(gridDim.x));
     return;
   }
   int cpb = cells_per_brick(P), CS = 8 * cpb;
@@ -2192,9 +2369,9 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
       f3 vc = f3{0.f, 0.f, 0.f};
       if (cell_live(live_mask, lane)) vc = ld3(S.grid_vel, cell_corner_node(P, window_of(P, D, b), D, c, a_mine));
       m3 rate = separable_rate(vc, sg);
-      if (live) finish_update(P, S, p, rate);
+      if (live) finish_update<kMx>(P, S, p, rate, true);
     } else if (live) {
-      update_particle<false>(P, S, D, p, ld3(S.local, p), fidx, true);
+      update_particle<false, kMx>(P, S, D, p, ld3(S.local, p), fidx, true);
     }
   }
 }
@@ -2202,6 +2379,10 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
 __global__ void k_refresh_stress(const bm_params P, bm_state S) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
+  if (P.maxwell) {
+    st9(S.stress, p, maxwell_update(P, S, p, ld9(S.F, p), ld9(S.Fp, p), false, 0.0f));
+    return;
+  }
   float mu, lam;
   lame_of(P, S, p, mu, lam);
   st9(S.stress, p, elastic_stress(P, S, ld9(S.F, p), ld9(S.Fp, p), mu, lam));
@@ -2649,7 +2830,11 @@ inline int launch_status() {
 
 bool state_ok(const bm_params* p, const bm_state* s) {
   if (p == nullptr || s == nullptr || p->n < 0 || s->x == nullptr || s->v == nullptr) return false;
-  // Batched instances carry their Lame constants and the slots of their contact sums.
+  // The Maxwell branch needs its Fv - I; viscosity its grid impulses.
+  if ((p->maxwell && s->viscous == nullptr) || (p->viscosity > 0.0 && s->viscous_impulse == nullptr)) return false;
+  // Batched instances carry their Lame constants and the slots of their contact sums (and
+  // take neither the Maxwell branch nor viscosity).
+  if (s->instance != nullptr && (p->maxwell || p->viscosity > 0.0)) return false;
   return s->instance == nullptr || (s->inst_mu != nullptr && s->inst_lam != nullptr && s->inst_sums != nullptr &&
                                     s->inst_buckets != nullptr && s->inst_pair_builds != nullptr &&
                                     s->inst_pair_dirty != nullptr);
@@ -2820,6 +3005,22 @@ static int brick_threads(const bm_params* p) {
   t = ((t + 63) / 64) * 64;
   return int(t > 1024 ? 1024 : t);
 }
+// The node kernel of the re-scattered momentum (with `update_blocks` deferred-update
+// workgroups among `blocks`) and the F update, in the instantiation of the material model.
+static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
+                                   int update_blocks, hipStream_t st) {
+  if (p->maxwell)
+    hipLaunchKernelGGL(k_couple_remapped<true>, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
+  else
+    hipLaunchKernelGGL(k_couple_remapped<false>, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
+}
+static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
+                            int defer, int nbricks, hipStream_t st) {
+  if (p->maxwell)
+    hipLaunchKernelGGL(k_update_F<true>, dim3(blocks), dim3(threads), 0, st, *p, *s, D, defer, nbricks);
+  else
+    hipLaunchKernelGGL(k_update_F<false>, dim3(blocks), dim3(threads), 0, st, *p, *s, D, defer, nbricks);
+}
 static void launch_register(const bm_params* p, const bm_state* s, hipStream_t st) {
   int64_t total = node_launch(p);
   hipLaunchKernelGGL(k_topo_clear, dim3(blocks_for(total < 1 << 20 ? total : 1 << 20)), dim3(kBlock), 0, st, *p, *s);
@@ -2833,8 +3034,17 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
                       hipStream_t st, int defer = 0) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
   const Divisors D = divisors_of(p);
-  hipLaunchKernelGGL(k_p2g, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
+  if (p->maxwell)
+    hipLaunchKernelGGL(k_p2g<true>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
+  else
+    hipLaunchKernelGGL(k_p2g<false>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
   hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
+  // Viscosity: the impulse of the coupled old velocities, coupled and added to new_v
+  // (solver.py:433-458).
+  if (p->viscosity > 0.0) {
+    hipLaunchKernelGGL(k_viscous_impulse, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
+    hipLaunchKernelGGL(k_viscous_couple, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s);
+  }
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
@@ -2843,15 +3053,14 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   if (defer && s->instance == nullptr) {
     // The F update is deferred to the next P2G: a few extra workgroups of the node kernel set
     // that up (and make the early updates), and no F-update launch follows.
-    hipLaunchKernelGGL(k_couple_remapped, dim3(node_blocks + kDeferredUpdateBlocks), dim3(kBlock), 0, st, *p, *s, D,
-                       kDeferredUpdateBlocks);
+    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st);
     return;
   }
   // Batched instances relabel in many instances each substep: their early updates read
   // grid_vel after the node kernel, in a small deferred launch.
-  hipLaunchKernelGGL(k_couple_remapped, dim3(node_blocks), dim3(kBlock), 0, st, *p, *s, D, 0);
+  launch_couple_remapped(p, s, D, node_blocks, 0, st);
   int ub = defer ? kDeferredUpdateBlocks : update_blocks(tb);
-  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(tb), 0, st, *p, *s, D, defer ? 1 : 0, nb);
+  launch_update_F(p, s, D, nb < ub ? nb : ub, tb, defer ? 1 : 0, nb, st);
 }
 }  // namespace
 
@@ -2881,8 +3090,13 @@ int bm_register(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_p2g, dim3(int(brick_launch(p))), dim3(brick_threads(p)), lds_bytes(p, 7, 7), (hipStream_t)stream,
-                     *p, *s, 0, divisors_of(p));
+  const int nb = int(brick_launch(p)), tb = brick_threads(p);
+  if (p->maxwell)
+    hipLaunchKernelGGL(k_p2g<true>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), (hipStream_t)stream, *p, *s, 0,
+                       divisors_of(p));
+  else
+    hipLaunchKernelGGL(k_p2g<false>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), (hipStream_t)stream, *p, *s, 0,
+                       divisors_of(p));
   return launch_status();
 }
 
@@ -2902,16 +3116,14 @@ int bm_g2p_remap(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_couple_remapped(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
-                     *s, divisors_of(p), 0);
+  launch_couple_remapped(p, s, divisors_of(p), int(blocks_for(node_launch(p))), 0, (hipStream_t)stream);
   return launch_status();
 }
 
 int bm_update_F(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   int nb = int(brick_launch(p)), ub = update_blocks(brick_threads(p));
-  hipLaunchKernelGGL(k_update_F, dim3(nb < ub ? nb : ub), dim3(brick_threads(p)), 0,
-                     (hipStream_t)stream, *p, *s, divisors_of(p), 0, nb);
+  launch_update_F(p, s, divisors_of(p), nb < ub ? nb : ub, brick_threads(p), 0, nb, (hipStream_t)stream);
   return launch_status();
 }
 

@@ -65,11 +65,17 @@ typedef struct bm_params {
   int64_t regular;        /* every particle at local cell coordinates 1/4 or 3/4 per axis, in the
                              cell slot of its position bits (4 bx + 2 by + bz): the transfer
                              kernels may use the separable stencil */
+  int64_t maxwell;        /* Maxwell branch on (maxwell_tau > 0 and maxwell_fraction > 0,
+                             solver.py:189-191): the stress is that of maxwell_at (material.py:
+                             485-535) and every F update relaxes the branch */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
   double drive_k, drive_c, drive_limit, rot_k, rot_c;
   double pair_skin;
+  double viscosity;       /* dynamic viscosity (Pa s); > 0 adds viscous_impulse (material.py:321-340) */
+  double mu_maxwell;      /* maxwell_fraction * mu */
+  double maxwell_relax;   /* 1 - exp(-dt / maxwell_tau) */
 } bm_params;
 
 typedef struct bm_state {
@@ -155,6 +161,12 @@ typedef struct bm_state {
    * r of particle p may be nonzero.  The contact kernel writes a zero history only over a
    * set bit; host writes to the histories must set every bit.  Null: always write. */
   uint64_t *history_mask;
+  /* [n][9] float64 Fv - I of the Maxwell branch (f_maxwell = F Fp^-1 Fv^-1), used when
+   * maxwell is set */
+  double *viscous;
+  /* [F][N][3] viscous grid impulses of a substep; all zero between substeps (the node
+   * kernel that applies them clears them), used when viscosity > 0 */
+  float *viscous_impulse;
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */

@@ -21,7 +21,9 @@ REAL_ARRAYS = {
     "impulses": 3, "histories": None, "tool_pos": None, "tool_vel": None, "tool_rot": None, "tool_omega": None,
     "tool_mobility": None, "reaction": None, "torque": None, "board": None, "work": None, "speed": None,
     "tool_table_impulse": None, "interval_impulse": None, "interval_angular_impulse": None,
-    "bc_side": 1, "bc_impulse": 1, "bc_normal": 3,
+    "bc_side": 1, "bc_impulse": 1, "bc_normal": 3, "viscous": 9,
+    # grid arrays: one entry per (particle, corner) field slot at most
+    "grid_mass": 8, "old_v": 24, "new_v": 24, "grid_impulse": 24, "momentum": 24, "coupled_remapped": 24,
 }
 FIXED = {
     "histories": lambda n: 6 * n, "tool_pos": 3, "tool_vel": 3, "tool_rot": 4, "tool_omega": 3, "tool_mobility": 36,
@@ -44,6 +46,8 @@ class OracleConfig(ctypes.Structure):
         ("free_blade", ctypes.c_int32), ("pose_servo", ctypes.c_int32), ("external_tool", ctypes.c_int32),
         ("predict", ctypes.c_int32), ("accumulate_tool_impulses", ctypes.c_int32), ("cache_topology", ctypes.c_int32),
         ("cut_work_advance", ctypes.c_int32), ("cut_hold", ctypes.c_int32),
+        ("viscosity", ctypes.c_double), ("mu_maxwell", ctypes.c_double), ("maxwell_relax", ctypes.c_double),
+        ("maxwell", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
 
@@ -104,6 +108,10 @@ def constants(cfg):
         gk=cfg.tangent_stiffness * ratio,
         board_sn=cfg.board_normal_scale or sn,
         inertia_inv=3 / (cfg.blade_mass * np.array([hy * hy + hz * hz, hx * hx + hz * hz, hx * hx + hy * hy])),
+        # Maxwell branch (solver.py:189-191).
+        maxwell=cfg.maxwell_tau > 0 and cfg.maxwell_fraction > 0,
+        mu_maxwell=cfg.maxwell_fraction * E / (2 * (1 + nu)),
+        maxwell_relax=float(-np.expm1(-cfg.dt / cfg.maxwell_tau)) if cfg.maxwell_tau > 0 else 0.0,
     )
 
 
@@ -141,6 +149,9 @@ class OracleSim:
         c.cache_topology = int(cfg.cache_topology)
         c.cut_work_advance = int(cfg.cut_work == "advance")
         c.cut_hold = int(cfg.cut_hold)
+        c.viscosity = cfg.dynamic_viscosity
+        c.maxwell = int(k["maxwell"])
+        c.mu_maxwell, c.maxwell_relax = k["mu_maxwell"], k["maxwell_relax"]
         self.c = c
         self.precision = precision
         self.handle = self.lib.oracle_create(precision, ctypes.byref(c))

@@ -330,12 +330,15 @@ struct OracleConfig {
   double drive_k, drive_c, drive_limit, rot_k, rot_c;
   int32_t free_blade, pose_servo, external_tool, predict, accumulate_tool_impulses, cache_topology;
   int32_t cut_work_advance, cut_hold;
+  double viscosity, mu_maxwell, maxwell_relax;  // dynamic viscosity; Maxwell branch constants
+  int32_t maxwell, pad;
 };
 
 template <class R> struct Sim {
   OracleConfig c;
   int64_t n;
   std::vector<R> x, v, F, Fp, stress, damage, ref, weights, gradients, impulses, histories;
+  std::vector<double> viscous;  // Fv - I of the Maxwell branch, float64 at either precision
   std::vector<int64_t> codes, field_codes_override;
   std::vector<int32_t> nodes;
   std::vector<R> tool_pos, tool_vel, tool_rot, tool_omega, mobility, reaction, torque, board, work, speed, table;
@@ -363,6 +366,7 @@ template <class R> struct Sim {
     F.assign(9 * n, R(0)); Fp.assign(9 * n, R(0)); stress.assign(9 * n, R(0));
     for (int64_t p = 0; p < n; ++p) for (int i = 0; i < 3; ++i) F[9 * p + 4 * i] = Fp[9 * p + 4 * i] = R(1);
     damage.assign(n, R(0)); codes.assign(n, 0); nodes.assign(8 * n, 0);
+    viscous.assign(9 * n, 0.0);
     weights.assign(8 * n, R(0)); gradients.assign(24 * n, R(0));
     impulses.assign(3 * n, R(0)); histories.assign(6 * n, R(0));
     tool_pos.assign(3, R(0)); tool_vel.assign(3, R(0)); tool_rot = {R(0), R(0), R(0), R(1)};
@@ -783,7 +787,9 @@ template <class R> struct Sim {
 
   // corotated_stress (material.py:78-91)
   M3<R> corotated(const M3<R>& U, V3<R> sg, const M3<R>& V, const M3<R>& plastic_inverse) {
-    R mu = R(c.mu), lam = R(c.lam);
+    return corotated(U, sg, V, plastic_inverse, R(c.mu), R(c.lam));
+  }
+  M3<R> corotated(const M3<R>& U, V3<R> sg, const M3<R>& V, const M3<R>& plastic_inverse, R mu, R lam) {
     R J = sg.x * sg.y * sg.z;
     V3<R> d;
     for (int i = 0; i < 3; ++i) setc(d, i, R(2) * mu * (comp(sg, i) - R(1)) + lam * (J - R(1)) * J / comp(sg, i));
@@ -831,6 +837,118 @@ template <class R> struct Sim {
     return true;
   }
 
+  // maxwell_at (material.py:485-535): generalized standard linear solid on the elastic part
+  // f = U diag(sg) V^T.  The Maxwell branch f_m = f (I + D)^-1, D = viscous[p] (float64,
+  // rounded to R where it enters f_m as the reference does), has Kirchhoff stress
+  // 2 mu_m dev(log s_m) in the basis U_m; relax > 0 first moves that fraction of the
+  // deviatoric Hencky strain into Fv: D <- D + delta + delta D with
+  // delta = V_m diag(expm1(relax dev)) V_m^T.  (The reference evaluates expm1 by a
+  // three-term series below |x| = 1e-3, exact to far below R's resolution there; std::expm1
+  // is used here.)  Returns equilibrium(mu - mu_m, lam + 2/3 mu_m) + tau_m F^-T.
+  M3<R> maxwell_at(int64_t p, const M3<R>& U, V3<R> sg, const M3<R>& V, R relax) {
+    M3<R> S = zero3<R>();
+    S.m[0][0] = sg.x; S.m[1][1] = sg.y; S.m[2][2] = sg.z;
+    M3<R> f = mul(mul(U, S), transpose(V));
+    double* D = &viscous[9 * p];
+    M3<R> Fv = eye3<R>();
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) Fv.m[i][j] += R(D[3 * i + j]);
+    M3<R> Um, Vm;
+    V3<R> sm;
+    svd3(mul(f, inverse(Fv)), Um, sm, Vm);
+    if (std::min(std::min(sm.x, sm.y), sm.z) <= R(0)) {
+      errors[1] += 1;
+      return zero3<R>();
+    }
+    V3<R> e = {std::log(sm.x), std::log(sm.y), std::log(sm.z)};
+    R mean = (e.x + e.y + e.z) / R(3);
+    V3<R> dev = e - V3<R>{mean, mean, mean};
+    if (relax > R(0)) {
+      V3<R> step = relax * dev;
+      dev = dev - step;
+      M3<R> G = zero3<R>();
+      G.m[0][0] = std::expm1(step.x); G.m[1][1] = std::expm1(step.y); G.m[2][2] = std::expm1(step.z);
+      M3<R> delta = mul(mul(Vm, G), transpose(Vm));
+      double next[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double dd = 0.0;
+          for (int k = 0; k < 3; ++k) dd += double(delta.m[i][k]) * D[3 * k + j];
+          next[3 * i + j] = D[3 * i + j] + double(delta.m[i][j]) + dd;
+        }
+      for (int k = 0; k < 9; ++k) D[k] = next[k];
+    }
+    R mu_m = R(c.mu_maxwell);
+    M3<R> K = zero3<R>();
+    K.m[0][0] = R(2) * mu_m * dev.x; K.m[1][1] = R(2) * mu_m * dev.y; K.m[2][2] = R(2) * mu_m * dev.z;
+    M3<R> kirchhoff = mul(mul(Um, K), transpose(Um));
+    M3<R> equilibrium = corotated(U, sg, V, inverse(g9(Fp, p)), R(c.mu) - mu_m, R(c.lam) + R(2) / R(3) * mu_m);
+    return addm(equilibrium, mul(kirchhoff, transpose(inverse(g9(F, p)))));
+  }
+
+  // maxwell_stress_at (material.py:538-566) with project; compute_stress_maxwell
+  // (material.py:569-587) without it (relax 0 there).
+  void maxwell_stress_at(int64_t p, R relax, bool project) {
+    M3<R> U, V;
+    V3<R> sg;
+    if (project && c.yield_stress > 0) {
+      if (!plastic_at(p, U, sg, V)) {
+        s9(stress, p, zero3<R>());
+        return;
+      }
+    } else {
+      M3<R> f = mul(g9(F, p), inverse(g9(Fp, p)));
+      R J = det(f);
+      if (J <= R(0) || !std::isfinite(double(J))) {
+        errors[1] += 1;
+        s9(stress, p, zero3<R>());
+        return;
+      }
+      svd3(f, U, sg, V);
+    }
+    s9(stress, p, maxwell_at(p, U, sg, V, relax));
+  }
+
+  // refresh_stress (solver.py:318-329)
+  void refresh_stress() {
+    for (int64_t p = 0; p < n; ++p) {
+      if (c.maxwell) maxwell_stress_at(p, R(0), false);
+      else stress_at(p);
+    }
+  }
+
+  // viscous_impulse (material.py:321-340): with the coupled old velocities at the particle's
+  // slots, L = (sum_a old_v_a (x) grad w_a) F^-1, stress = eta J (L + L^T) F^-T, and
+  // -dt V0 stress grad w_a added to grid_impulse at each slot.
+  void viscous_impulse() {
+    R eta = R(c.viscosity), dt = R(c.dt), vol = R(c.volume);
+    for (int64_t p = 0; p < n; ++p) {
+      M3<R> rate = zero3<R>();
+      for (int a = 0; a < 8; ++a) {
+        if (slots[8 * p + a] < 0) continue;  // unregistered (registration fills every slot)
+        V3<R> g = {gradients[24 * p + 3 * a], gradients[24 * p + 3 * a + 1], gradients[24 * p + 3 * a + 2]};
+        rate = addm(rate, outer(g3(old_v, slots[8 * p + a]), g));
+      }
+      M3<R> Fm = g9(F, p);
+      M3<R> inv = inverse(Fm);
+      M3<R> L = mul(rate, inv);
+      M3<R> P = mul(scale(eta * det(Fm), addm(L, transpose(L))), transpose(inv));
+      for (int a = 0; a < 8; ++a) {
+        if (slots[8 * p + a] < 0) continue;
+        V3<R> g = {gradients[24 * p + 3 * a], gradients[24 * p + 3 * a + 1], gradients[24 * p + 3 * a + 2]};
+        add3(grid_impulse, slots[8 * p + a], mulv(scale(-dt * vol, P), g));
+      }
+    }
+  }
+
+  // grid_vel (material.py:212-236) over the active fields.
+  void grid_vel(const std::vector<R>& mom, const std::vector<R>& imp) {
+    for (int64_t s = 0; s < active_count; ++s) {
+      if (grid_mass[s] <= R(0)) continue;
+      s3(old_v, s, g3(mom, s) / grid_mass[s]);
+      s3(new_v, s, (g3(mom, s) + g3(imp, s)) / grid_mass[s]);
+    }
+  }
+
   // update_F_plastic (material.py:445-475), reading the coupled re-scattered momentum.
   void update_F() {
     R dt = R(c.dt);
@@ -838,13 +956,16 @@ template <class R> struct Sim {
       M3<R> rate = zero3<R>();
       for (int a = 0; a < 8; ++a) {
         int32_t s = slots[8 * p + a];
-        if (grid_mass[s] > R(0)) {
+        if (s >= 0 && grid_mass[s] > R(0)) {
           V3<R> g = {gradients[24 * p + 3 * a], gradients[24 * p + 3 * a + 1], gradients[24 * p + 3 * a + 2]};
           rate = addm(rate, outer(g3(coupled_remapped, s) / grid_mass[s], g));
         }
       }
       s9(F, p, addm(g9(F, p), scale(dt, rate)));
-      if (c.yield_stress > 0) {
+      if (c.maxwell) {
+        // update_F_maxwell (material.py:590-618)
+        maxwell_stress_at(p, R(c.maxwell_relax), true);
+      } else if (c.yield_stress > 0) {
         M3<R> U, V;
         V3<R> sg;
         if (plastic_at(p, U, sg, V))
@@ -971,7 +1092,18 @@ template <class R> struct Sim {
   void material_step() {
     if (!c.cache_topology || topology_dirty) register_topology();
     p2g();
-    couple_and_velocity();
+    if (c.viscosity > 0) {
+      // The viscous path of solver.py:433-458: coupled momentum, old_v from it, the viscous
+      // impulse on top of the P2G impulse, then both coupled together.
+      std::vector<R> coupled_momentum(momentum.size(), R(0)), coupled_impulse(grid_impulse.size(), R(0));
+      couple(momentum, coupled_momentum);
+      grid_vel(coupled_momentum, grid_impulse);
+      viscous_impulse();
+      couple(grid_impulse, coupled_impulse);
+      grid_vel(coupled_momentum, coupled_impulse);
+    } else {
+      couple_and_velocity();
+    }
     if (c.external_tool && !c.predict) blade_constraint();
     g2p_remap();
     couple(remapped, coupled_remapped);
@@ -1397,6 +1529,15 @@ template <class R> bool real_array(Sim<R>& s, const std::string& name, std::vect
 template <class R> int get_set(Sim<R>& s, const char* cname, void* data, int64_t count, bool set) {
   std::string name(cname);
   std::vector<R>* arr = nullptr;
+  if (name == "viscous") {
+    if (count != int64_t(s.viscous.size())) return -2;
+    double* d = static_cast<double*>(data);
+    for (int64_t i = 0; i < count; ++i) {
+      if (set) s.viscous[i] = d[i];
+      else d[i] = s.viscous[i];
+    }
+    return 0;
+  }
   if (real_array(s, name, arr)) {
     if (count != int64_t(arr->size())) return -2;
     double* d = static_cast<double*>(data);
@@ -1503,7 +1644,11 @@ int oracle_stage(void* handle, const char* stage, int arg0, int64_t arg1) {
     }
     if (st == "assign") { s->assign(arg1, arg0); return 0; }
     if (st == "finish_external") { s->finish_external(); return 0; }
-    if (st == "refresh_stress") { for (int64_t p = 0; p < s->n; ++p) s->stress_at(p); return 0; }
+    if (st == "refresh_stress") { s->refresh_stress(); return 0; }
+    // The F update alone on the registered slots (grid_mass and coupled_remapped as set).
+    if (st == "update_F") { s->update_F(); return 0; }
+    // viscous_impulse alone: adds to grid_impulse from old_v on the registered slots.
+    if (st == "viscous") { s->viscous_impulse(); return 0; }
     if (st == "plastic_project") {
       for (int64_t p = 0; p < s->n; ++p) {
         using RR = std::decay_t<decltype(s->x[0])>;

@@ -38,6 +38,8 @@ Parity unpinned: the exact float32 bits of the reference's wp.svd3 and of Warp's
 code generation (no Warp in this environment); the oracle's SVD is evaluated in float64.
 """
 
+from dataclasses import replace
+
 import numpy as np
 import pytest
 from scipy.linalg import polar
@@ -637,3 +639,200 @@ def test_mls_softening_follows_the_plastic_increment():
     dmg = o.get("damage") > 0
     assert np.array_equal(dmg[clear], (want_ys <= 0)[clear])
     assert yielded.sum() > 50 and (~yielded).sum() > 10 and (want_ys <= 0).sum() > 5
+
+
+# ------------------------------------------------------------------ Maxwell branch, viscosity
+def _rotation(axis, angle):
+    return Rotation.from_rotvec(angle * np.asarray(axis, float) / np.linalg.norm(axis)).as_matrix()
+
+
+# Finite stretch with shear and a 1 % volume change, rotated off the axes.
+HELD_F = _rotation([1, 2, 3], 0.4) @ np.diag([1.08, 0.95, 0.985]) @ _rotation([0, 1, 1], -0.3)
+
+
+def maxwell_closed_form(F, mu, lam, mu_m, factor):
+    """First Piola stress of the standard linear solid at a held F when the Maxwell branch's
+    deviatoric Hencky strain has decayed to `factor` of its initial value: fixed corotated
+    2 mu_eq (F - R) + lam_eq (J - 1) J F^-T (mu_eq = mu - mu_m, lam_eq = lam + 2/3 mu_m), plus
+    the Kirchhoff stress 2 mu_m factor dev(log V) pulled back by F^-T (V = R-rotated stretch,
+    from numpy's SVD)."""
+    u, s, vt = np.linalg.svd(F)
+    R, J = u @ vt, np.prod(s)
+    inv_t = np.linalg.inv(F).T
+    equilibrium = 2 * (mu - mu_m) * (F - R) + (lam + 2 / 3 * mu_m) * (J - 1) * J * inv_t
+    e = np.log(s)
+    return equilibrium + u @ np.diag(2 * mu_m * factor * (e - e.mean())) @ u.T @ inv_t
+
+
+def _held(cfg, F, precision):
+    """Oracle with every particle at deformation F, registered, and no grid motion (zero grid
+    mass: the F update only relaxes the Maxwell branch)."""
+    o, _ = sim(cfg, precision)
+    o.set("F", np.tile(F, (o.n, 1, 1)))
+    o.stage("register")
+    o.set("grid_mass", np.zeros(8 * o.n))
+    o.stage("refresh_stress")
+    return o
+
+
+SMALL = Config(size=(0.02, 0.02, 0.02), grid_spacing=0.01)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("tau_steps", [0.01, 1.0, 40.0])
+def test_maxwell_held_deformation_relaxes_exactly(tau_steps, precision):
+    """A held deformation relaxes the Maxwell branch by exp(-dt / tau) per step exactly, for
+    any dt / tau (the reference's tests/test_maxwell.py:90-105); Fv stays isochoric and F
+    unchanged.  A relaxation of the full Hencky strain, of mu instead of mu_eq, or a missing
+    F^-T fails at step 0 or 1."""
+    cfg = replace(SMALL, maxwell_tau=tau_steps * SMALL.dt, maxwell_fraction=0.6)
+    k = oracle.constants(cfg)
+    o = _held(cfg, HELD_F, precision)
+    tol = 1e-12 if precision == 64 else 2e-5
+    for step in range(121):
+        if step:
+            o.stage("update_F")
+        if step in (0, 1, 5, 40, 120):
+            want = maxwell_closed_form(HELD_F, k["mu"], k["lam"], k["mu_maxwell"], np.exp(-step / tau_steps))
+            got = o.get("stress").reshape(-1, 3, 3)
+            np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), atol=tol * np.abs(want).max(),
+                                       err_msg=f"step {step}")
+    assert not o.get("errors").any()
+    np.testing.assert_allclose(o.get("F").reshape(-1, 3, 3), np.broadcast_to(HELD_F, (o.n, 3, 3)), atol=1e-7)
+    det = np.linalg.det(np.eye(3) + o.get("viscous").reshape(-1, 3, 3))
+    np.testing.assert_allclose(det, 1.0, atol=1e-12 if precision == 64 else 2e-6)
+
+
+def test_maxwell_slow_relaxation_is_resolved_in_float32():
+    """tau = 3.5e5 steps: the per-step change of Fv is below float32 resolution, and the
+    float64 Fv keeps the float32 mirror within 1 % of the relaxation (the reference's
+    tests/test_maxwell.py:108-119, where a float32 Fv was 39 % off)."""
+    tau_steps, steps = 3.5e5, 20000
+    cfg = replace(Config(size=(0.01, 0.01, 0.01), grid_spacing=0.005), maxwell_tau=tau_steps * SMALL.dt,
+                  maxwell_fraction=0.6)
+    k = oracle.constants(cfg)
+    o = _held(cfg, HELD_F, 32)
+    start = o.get("stress").reshape(-1, 3, 3)[0]
+    for _ in range(steps):
+        o.stage("update_F")
+    want = maxwell_closed_form(HELD_F, k["mu"], k["lam"], k["mu_maxwell"], np.exp(-steps / tau_steps))
+    error = np.abs(o.get("stress").reshape(-1, 3, 3)[0] - want).max()
+    assert error < 0.01 * np.abs(start - want).max()
+
+
+def test_maxwell_instantaneous_and_volumetric_response():
+    """The instantaneous small-strain response equals the elastic model's (mu_eq and lam_eq
+    are chosen for it); a pure volume change loads no Maxwell stress and does not relax
+    (tests/test_maxwell.py:122-136)."""
+    cfg = replace(SMALL, maxwell_tau=10 * SMALL.dt, maxwell_fraction=0.7)
+    small = np.eye(3) + 1e-3 * np.array([[0.5, 0.8, 0.0], [0.1, -0.3, 0.2], [0.0, 0.4, 0.6]])
+    elastic = _held(SMALL, small, 64).get("stress").reshape(-1, 3, 3)[0]
+    viscoelastic = _held(cfg, small, 64).get("stress").reshape(-1, 3, 3)[0]
+    np.testing.assert_allclose(viscoelastic, elastic, atol=3e-3 * np.abs(elastic).max())
+    o = _held(cfg, 0.98 * np.eye(3), 64)
+    first = o.get("stress").reshape(-1, 3, 3)[0].copy()
+    for _ in range(50):
+        o.stage("update_F")
+    scale = np.abs(first).max()
+    np.testing.assert_allclose(o.get("stress").reshape(-1, 3, 3)[0], first, atol=1e-12 * scale)
+    np.testing.assert_allclose(first, np.diag(np.diag(first)), atol=1e-12 * scale)
+    # At zero fraction or zero tau the branch is off: the stress is the elastic one.
+    for off in (replace(SMALL, maxwell_tau=0.0, maxwell_fraction=0.5), replace(SMALL, maxwell_tau=0.1)):
+        assert not oracle.constants(off)["maxwell"]
+        np.testing.assert_array_equal(_held(off, HELD_F, 64).get("stress"), _held(SMALL, HELD_F, 64).get("stress"))
+
+
+def _viscous_fixture(F):
+    """Registered oracle (float64) of a 4 x 3 x 2-cell block at uniform deformation F, with
+    the node position of every field slot."""
+    cfg = Config(size=(0.04, 0.03, 0.02), grid_spacing=0.01, dynamic_viscosity=2.0)
+    ref, origin, cells = points(cfg.size, cfg.grid_spacing, cfg.shape, cfg.sampling_seed)
+    nodes, weights, gradients = stencil(ref, origin, cfg.grid_spacing, cells)
+    dims = np.asarray(cells) + 1
+    N = int(np.prod(dims))
+    o = OracleSim(cfg, ref, nodes, weights, gradients, N, 64)
+    o.set("F", np.tile(F, (o.n, 1, 1)))
+    o.stage("register")
+    count = o.scalars()["active_count"]
+    node = o.get("grid_keys", 8 * o.n)[:count] % N
+    index = np.stack([node // (dims[1] * dims[2]), (node // dims[2]) % dims[1], node % dims[2]], 1)
+    return cfg, o, count, index, np.asarray(origin) + cfg.grid_spacing * index, np.asarray(cells)
+
+
+def _viscous_impulses(o, count, old_v):
+    full = np.zeros((8 * o.n, 3))
+    full[:count] = old_v
+    o.set("old_v", full)
+    o.set("grid_impulse", np.zeros(24 * o.n))
+    o.stage("viscous")
+    return o.get("grid_impulse").reshape(-1, 3)[:count]
+
+
+@pytest.mark.parametrize("deformed", [False, True])
+def test_viscous_impulse_face_tractions(deformed):
+    """viscous_impulse (material.py:321-340) of the node velocities v = G X + b (X the
+    reference node positions, so dF/dt = G) on a block at uniform F: the Newtonian Cauchy
+    stress 2 eta sym(L), L = G F^-1, pulled back to P = J sigma F^-T.  The divergence of a
+    uniform stress vanishes, so interior nodes get nothing, and the nodes of each face
+    together get -dt P N A (N its outward normal, A its reference area)."""
+    F = np.diag([1.1, 0.95, 1.02]) @ _rotation([1, -1, 2], 0.3) if deformed else np.eye(3)
+    cfg, o, count, index, X, cells = _viscous_fixture(F)
+    G = np.array([[0.3, 1.2, -0.4], [0.5, -0.2, 0.9], [0.1, 0.7, 0.25]])
+    imp = _viscous_impulses(o, count, X @ G.T + [0.1, -0.2, 0.3])
+    L = G @ np.linalg.inv(F)
+    P = cfg.dynamic_viscosity * np.linalg.det(F) * (L + L.T) @ np.linalg.inv(F).T
+    scale = cfg.dt * np.abs(P).max() * cfg.size[0] * cfg.size[1]
+    interior = np.all((index > 0) & (index < cells), axis=1)
+    assert interior.sum() > 0
+    np.testing.assert_allclose(imp[interior], 0.0, atol=1e-12 * scale)
+    for axis in range(3):
+        area = np.prod([cfg.size[k] for k in range(3) if k != axis])
+        for side, sign in ((0, -1.0), (cells[axis], 1.0)):
+            face = index[:, axis] == side
+            normal = np.zeros(3)
+            normal[axis] = sign
+            np.testing.assert_allclose(imp[face].sum(0), -cfg.dt * P @ normal * area, rtol=1e-10,
+                                       atol=1e-12 * scale)
+
+
+def test_viscous_impulse_of_a_rigid_spin_is_zero():
+    """A rigid spin of a deformed block (dF/dt = W F, W skew) has L = W: no viscous stress.
+    A velocity gradient taken as F^-1 G (a swapped product) is not skew for a stretched F and
+    fails."""
+    F = np.diag([1.2, 0.9, 1.05]) @ _rotation([0, 1, 1], 0.5)
+    cfg, o, count, index, X, cells = _viscous_fixture(F)
+    W = np.array([[0.0, -0.7, 0.4], [0.7, 0.0, -1.1], [-0.4, 1.1, 0.0]])
+    G = W @ F
+    imp = _viscous_impulses(o, count, X @ G.T)
+    moving = _viscous_impulses(o, count, X @ (np.diag([0.3, -0.2, 0.1]) @ F).T)
+    assert np.abs(moving).max() > 1e-9
+    np.testing.assert_allclose(imp, 0.0, atol=1e-12 * np.abs(moving).max())
+
+
+def test_viscous_material_step_conserves_momentum():
+    """A material step with viscosity (solver.py:433-458): the viscous impulses sum to zero
+    (the stencil gradients of a particle sum to zero), so the particles' momentum changes by
+    the contact impulses only (tests/test_mechanics.py:183-230, in float64)."""
+    cfg = Config(size=(0.04, 0.03, 0.02), grid_spacing=0.005, dynamic_viscosity=2.0)
+    o, ref = sim(cfg, 64)
+    rng = np.random.default_rng(4)
+    v = rng.normal(scale=0.01, size=(o.n, 3))
+    impulse = rng.normal(scale=1e-8, size=(o.n, 3))
+    o.set("v", v)
+    o.set("F", np.tile(np.diag([1.01, 0.995, 1.0]), (o.n, 1, 1)))
+    o.stage("refresh_stress")
+    o.set("codes", np.where(ref[:, 0] < 0, 1, 2))
+    o.set("impulses", impulse)
+    mass = oracle.constants(cfg)["mass"]
+    o.stage("material")
+    after = mass * o.get("v").reshape(-1, 3).sum(0)
+    np.testing.assert_allclose(after, mass * v.sum(0) + impulse.sum(0), rtol=0, atol=1e-15)
+    # and the viscosity acted: the same step without it differs
+    o2, _ = sim(replace(cfg, dynamic_viscosity=0.0), 64)
+    o2.set("v", v)
+    o2.set("F", np.tile(np.diag([1.01, 0.995, 1.0]), (o.n, 1, 1)))
+    o2.stage("refresh_stress")
+    o2.set("codes", np.where(ref[:, 0] < 0, 1, 2))
+    o2.set("impulses", impulse)
+    o2.stage("material")
+    assert np.abs(o2.get("v") - o.get("v")).max() > 1e-9
