@@ -42,6 +42,7 @@ class Params(ctypes.Structure):
         ("pair_mode", ctypes.c_int64),
         ("regular", ctypes.c_int64),
         ("maxwell", ctypes.c_int64),
+        ("tools", ctypes.c_int64),
     ] + [
         (name, ctypes.c_double)
         for name in (
@@ -90,6 +91,7 @@ STATE_FIELDS = [
     ("bc_node_mark", c_i32p), ("bc_grid", c_f32p), ("bc_coupled", c_f32p), ("bc_counts", c_i32p),
     ("bc_sums", c_f32p), ("history_mask", ctypes.POINTER(ctypes.c_uint64)),
     ("viscous", ctypes.POINTER(ctypes.c_double)), ("viscous_impulse", c_f32p),
+    ("extra_histories", c_f32p), ("extra_damage", c_f32p), ("extra_partial", c_f32p),
 ]
 
 

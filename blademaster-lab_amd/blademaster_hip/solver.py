@@ -78,8 +78,9 @@ class Solver:
         self.device = _Device(device)
         if not self.device.is_cuda or not torch.cuda.is_available():
             raise ValueError("CUDA device required")
-        if cfg.tool_count != 1:
-            raise ValueError("tool_count > 1 (several free knives) is not provided by this build")
+        self.tools = int(cfg.tool_count)
+        if _layout is not None and self.tools != 1:
+            raise ValueError("Batched instances carry one knife each")
         self._lib = _lib.load()
         dev = self.device.torch
         if _layout is None:
@@ -156,17 +157,20 @@ class Solver:
             T[name] = t((F_, N_, 3))
         hz = cfg.blade_half_size[2]
         I = self.instances
-        T["tool_pos"] = torch.tensor([[0.0, 0.0, cfg.size[2] + 0.012 + hz]] * I, dtype=f32, device=dev)
-        T["tool_vel"] = t((I, 3))
-        T["tool_rot"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]] * I, dtype=f32, device=dev)
-        T["tool_omega"] = t((I, 3))
+        # One row per batched instance, or per knife of a single solver (knife k > 0 is an
+        # extra free knife, contact.py:500-566).
+        K = self.tools if _layout is None else I
+        T["tool_pos"] = torch.tensor([[0.0, 0.0, cfg.size[2] + 0.012 + hz]] * K, dtype=f32, device=dev)
+        T["tool_vel"] = t((K, 3))
+        T["tool_rot"] = torch.tensor([[0.0, 0.0, 0.0, 1.0]] * K, dtype=f32, device=dev)
+        T["tool_omega"] = t((K, 3))
         T["tool_mobility"] = t((6, 6)) if I == 1 else t((I, 6, 6))
-        T["reaction"] = t((I, 3))
-        T["torque"] = t((I, 3))
+        T["reaction"] = t((K, 3))
+        T["torque"] = t((K, 3))
         T["board"] = t((I, 3))
         T["work"] = t((I,))
         T["speed"] = t((1,))
-        T["table"] = t((1, 3))
+        T["table"] = t((K, 3))
         T["interval_impulse"] = t((I, 3))
         T["interval_angular"] = t((I, 3))
         if _layout is not None:
@@ -235,6 +239,11 @@ class Solver:
         T["remap_partial"] = t((max(nb * F_ * window * 3, 1),))
         self.rigid_blocks = max(1, -(-n // 256))
         T["rigid_partial"] = t((self.rigid_blocks * 12,))
+        if self.tools > 1:
+            # Knives 1 .. K-1: friction history row, damage and per-block contact sums of each.
+            T["extra_histories"] = t((self.tools - 1, n, 3))
+            T["extra_damage"] = t((self.tools - 1, n))
+            T["extra_partial"] = t(((self.tools - 1) * self.rigid_blocks * 8,))
         T["counters"] = t((4,), i32)
         # Held-blade grid constraint (blade_constraint.py): allocated for an externally held tool.
         self.blade_hold = bool(cfg.external_tool and cfg.tool_substep == "hold")
@@ -266,7 +275,9 @@ class Solver:
         self.codes = arr(T["codes"], mark_topology)
         self.single_field_codes = arr(T["single_codes"], mark_topology)
         self.damage = arr(T["damage"])
-        self.tool_damage = [self.damage]
+        # Damage of each particle against knife k (knife 0: `damage`).
+        self.tool_damage = [self.damage] + [arr(T["extra_damage"][k]) for k in range(self.tools - 1)]
+        self.extra_histories = arr(T["extra_histories"]) if self.tools > 1 else None
         self.impulses = arr(T["impulses"])
         # Host writes to the histories: every particle's history may now be nonzero.
         self.histories = arr(T["histories"], lambda: T["history_mask"].fill_(-1))
@@ -295,9 +306,15 @@ class Solver:
         self.cut_index = 0
         self._batches = {}
         self.graph_build_s = 0.0
+        # Knife 0 uses the full arrays; knife k > 0 row views (solver.py:288-300).
         self.tool_views = [
             dict(pos=self.tool_pos, vel=self.tool_vel, rot=self.tool_rot, omega=self.tool_omega,
                  reaction=self.reaction, torque=self.torque, table=self.tool_table_impulse)
+        ] + [
+            dict(pos=self.tool_pos[k:k + 1], vel=self.tool_vel[k:k + 1], rot=self.tool_rot[k:k + 1],
+                 omega=self.tool_omega[k:k + 1], reaction=self.reaction[k:k + 1], torque=self.torque[k:k + 1],
+                 table=self.tool_table_impulse[k:k + 1])
+            for k in range(1, self.tools if _layout is None else 1)
         ]
         self._params = self._make_params()
         self._state = self._make_state()
@@ -346,6 +363,7 @@ class Solver:
         p.regular = int(self.regular)
         p.pair_mode = PAIR_MODES[cfg.pair_search]
         p.maxwell = int(self.maxwell)
+        p.tools = self.tools
         p.mu_maxwell, p.maxwell_relax = self.mu_maxwell, self.maxwell_relax
         p.viscosity = cfg.dynamic_viscosity
         return p
@@ -438,7 +456,7 @@ class Solver:
         self._call("bm_pair_contacts", self.step_count % 2)
 
     def assign_sides(self, cut_index, cutting=True):
-        """Side digit 3^cut_index for particles crossed by the blade edge (solver.py:866-890)."""
+        """Side digit 3^(cut_index + k) for particles crossed by knife k's edge (solver.py:866-890)."""
         self._call("bm_assign", 3**cut_index, int(cutting))
 
     def step(self, target_edge=(0.0, 0.0, 0.042), cutting=True, cut_index=0, orientation=None):
@@ -592,6 +610,8 @@ class Solver:
         }
         if self.maxwell:
             state["viscous_deformation"] = np.eye(3) + self.viscous.numpy()
+        if self.tools > 1:
+            state["tool_damage"] = np.stack([damage.numpy() for damage in self.tool_damage])
         state["time"] = self.step_count * self.config.dt
         if any(not np.isfinite(v).all() for v in state.values()):
             raise RuntimeError("Nonfinite state; run is invalid")

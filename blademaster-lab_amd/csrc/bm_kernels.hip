@@ -255,7 +255,7 @@ struct ToolTarget {
 };
 
 __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float* targets, const float* rotations,
-                              int index);
+                              int index, int k = 0);
 __device__ void finish_external_dev(const bm_params& P, const bm_state& S);
 __device__ void accumulate_reaction_dev(const bm_state& S);
 
@@ -535,6 +535,99 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   }
 }
 
+// ------------------------------------------------------------------ extra knives
+constexpr int kExtraSums = 8;  // reaction 3, torque 3, work, pad
+
+// blade_at (contact.py:235-329) of knife k >= 1 on particle p (extra_blade, contact.py:500-566):
+// the friction history row and the damage are knife k's own; the reaction, torque and work go
+// to the caller's sums.  Returns the impulse on the particle.
+__device__ inline f3 extra_knife_contact(const bm_params& P, const bm_state& S, const Contact& c, int64_t p, f3 xp,
+                                         f3 vp, int k, int cut_bits, f3& react, f3& torq, float& work) {
+  const float dt = c.dt;
+  const f3 zero = f3{0.f, 0.f, 0.f};
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  quat q = ldq(S.tool_rot + 4 * k);
+  f3 origin = ld3(S.tool_pos, k), tvel = ld3(S.tool_vel, k), tom = ld3(S.tool_omega, k);
+  float* hist = S.extra_histories + 3 * int64_t(k - 1) * P.n;
+  float* dmg_row = S.extra_damage + int64_t(k - 1) * P.n;
+  float blade_radius = len(hf);
+  float max_closing = len(vp) + len(tvel) + len(tom) * blade_radius;
+  float offset = float(P.blade_offset);
+  if (len(xp - origin) - blade_radius - offset > dt * max_closing) {
+    st3(hist, p, zero);
+    return zero;
+  }
+  f3 local = rotate_inv(q, xp - origin);
+  f3 closest, normal;
+  float distance;
+  blade_query(local, hf, float(P.bevel), closest, normal, distance);
+  float front = -hf.z - local.z;
+  float dmg = dmg_row[p];
+  if ((cut_bits & BM_CUT_HOLD) && dmg < 1.0f && fabsf(local.x) <= hf.x + offset && fabsf(local.y) <= hf.y &&
+      front >= 0.0f) {
+    closest = f3{local.x, local.y, -hf.z};
+    normal = f3{0.f, 0.f, -1.f};
+    distance = front;
+  }
+  f3 lever = rotate(q, closest);
+  f3 n = rotate(q, normal);
+  f3 relative = vp - tvel - cross(tom, lever);
+  if (distance - offset + dt * fminf(dot(relative, n), 0.0f) >= 0.0f) {
+    st3(hist, p, zero);
+    return zero;
+  }
+  float me = effective_mass(P, S.mobility, lever, n, q);
+  f3 j, h;
+  penalty(c, distance - offset, relative, n, me, ld3(hist, p), float(P.sn), j, h);
+  st3(hist, p, h);
+  if (!(dot(j, j) > 0.0f)) return zero;
+  if (cut_bits != 0) {
+    float d = dmg, w = 0.0f;
+    j = resist_cut(j, xp, vp, origin, tvel, q, tom, hf, offset, float(P.spacing), float(P.threshold), cut_bits, d, w);
+    if (d != dmg) dmg_row[p] = d;
+    work += w;
+  }
+  react = react - j;
+  torq = torq - cross(lever, j);
+  return j;
+}
+
+// extra_blade (contact.py:500-566) for knives 1 .. tools-1, after k_rigid: each knife's impulse is
+// added to the particle's in knife order, and each knife's block sums go to extra_partial.
+__global__ void __launch_bounds__(kBlock) k_extra_blades(const bm_params P, bm_state S, int cut_bits) {
+  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = p < P.n;
+  Contact c{float(P.dt), float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
+  f3 xp = f3{0.f, 0.f, 0.f}, vp = xp, imp = xp;
+  if (live) {
+    xp = ld3(S.x, p);
+    vp = ld3(S.v, p);
+    imp = ld3(S.impulses, p);
+  }
+  __shared__ float wsum[kBlock / 64][kExtraSums];
+  const int wave = int(threadIdx.x >> 6);
+  for (int k = 1; k < int(P.tools); ++k) {
+    f3 react = f3{0.f, 0.f, 0.f}, torq = react;
+    float work = 0.0f;
+    if (live) imp = imp + extra_knife_contact(P, S, c, p, xp, vp, k, cut_bits, react, torq, work);
+    float v[kExtraSums] = {react.x, react.y, react.z, torq.x, torq.y, torq.z, work, 0.0f};
+#pragma unroll
+    for (int i = 0; i < kExtraSums - 1; ++i) v[i] = wave_sum(v[i]);
+    if (lane0()) {
+#pragma unroll
+      for (int i = 0; i < kExtraSums; ++i) wsum[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < kExtraSums) {
+      float acc = wsum[0][threadIdx.x];
+      for (int w = 1; w < kBlock / 64; ++w) acc += wsum[w][threadIdx.x];
+      S.extra_partial[(int64_t(k - 1) * gridDim.x + blockIdx.x) * kExtraSums + threadIdx.x] = acc;
+    }
+    __syncthreads();
+  }
+  if (live) st3(S.impulses, p, imp);
+}
+
 __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
 __device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
 
@@ -733,12 +826,55 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
   if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
 }
 
-__global__ void k_clear_step(bm_state S) {
-  for (int i = 0; i < 3; ++i) {
+// The step sums of knives 1 .. tools-1 from extra_partial (nblocks blocks each, in block order):
+// stored as their reaction and torque (added with kAccumulate), their work added to the shared
+// work; with kToolStep each of them then takes its tool.step (tool.py:146-259).  One workgroup.
+__global__ void __launch_bounds__(kPrepBlock) k_extra_tools(const bm_params P, bm_state S, int nblocks, int mode,
+                                                            ToolTarget tt) {
+  __shared__ float part[kPrepBlock / 64][kExtraSums];
+  for (int k = 1; k < int(P.tools); ++k) {
+    float acc[kExtraSums - 1];
+#pragma unroll
+    for (int i = 0; i < kExtraSums - 1; ++i) acc[i] = 0.0f;
+    for (int b = int(threadIdx.x); b < nblocks; b += int(blockDim.x)) {
+      const float* src = S.extra_partial + (int64_t(k - 1) * nblocks + b) * kExtraSums;
+#pragma unroll
+      for (int i = 0; i < kExtraSums - 1; ++i) acc[i] += src[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kExtraSums - 1; ++i) acc[i] = wave_sum(acc[i]);
+    if (lane0()) {
+#pragma unroll
+      for (int i = 0; i < kExtraSums - 1; ++i) part[threadIdx.x >> 6][i] = acc[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t[kExtraSums - 1];
+      for (int i = 0; i < kExtraSums - 1; ++i) {
+        t[i] = part[0][i];
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) t[i] += part[w][i];
+      }
+      f3 react = f3{t[0], t[1], t[2]}, torq = f3{t[3], t[4], t[5]};
+      if (!(mode & kOverwrite)) {
+        react = react + ld3(S.reaction, k);
+        torq = torq + ld3(S.torque, k);
+      }
+      st3(S.reaction, k, react);
+      st3(S.torque, k, torq);
+      S.work[0] += t[6];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && (mode & kToolStep))
+    for (int k = 1; k < int(P.tools); ++k) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index, k);
+}
+
+__global__ void k_clear_step(const bm_params P, bm_state S) {
+  for (int i = 0; i < 3 * max(int(P.tools), 1); ++i) {
     S.reaction[i] = 0.0f;
     S.torque[i] = 0.0f;
-    S.board[i] = 0.0f;
   }
+  for (int i = 0; i < 3; ++i) S.board[i] = 0.0f;
   S.speed[0] = 0.0f;
 }
 
@@ -1748,7 +1884,8 @@ __global__ void __launch_bounds__(kBlock) k_viscous_couple(const bm_params P, bm
   for (int f = 0; f < nf; ++f) st3(S.viscous_impulse, int64_t(f) * N + node, f3{0.f, 0.f, 0.f});
 }
 
-__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp);
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp,
+                           int knife = 0);
 
 // g2p_remap (material.py:384-401) for one brick: each particle lane gathers the old and new
 // velocities of its 8 corners, updates v and x, and re-scatters m w v through the same
@@ -2425,12 +2562,12 @@ __device__ __forceinline__ f3 blade_vertex(int i, f3 hf, float bevel) {
 // tool.step (tool.py:146-259): drive, gyroscopic term, 6 sweeps x 10 vertices of the
 // knife/board velocity constraint with Coulomb friction, integration.
 __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float* targets, const float* rotations,
-                              int index) {
+                              int index, int k) {
   float dt = float(P.dt), mass = float(P.blade_mass), bevel = float(P.bevel);
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
-  f3 x = ld3(S.tool_pos, 0);
-  quat q = qnormalize(ldq(S.tool_rot));
-  f3 v = ld3(S.tool_vel, 0), w = ld3(S.tool_omega, 0);
+  f3 x = ld3(S.tool_pos, k);
+  quat q = qnormalize(ldq(S.tool_rot + 4 * k));
+  f3 v = ld3(S.tool_vel, k), w = ld3(S.tool_omega, k);
   f3 force = f3{0.f, 0.f, -9.81f * mass};
   f3 servo = f3{0.f, 0.f, 0.f};
   if (P.free_blade == 0) {
@@ -2447,7 +2584,7 @@ __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float
     if (s > 1.0e-8f) error = vec * (2.0f * atan2f(s, dq.w) / s);
     servo = float(P.rot_k) * error - float(P.rot_c) * w;
   }
-  v = v + (force * dt + ld3(S.reaction, 0)) / mass;
+  v = v + (force * dt + ld3(S.reaction, k)) / mass;
   f3 ii = f3{float(P.inertia_inv[0]), float(P.inertia_inv[1]), float(P.inertia_inv[2])};
   if (P.lock_rotation) {
     ii = f3{0.f, 0.f, 0.f};
@@ -2456,7 +2593,7 @@ __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float
     f3 lw = rotate_inv(q, w);
     f3 inertia = f3{1.0f / ii.x, 1.0f / ii.y, 1.0f / ii.z};
     f3 gyro = rotate(q, cross(lw, f3{inertia.x * lw.x, inertia.y * lw.y, inertia.z * lw.z}));
-    w = w + inv_inertia(q, ii, ld3(S.torque, 0) + dt * (servo - gyro));
+    w = w + inv_inertia(q, ii, ld3(S.torque, k) + dt * (servo - gyro));
   }
   f3 normal = f3{0.f, 0.f, 1.f};
   f3 total = f3{0.f, 0.f, 0.f};
@@ -2497,17 +2634,18 @@ __device__ void tool_step_dev(const bm_params& P, const bm_state& S, const float
   x = x + dt * v;
   float sp = len(w);
   if (sp > 1.0e-10f) q = qnormalize(qmul(axis_angle(w / sp, dt * sp), q));
-  st3(S.tool_pos, 0, x);
-  st3(S.tool_vel, 0, v);
-  S.tool_rot[0] = q.x;
-  S.tool_rot[1] = q.y;
-  S.tool_rot[2] = q.z;
-  S.tool_rot[3] = q.w;
-  st3(S.tool_omega, 0, w);
-  st3(S.table, 0, total);
+  st3(S.tool_pos, k, x);
+  st3(S.tool_vel, k, v);
+  S.tool_rot[4 * k] = q.x;
+  S.tool_rot[4 * k + 1] = q.y;
+  S.tool_rot[4 * k + 2] = q.z;
+  S.tool_rot[4 * k + 3] = q.w;
+  st3(S.tool_omega, k, w);
+  st3(S.table, k, total);
 }
+// Every knife steps with the same target (free knives ignore it).
 __global__ void k_tool_step(const bm_params P, bm_state S, const float* targets, const float* rotations, int index) {
-  tool_step_dev(P, S, targets, rotations, index);
+  for (int k = 0; k < max(int(P.tools), 1); ++k) tool_step_dev(P, S, targets, rotations, index, k);
 }
 
 // finish_external_step (tool.py:70-115)
@@ -2609,11 +2747,12 @@ __device__ void accumulate_reaction_dev(const bm_state& S) {
 __global__ void k_accumulate_reaction(bm_state S) { accumulate_reaction_dev(S); }
 
 // assign (contact.py:996-1039): side digit for a particle behind the edge band or pinched
-// on the board under the edge.
-__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp) {
+// on the board under the edge of knife `knife` (batched instances: the knife of its instance).
+__device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int64_t digit, f3 xp, f3 vp,
+                           int knife) {
   // The band test below needs only the particle's x and v and the knife, so it comes first;
   // the code (an int64 load and division) is read only for particles inside the band.
-  int inst = instance_of(S, p);
+  int inst = instance_of(S, p) + knife;
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
   quat q = ldq(S.tool_rot + 4 * inst);
   f3 pos = ld3(S.tool_pos, inst);
@@ -2654,10 +2793,14 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting) {
+// assign_sides (solver.py:866-890) for knives first .. tools-1: knife k labels digit * 3^k.
+__global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting,
+                                                   int first) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n || cutting == 0) return;
-  assign_dev(P, S, p, digit, ld3(S.x, p), ld3(S.v, p));
+  f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
+  for (int k = 0; k < max(int(P.tools), 1); ++k, digit *= 3)
+    if (k >= first) assign_dev(P, S, p, digit, xp, vp, k);
 }
 
 // ------------------------------------------------------------------------ port kernels
@@ -2830,6 +2973,10 @@ inline int launch_status() {
 
 bool state_ok(const bm_params* p, const bm_state* s) {
   if (p == nullptr || s == nullptr || p->n < 0 || s->x == nullptr || s->v == nullptr) return false;
+  // Several knives: free knives of a single solver, with their own histories, damage and sums.
+  if (p->tools > 1 && (s->extra_histories == nullptr || s->extra_damage == nullptr || s->extra_partial == nullptr ||
+                       s->instance != nullptr || p->tool_model < 0))
+    return false;
   // The Maxwell branch needs its Fv - I; viscosity its grid impulses.
   if ((p->maxwell && s->viscous == nullptr) || (p->viscosity > 0.0 && s->viscous_impulse == nullptr)) return false;
   // Batched instances carry their Lame constants and the slots of their contact sums (and
@@ -2922,7 +3069,7 @@ int bm_refresh_stress(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_clear_step(const bm_params* p, const bm_state* s, void* stream) {
   if (!state_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_clear_step, dim3(1), dim3(1), 0, (hipStream_t)stream, *s);
+  hipLaunchKernelGGL(k_clear_step, dim3(1), dim3(1), 0, (hipStream_t)stream, *p, *s);
   return launch_status();
 }
 
@@ -2933,8 +3080,13 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   int nb = blocks_for(p->n);
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
                      int(kAccumulate), 0);
+  if (p->tools > 1)
+    hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
                      ToolTarget{nullptr, nullptr, 0}, 0, 0);
+  if (p->tools > 1)
+    hipLaunchKernelGGL(k_extra_tools, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb,
+                       int(kAccumulate), ToolTarget{nullptr, nullptr, 0});
   return launch_status();
 }
 
@@ -3048,6 +3200,10 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   // Held blade: constrain new_v before it moves particles (solver.py:476-477).
   if (p->blade_hold) bm_blade_constraint(p, s, st);
   hipLaunchKernelGGL(k_g2p, dim3(nb), dim3(tb), lds_bytes(p, 3, 3), st, *p, *s, digit, g2p_flags, D);
+  // Knives 1 .. tools-1 label their digits (G2P labelled knife 0's) before the node kernel's
+  // deferred update reads this step's relabelling.
+  if (p->tools > 1 && (g2p_flags & 1))
+    hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, digit, 1, 1);
   const int node_blocks = int(blocks_for(node_launch(p)));
   defer = defer && s->update_pending && s->update_done;
   if (defer && s->instance == nullptr) {
@@ -3160,7 +3316,7 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
     return -1;
   if (p->n == 0 || !cutting) return 0;
   hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, digit,
-                     int(cutting));
+                     int(cutting), 0);
   return launch_status();
 }
 
@@ -3194,8 +3350,14 @@ static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int3
   int mark_blocks = int((p->changed_capacity + kPrepBlock / 64 - 1) / (kPrepBlock / 64));
   mark_blocks = mark_blocks < 16 ? mark_blocks : 16;
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
+  // Knives 1 .. tools-1: their contacts after knife 0's, their sums and tool steps after k_prep.
+  if (p->tools > 1)
+    hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0);
   hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
                      ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);
+  if (p->tools > 1)
+    hipLaunchKernelGGL(k_extra_tools, dim3(1), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
+                       ToolTarget{targets, rotations, int(index)});
   // 3. particle pairs across cuts (partner lists rebuilt first when stale)
   hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p, 1);
   // 4. transfers; G2P labels the moved particles and checks the pair list's skin

@@ -68,6 +68,9 @@ typedef struct bm_params {
   int64_t maxwell;        /* Maxwell branch on (maxwell_tau > 0 and maxwell_fraction > 0,
                              solver.py:189-191): the stress is that of maxwell_at (material.py:
                              485-535) and every F update relaxes the branch */
+  int64_t tools;          /* knives (tool_count; 0 or 1: one).  Knife k > 0 is a free knife with its
+                             own friction history row and damage (extra_blade, contact.py:500-566)
+                             and labels digit 3^(cut index + k); the tool arrays hold k rows */
   double dt, spacing, mass, volume, radius, blade_offset, mu, lam, yield_stress, flip;
   double sn, gk, mus, mud, gc, board_sn, threshold;
   double half[3], bevel, blade_mass, inertia_inv[3], board_half[2];
@@ -167,6 +170,12 @@ typedef struct bm_state {
   /* [F][N][3] viscous grid impulses of a substep; all zero between substeps (the node
    * kernel that applies them clears them), used when viscosity > 0 */
   float *viscous_impulse;
+  /* knives k = 1 .. tools-1 (used when tools > 1): friction history [tools-1][n][3], damage
+   * [tools-1][n] and per-block contact sums [tools-1][rigid_blocks][8] (reaction 3, torque 3,
+   * work, pad) */
+  float *extra_histories;
+  float *extra_damage;
+  float *extra_partial;
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */

@@ -22,6 +22,8 @@ REAL_ARRAYS = {
     "tool_mobility": None, "reaction": None, "torque": None, "board": None, "work": None, "speed": None,
     "tool_table_impulse": None, "interval_impulse": None, "interval_angular_impulse": None,
     "bc_side": 1, "bc_impulse": 1, "bc_normal": 3, "viscous": 9,
+    # knives 1 .. K-1: one row of n entries each (sized in OracleSim._size)
+    "extra_histories": None, "extra_damage": None,
     # grid arrays: one entry per (particle, corner) field slot at most
     "grid_mass": 8, "old_v": 24, "new_v": 24, "grid_impulse": 24, "momentum": 24, "coupled_remapped": 24,
 }
@@ -30,6 +32,11 @@ FIXED = {
     "reaction": 3, "torque": 3, "board": 3, "work": 1, "speed": 1, "tool_table_impulse": 3, "interval_impulse": 3,
     "interval_angular_impulse": 3, "bc_max_violation": 1,
 }
+
+
+# Arrays with one entry per knife (tool_count of them).
+PER_KNIFE = {"tool_pos": 3, "tool_vel": 3, "tool_rot": 4, "tool_omega": 3, "reaction": 3, "torque": 3,
+             "tool_table_impulse": 3}
 
 
 class OracleConfig(ctypes.Structure):
@@ -47,7 +54,7 @@ class OracleConfig(ctypes.Structure):
         ("predict", ctypes.c_int32), ("accumulate_tool_impulses", ctypes.c_int32), ("cache_topology", ctypes.c_int32),
         ("cut_work_advance", ctypes.c_int32), ("cut_hold", ctypes.c_int32),
         ("viscosity", ctypes.c_double), ("mu_maxwell", ctypes.c_double), ("maxwell_relax", ctypes.c_double),
-        ("maxwell", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("maxwell", ctypes.c_int32), ("tool_count", ctypes.c_int32),
     ]
 
 
@@ -152,6 +159,7 @@ class OracleSim:
         c.viscosity = cfg.dynamic_viscosity
         c.maxwell = int(k["maxwell"])
         c.mu_maxwell, c.maxwell_relax = k["mu_maxwell"], k["maxwell_relax"]
+        c.tool_count = self.knives = int(cfg.tool_count)
         self.c = c
         self.precision = precision
         self.handle = self.lib.oracle_create(precision, ctypes.byref(c))
@@ -161,7 +169,7 @@ class OracleSim:
         self.set("weights", weights)
         self.set("gradients", gradients)
         hz = cfg.blade_half_size[2]
-        self.set("tool_pos", [0.0, 0.0, cfg.size[2] + 0.012 + hz])
+        self.set("tool_pos", [[0.0, 0.0, cfg.size[2] + 0.012 + hz]] * self.knives)
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -178,6 +186,12 @@ class OracleSim:
             return 3
         if name == "scalars":
             return 5
+        if name in PER_KNIFE:
+            return PER_KNIFE[name] * self.knives
+        if name == "extra_histories":
+            return 3 * n * (self.knives - 1)
+        if name == "extra_damage":
+            return n * (self.knives - 1)
         if name in FIXED:
             f = FIXED[name]
             return f(n) if callable(f) else f

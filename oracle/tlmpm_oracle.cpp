@@ -331,7 +331,7 @@ struct OracleConfig {
   int32_t free_blade, pose_servo, external_tool, predict, accumulate_tool_impulses, cache_topology;
   int32_t cut_work_advance, cut_hold;
   double viscosity, mu_maxwell, maxwell_relax;  // dynamic viscosity; Maxwell branch constants
-  int32_t maxwell, pad;
+  int32_t maxwell, tool_count;                  // tool_count knives (> 1: free knives only)
 };
 
 template <class R> struct Sim {
@@ -341,7 +341,12 @@ template <class R> struct Sim {
   std::vector<double> viscous;  // Fv - I of the Maxwell branch, float64 at either precision
   std::vector<int64_t> codes, field_codes_override;
   std::vector<int32_t> nodes;
+  // Knife k's pose, twist, step reaction and table impulse at [3k .. 3k + 2] ([4k .. 4k + 3]).
   std::vector<R> tool_pos, tool_vel, tool_rot, tool_omega, mobility, reaction, torque, board, work, speed, table;
+  // Knives k >= 1 (extra_blade, contact.py:500-566): friction history row and damage of each
+  // particle, at [(k - 1) n + p].
+  std::vector<R> extra_histories, extra_damage;
+  int64_t knives = 1;
   std::vector<R> interval_impulse, interval_angular;
   std::vector<int32_t> errors;
   // Registration / grid (compact path).
@@ -369,9 +374,13 @@ template <class R> struct Sim {
     viscous.assign(9 * n, 0.0);
     weights.assign(8 * n, R(0)); gradients.assign(24 * n, R(0));
     impulses.assign(3 * n, R(0)); histories.assign(6 * n, R(0));
-    tool_pos.assign(3, R(0)); tool_vel.assign(3, R(0)); tool_rot = {R(0), R(0), R(0), R(1)};
-    tool_omega.assign(3, R(0)); mobility.assign(36, R(0)); reaction.assign(3, R(0)); torque.assign(3, R(0));
-    board.assign(3, R(0)); work.assign(1, R(0)); speed.assign(1, R(0)); table.assign(3, R(0));
+    knives = std::max<int64_t>(1, cfg.tool_count);
+    tool_pos.assign(3 * knives, R(0)); tool_vel.assign(3 * knives, R(0)); tool_rot.assign(4 * knives, R(0));
+    for (int64_t k = 0; k < knives; ++k) tool_rot[4 * k + 3] = R(1);
+    tool_omega.assign(3 * knives, R(0)); mobility.assign(36, R(0)); reaction.assign(3 * knives, R(0));
+    torque.assign(3 * knives, R(0));
+    board.assign(3, R(0)); work.assign(1, R(0)); speed.assign(1, R(0)); table.assign(3 * knives, R(0));
+    extra_histories.assign(3 * n * (knives - 1), R(0)); extra_damage.assign(n * (knives - 1), R(0));
     interval_impulse.assign(3, R(0)); interval_angular.assign(3, R(0));
     errors.assign(3, 0);
     keys.assign(cfg.key_capacity, -1);
@@ -396,7 +405,7 @@ template <class R> struct Sim {
   void s9(std::vector<R>& a, int64_t p, const M3<R>& m) {
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[9 * p + 3 * i + j] = m.m[i][j];
   }
-  Q<R> rot() const { return {tool_rot[0], tool_rot[1], tool_rot[2], tool_rot[3]}; }
+  Q<R> rot(int64_t k = 0) const { return {tool_rot[4 * k], tool_rot[4 * k + 1], tool_rot[4 * k + 2], tool_rot[4 * k + 3]}; }
   V3<R> half() const { return {R(c.half[0]), R(c.half[1]), R(c.half[2])}; }
 
   // ------------------------------------------------------------------ contact helpers
@@ -518,8 +527,12 @@ template <class R> struct Sim {
     return R(1) / (R(1) / R(c.mass) + inv);
   }
 
-  // resist_cut (contact.py:181-232); `rad` is the blade contact offset.
+  // resist_cut (contact.py:181-232); `rad` is the blade contact offset, `dmg` the damage of
+  // particle p against this knife.
   V3<R> resist_cut(int64_t p, V3<R> j, V3<R> origin, V3<R> tvel, Q<R> q, V3<R> tomega, R rad, int cutting) {
+    return resist_cut(p, j, origin, tvel, q, tomega, rad, cutting, damage[p]);
+  }
+  V3<R> resist_cut(int64_t p, V3<R> j, V3<R> origin, V3<R> tvel, Q<R> q, V3<R> tomega, R rad, int cutting, R& dmg) {
     V3<R> hf = half();
     V3<R> edge = origin + rotate(q, V3<R>{R(0), R(0), -hf.z});
     V3<R> down = rotate(q, V3<R>{R(0), R(0), R(-1)});
@@ -533,28 +546,33 @@ template <class R> struct Sim {
                  std::fabs(local_edge.y) <= hf.y;
     if (!ahead) return j;
     R push = std::max(dot(j, down), R(0));
-    if (damage[p] >= R(1)) return j - push * down;
+    if (dmg >= R(1)) return j - push * down;
     if (advance > R(0)) {
       R delta = push * advance;
       work[0] += delta;
-      damage[p] = std::min(R(1), damage[p] + delta / R(c.threshold));
+      dmg = std::min(R(1), dmg + delta / R(c.threshold));
     }
     return j;
   }
 
-  // blade_at (contact.py:235-329), knife 0, friction history row 1.
-  V3<R> blade_at(int64_t p, int tool_model, int cutting) {
+  // blade_at (contact.py:235-329) of knife k: knife 0 keeps its friction history in row 1 of
+  // `histories` and its damage in `damage`; knife k >= 1 (extra_blade, contact.py:500-566) in
+  // extra_histories and extra_damage.  The reaction and torque go to knife k's slots.
+  V3<R> blade_at(int64_t p, int tool_model, int cutting, int64_t k = 0) {
     V3<R> hf = half();
     R dt = R(c.dt);
     V3<R> xp = g3(x, p), vp = g3(v, p);
-    Q<R> q = rot();
-    V3<R> origin = g3(tool_pos, 0), tvel = g3(tool_vel, 0), tom = g3(tool_omega, 0);
+    Q<R> q = rot(k);
+    V3<R> origin = g3(tool_pos, k), tvel = g3(tool_vel, k), tom = g3(tool_omega, k);
+    std::vector<R>& hist = k == 0 ? histories : extra_histories;
+    const int64_t row = k == 0 ? n + p : (k - 1) * n + p;
+    R& dmg = k == 0 ? damage[p] : extra_damage[(k - 1) * n + p];
     V3<R> result = {R(0), R(0), R(0)};
     R blade_radius = length(hf);
     R max_closing = length(vp) + length(tvel) + length(tom) * blade_radius;
     R offset = R(c.blade_offset);
     if (length(xp - origin) - blade_radius - offset > dt * max_closing) {
-      s3(histories, n + p, {R(0), R(0), R(0)});
+      s3(hist, row, {R(0), R(0), R(0)});
       return result;
     }
     V3<R> local = rotate_inv(q, xp - origin);
@@ -562,7 +580,7 @@ template <class R> struct Sim {
     R distance;
     blade_query(local, hf, R(c.bevel), closest, normal, distance);
     R front = -hf.z - local.z;
-    if ((cutting & CUT_HOLD) && damage[p] < R(1) && std::fabs(local.x) <= hf.x + offset && std::fabs(local.y) <= hf.y &&
+    if ((cutting & CUT_HOLD) && dmg < R(1) && std::fabs(local.x) <= hf.x + offset && std::fabs(local.y) <= hf.y &&
         front >= R(0)) {
       closest = {local.x, local.y, -hf.z};
       normal = {R(0), R(0), R(-1)};
@@ -572,18 +590,18 @@ template <class R> struct Sim {
     V3<R> nrm = rotate(q, normal);
     V3<R> relative = vp - tvel - cross(tom, lever);
     if (distance - offset + dt * std::min(dot(relative, nrm), R(0)) >= R(0)) {
-      s3(histories, n + p, {R(0), R(0), R(0)});
+      s3(hist, row, {R(0), R(0), R(0)});
       return result;
     }
     R me = effective_mass(lever, nrm, q, tool_model);
     V3<R> j, h;
-    penalty(distance - offset, relative, nrm, me, g3(histories, n + p), R(c.sn), j, h);
-    s3(histories, n + p, h);
+    penalty(distance - offset, relative, nrm, me, g3(hist, row), R(c.sn), j, h);
+    s3(hist, row, h);
     if (dot(j, j) > R(0)) {
-      if (cutting != 0) j = resist_cut(p, j, origin, tvel, q, tom, offset, cutting);
+      if (cutting != 0) j = resist_cut(p, j, origin, tvel, q, tom, offset, cutting, dmg);
       result = j;
-      add3(reaction, 0, -j);
-      add3(torque, 0, -cross(lever, j));
+      add3(reaction, k, -j);
+      add3(torque, k, -cross(lever, j));
     }
     return result;
   }
@@ -608,6 +626,9 @@ template <class R> struct Sim {
       total = total + blade_at(p, tool_model, cutting);
       s3(impulses, p, total);
     }
+    // extra_blade (contact.py:500-566) for knives 1 .. K-1, after all of knife 0.
+    for (int64_t k = 1; k < knives; ++k)
+      for (int64_t p = 0; p < n; ++p) add3(impulses, p, blade_at(p, tool_model, cutting, k));
   }
 
   // pair_at (contact.py:584-640)
@@ -1167,13 +1188,13 @@ template <class R> struct Sim {
   }
 
   // tool.step (tool.py:146-259)
-  void tool_step(V3<R> target, Q<R> target_q) {
+  void tool_step(V3<R> target, Q<R> target_q, int64_t k = 0) {
     R dt = R(c.dt), mass = R(c.blade_mass);
     V3<R> hf = half();
     R bevel = R(c.bevel);
-    V3<R> xt = g3(tool_pos, 0);
-    Q<R> q = qnormalize(rot());
-    V3<R> vt = g3(tool_vel, 0), w = g3(tool_omega, 0);
+    V3<R> xt = g3(tool_pos, k);
+    Q<R> q = qnormalize(rot(k));
+    V3<R> vt = g3(tool_vel, k), w = g3(tool_omega, k);
     V3<R> force = {R(0), R(0), R(-9.81) * mass};
     V3<R> servo = {R(0), R(0), R(0)};
     int free_blade = c.free_blade;
@@ -1192,7 +1213,7 @@ template <class R> struct Sim {
       if (s > R(1e-8)) error = vec * (R(2) * std::atan2(s, dq.w) / s);
       servo = R(c.rot_k) * error - R(c.rot_c) * w;
     }
-    vt = vt + (force * dt + g3(reaction, 0)) / mass;
+    vt = vt + (force * dt + g3(reaction, k)) / mass;
     V3<R> ii = {R(c.inertia_inv[0]), R(c.inertia_inv[1]), R(c.inertia_inv[2])};
     if (lock_rotation) {
       ii = {R(0), R(0), R(0)};
@@ -1201,7 +1222,7 @@ template <class R> struct Sim {
       V3<R> lw = rotate_inv(q, w);
       V3<R> inertia = {R(1) / ii.x, R(1) / ii.y, R(1) / ii.z};
       V3<R> gyro = rotate(q, cross(lw, V3<R>{inertia.x * lw.x, inertia.y * lw.y, inertia.z * lw.z}));
-      w = w + inv_inertia(q, ii, g3(torque, 0) + dt * (servo - gyro));
+      w = w + inv_inertia(q, ii, g3(torque, k) + dt * (servo - gyro));
     }
     V3<R> normal = {R(0), R(0), R(1)};
     V3<R> total = {R(0), R(0), R(0)};
@@ -1242,19 +1263,19 @@ template <class R> struct Sim {
     xt = xt + dt * vt;
     R sp = length(w);
     if (sp > R(1e-10)) q = qnormalize(qmul(axis_angle(w / sp, dt * sp), q));
-    s3(tool_pos, 0, xt);
-    s3(tool_vel, 0, vt);
-    tool_rot = {q.x, q.y, q.z, q.w};
-    s3(tool_omega, 0, w);
-    s3(table, 0, total);
+    s3(tool_pos, k, xt);
+    s3(tool_vel, k, vt);
+    tool_rot[4 * k] = q.x; tool_rot[4 * k + 1] = q.y; tool_rot[4 * k + 2] = q.z; tool_rot[4 * k + 3] = q.w;
+    s3(tool_omega, k, w);
+    s3(table, k, total);
   }
 
   // assign (contact.py:996-1039)
-  void assign(int64_t digit, int cutting) {
+  void assign(int64_t digit, int cutting, int64_t k = 0) {
     if (!cutting) return;
     V3<R> hf = half();
-    Q<R> q = rot();
-    V3<R> pos = g3(tool_pos, 0), tvel = g3(tool_vel, 0), tom = g3(tool_omega, 0);
+    Q<R> q = rot(k);
+    V3<R> pos = g3(tool_pos, k), tvel = g3(tool_vel, k), tom = g3(tool_omega, k);
     R dt = R(c.dt), spacing = R(c.spacing), rad = R(c.radius);
     for (int64_t p = 0; p < n; ++p) {
       if ((codes[p] / digit) % 3 != 0) continue;
@@ -1285,8 +1306,10 @@ template <class R> struct Sim {
     int cut_bits = 0;
     if (cutting_flag) cut_bits = CUT_ON | (c.cut_work_advance ? CUT_ADVANCE : 0) | (c.cut_hold ? CUT_HOLD : 0);
     if (!c.external_tool) {
-      s3(reaction, 0, {R(0), R(0), R(0)});
-      s3(torque, 0, {R(0), R(0), R(0)});
+      for (int64_t k = 0; k < knives; ++k) {
+        s3(reaction, k, {R(0), R(0), R(0)});
+        s3(torque, k, {R(0), R(0), R(0)});
+      }
       s3(board, 0, {R(0), R(0), R(0)});
       speed[0] = R(0);
     }
@@ -1297,9 +1320,10 @@ template <class R> struct Sim {
       finish_external();
     } else {
       if (c.accumulate_tool_impulses) accumulate_reaction();
-      tool_step(target, target_q);
+      for (int64_t k = 0; k < knives; ++k) tool_step(target, target_q, k);
     }
-    assign(digit, cutting_flag);
+    // Knife k labels digit 3^(cut index + k) (solver.py:866-890).
+    for (int64_t k = 0, d = digit; k < knives; ++k, d *= 3) assign(d, cutting_flag, k);
     step_count += 1;
   }
 };
@@ -1519,7 +1543,7 @@ template <class R> bool real_array(Sim<R>& s, const std::string& name, std::vect
       {"interval_angular_impulse", &s.interval_angular}, {"grid_mass", &s.grid_mass}, {"momentum", &s.momentum},
       {"grid_impulse", &s.grid_impulse}, {"old_v", &s.old_v}, {"new_v", &s.new_v}, {"remapped", &s.remapped},
       {"coupled_remapped", &s.coupled_remapped}, {"bc_side", &s.bc_side}, {"bc_impulse", &s.bc_impulse},
-      {"bc_normal", &s.bc_normal}};
+      {"bc_normal", &s.bc_normal}, {"extra_histories", &s.extra_histories}, {"extra_damage", &s.extra_damage}};
   auto it = table.find(name);
   if (it == table.end()) return false;
   out = it->second;

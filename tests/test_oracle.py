@@ -836,3 +836,95 @@ def test_viscous_material_step_conserves_momentum():
     o2.set("impulses", impulse)
     o2.stage("material")
     assert np.abs(o2.get("v") - o.get("v")).max() > 1e-9
+
+
+# ------------------------------------------------------------------ several free knives
+def two_knives(precision=64, **fields):
+    cfg = Config(size=(0.06, 0.03, 0.03), grid_spacing=0.0075, dt=0.0005, free_blade=True, tool_count=2,
+                 blade_half_size=(0.0005, 0.025, 0.02), **fields)
+    o, ref = sim(cfg, precision)
+    return cfg, o, ref
+
+
+def _dropped(xs, steps, precision=64):
+    cfg, o, ref = two_knives(precision)
+    z = 0.03 + 0.02 + 0.002
+    o.set("tool_pos", [[x, 0.0, z] for x in xs])
+    o.set("tool_vel", [[0.0, 0.0, -0.6]] * 2)
+    for _ in range(steps):
+        o.substep(cutting=True)
+    return o, ref
+
+
+def test_second_knife_beside_the_block_falls_freely():
+    """Knife 1 well away from the block falls as -g t (tests/test_multi_tool.py:68-79) while
+    knife 0 cuts; neither disturbs the other."""
+    cfg, o, _ = two_knives(32)
+    z0 = 0.2
+    o.set("tool_pos", [[-0.015, 0.0, 0.03 + 0.02 + 0.002], [0.2, 0.0, z0]])
+    n = 100
+    for _ in range(n):
+        o.substep(cutting=True)
+    dt, g = cfg.dt, 9.81
+    vel, pos = o.get("tool_vel").reshape(2, 3), o.get("tool_pos").reshape(2, 3)
+    assert vel[1, 2] == pytest.approx(-g * n * dt, rel=1.0e-5)
+    assert pos[1, 2] == pytest.approx(z0 - g * dt * dt * n * (n + 1) / 2, abs=1.0e-6)
+    np.testing.assert_allclose(pos[1, :2], [0.2, 0.0], atol=1e-9)
+    assert pos[0, 2] > 0.02 and vel[0, 2] > -0.6  # knife 0 met the block
+    assert not o.get("extra_damage").any()  # knife 1 damaged nothing
+
+
+def test_mirrored_knives_move_alike_and_label_their_own_digits():
+    """Two knives dropped at x = -15 and +15 mm into a block symmetric about x = 0 move as mirror
+    images and each labels its own ternary digit on its own side split
+    (tests/test_multi_tool.py:46-65).  A knife that read another's pose, history or damage
+    breaks the mirror."""
+    steps = 60
+    o, ref = _dropped([-0.015, 0.015], steps)
+    p, v = o.get("tool_pos").reshape(2, 3), o.get("tool_vel").reshape(2, 3)
+    np.testing.assert_allclose(p[0] * [-1, 1, 1], p[1], atol=1.0e-10)
+    np.testing.assert_allclose(v[0] * [-1, 1, 1], v[1], atol=1.0e-8)
+    assert 0.0 < p[0, 2] - 0.02 < 0.028
+    assert v[0, 2] > -0.6 - 9.81 * steps * 0.0005
+    codes = o.get("codes")
+    first, second = codes % 3, (codes // 3) % 3
+    assert set(np.unique(first)) == {0, 1, 2} and set(np.unique(second)) == {0, 1, 2}
+    assert np.all(codes < 9)
+    assert ref[first == 1, 0].max() < -0.015 < ref[first == 2, 0].min()
+    assert ref[second == 1, 0].max() < 0.015 < ref[second == 2, 0].min()
+
+
+def test_knife_index_does_not_matter():
+    """Swapping the knives' slots swaps their results (tests/test_multi_tool.py:34-43)."""
+    a, _ = _dropped([-0.015, 0.015], 40)
+    b, _ = _dropped([0.015, -0.015], 40)
+    np.testing.assert_allclose(a.get("tool_pos").reshape(2, 3), b.get("tool_pos").reshape(2, 3)[::-1], atol=1e-10)
+    np.testing.assert_allclose(a.get("x"), b.get("x"), atol=1e-10)
+
+
+@pytest.mark.parametrize("first", [0, 1])
+def test_each_knife_pays_its_own_cutting_work(first):
+    """One particle under the edge of knife `first`: that knife's damage of it reaches 1 and its
+    reaction is recorded; moved under the other knife, the other knife's damage starts from
+    zero and its reaction repeats the first (tests/test_multi_tool.py:82-104)."""
+    cfg, o, ref = two_knives(64, cutting_energy=1.0)
+    x = ref.copy()
+    x[:, 0] += 10
+    x[0] = [0, 0, 0.039]
+    o.set("x", x)
+    positions = np.array([[3.0, 0, 0.06], [3.0, 0, 0.06]])
+    positions[first, 0] = 0
+    o.set("tool_pos", positions)
+    o.set("tool_vel", [[0, 0, -1], [0, 0, -1]])
+    o.stage("rigid", 1)
+    reaction = o.get("reaction").reshape(2, 3)
+    assert np.abs(reaction[first]).max() > 0 and not reaction[1 - first].any()
+    damage = [o.get("damage")[0], o.get("extra_damage")[0]]
+    assert damage[first] == 1 and damage[1 - first] == 0
+    positions[:, 0] = positions[::-1, 0]
+    o.set("tool_pos", positions)
+    o.set("reaction", np.zeros(6))
+    o.set("torque", np.zeros(6))
+    o.stage("rigid", 1)
+    np.testing.assert_allclose(o.get("reaction").reshape(2, 3)[1 - first], reaction[first], rtol=1e-12)
+    assert [o.get("damage")[0], o.get("extra_damage")[0]][1 - first] == 1
