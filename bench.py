@@ -208,7 +208,7 @@ def run_gpu(args):
     )
 
 
-def run_cpu_baseline(cfg, q, edge_z0, hz, substeps):
+def run_cpu_baseline(cfg, q, edge_z0, hz, substeps, threads=1):
     sys.path.insert(0, ROOT)
     from oracle import OracleSim
     from blademaster_hip.geometry import points, stencil
@@ -218,6 +218,7 @@ def run_cpu_baseline(cfg, q, edge_z0, hz, substeps):
     nodes, w, g = stencil(ref, origin, cfg.grid_spacing, cells)
     node_count = int(np.prod(cells + 1))
     o = OracleSim(cfg, ref, nodes, w, g, node_count, 32, max_cut_index(node_count))
+    threads = o.set_threads(threads)
     o.set("tool_pos", pose_at(0.0, q, edge_z0, hz))
     o.set("tool_rot", q)
     o.set("tool_vel", [0.0, 0.0, -0.1])
@@ -226,7 +227,7 @@ def run_cpu_baseline(cfg, q, edge_z0, hz, substeps):
     for _ in range(substeps):
         o.substep(cutting=True)
     dt = time.perf_counter() - t0
-    return len(ref) * substeps / dt, dt
+    return len(ref) * substeps / dt, dt, threads
 
 
 def run_batch(args):
@@ -450,14 +451,18 @@ def main():
     cpu = None
     if not args.no_cpu:
         _, q, edge_z0, hz, _ = workload()
-        cpu_rate, cpu_s = run_cpu_baseline(cfg, q, edge_z0, hz, args.cpu_substeps)
+        # The host's share of cores (OMP_NUM_THREADS where the environment sets it), and one.
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        many_rate, many_s, threads = run_cpu_baseline(cfg, q, edge_z0, hz, args.cpu_substeps, threads)
+        one_rate, one_s, _ = run_cpu_baseline(cfg, q, edge_z0, hz, args.cpu_substeps, 1)
         cpu = dict(
-            value=cpu_rate,
+            value=many_rate,
             unit="particle-substeps/s",
-            cores=1,
+            cores=threads,
             kind="port",
             sample=f"{args.cpu_substeps} substeps of the same 1M-particle workload after one setup substep, "
-            f"float32 oracle, one thread, {cpu_s:.1f} s",
+            f"float32 oracle on {threads} OpenMP threads ({many_s:.1f} s) and on one thread ({one_s:.1f} s)",
+            one_thread=dict(value=one_rate, cores=1),
         )
     peak = HBM_PEAK_GBS
     line = {

@@ -86,6 +86,8 @@ def load():
             ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         lib.oracle_couple.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int64, ctypes.c_int64]
         lib.oracle_g2p.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_double, ctypes.c_double, ctypes.c_int64]
+        lib.oracle_set_threads.restype = ctypes.c_int
+        lib.oracle_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -220,6 +222,11 @@ class OracleSim:
         if code:
             raise RuntimeError(f"oracle_array({name}) failed: {code}")
         return buf
+
+    def set_threads(self, threads):
+        """Run the per-particle and per-field loops on `threads` OpenMP threads (the results
+        equal the one-thread run bit for bit); returns the count in effect."""
+        return int(self.lib.oracle_set_threads(self.handle, int(threads)))
 
     def scalars(self):
         a = self.get("scalars")

@@ -16,6 +16,11 @@
 // of 3x3 matrices with minimal branching and elementary floating point operations"), and
 // rounded to R.
 //
+// Threads: `threads` > 1 (oracle_set_threads) runs the per-particle and per-field loops with
+// OpenMP for the CPU baseline.  Every sum keeps its order: contact sums are added in particle
+// order after the parallel loop, and grid scatters become gathers over each field slot's
+// (particle, corner) list in particle order, so results equal the one-thread run bit for bit.
+//
 // Parity status of each function is stated in the header of tests/test_oracle.py.
 
 #include <algorithm>
@@ -26,6 +31,9 @@
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -284,7 +292,8 @@ enum { CUT_ON = 1, CUT_ADVANCE = 2, CUT_HOLD = 4 };
 // sum_{b at the node} m_b, walking the node's linked list.
 template <class R>
 void couple_fields(const int64_t* keys, const int32_t* heads, const int32_t* links, const R* mass, const R* state,
-                   R* result, int64_t count, int64_t node_count) {
+                   R* result, int64_t count, int64_t node_count, int threads = 1) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
   for (int64_t a = 0; a < count; ++a) {
     if (keys[a] < 0 || mass[a] <= R(0)) continue;
     int64_t node = keys[a] % node_count;
@@ -304,7 +313,8 @@ void couple_fields(const int64_t* keys, const int32_t* heads, const int32_t* lin
 // g2p_at (material.py:343-367) over explicit slots and weights.
 template <class R>
 void g2p_fields(R* x, R* v, const int32_t* slots, const R* weights, const R* old_v, const R* new_v, R dt, R flip,
-                int64_t n) {
+                int64_t n, int threads = 1) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
   for (int64_t p = 0; p < n; ++p) {
     R pic[3] = {R(0), R(0), R(0)}, dv[3] = {R(0), R(0), R(0)};
     for (int a = 0; a < 8; ++a) {
@@ -360,6 +370,14 @@ template <class R> struct Sim {
   std::unordered_map<int64_t, V3<R>> pair_old, pair_new;
   int64_t pair_count = 0;
   int64_t step_count = 0;
+  int threads = 1;
+  // (particle, corner) entries 8 p + a of each field slot, in particle order (built at
+  // registration): the gather form of the grid scatters for threads > 1.
+  std::vector<int64_t> slot_start, slot_items;
+  void count_error(int k) {
+#pragma omp atomic
+    errors[k] += 1;
+  }
   int64_t cut_index = 0;
   // Held-blade grid constraint state (blade_constraint.py:221-240).
   std::vector<R> bc_side, bc_normal, bc_lever, bc_rhs, bc_bound, bc_impulse, bc_extrap, bc_grid, bc_coupled;
@@ -530,9 +548,15 @@ template <class R> struct Sim {
   // resist_cut (contact.py:181-232); `rad` is the blade contact offset, `dmg` the damage of
   // particle p against this knife.
   V3<R> resist_cut(int64_t p, V3<R> j, V3<R> origin, V3<R> tvel, Q<R> q, V3<R> tomega, R rad, int cutting) {
-    return resist_cut(p, j, origin, tvel, q, tomega, rad, cutting, damage[p]);
+    R delta = R(0);
+    bool worked = false;
+    V3<R> out = resist_cut(p, j, origin, tvel, q, tomega, rad, cutting, damage[p], delta, worked);
+    if (worked) work[0] += delta;
+    return out;
   }
-  V3<R> resist_cut(int64_t p, V3<R> j, V3<R> origin, V3<R> tvel, Q<R> q, V3<R> tomega, R rad, int cutting, R& dmg) {
+  // The work done comes back in `delta` (with `worked` set) for the caller to add.
+  V3<R> resist_cut(int64_t p, V3<R> j, V3<R> origin, V3<R> tvel, Q<R> q, V3<R> tomega, R rad, int cutting, R& dmg,
+                   R& delta_out, bool& worked) {
     V3<R> hf = half();
     V3<R> edge = origin + rotate(q, V3<R>{R(0), R(0), -hf.z});
     V3<R> down = rotate(q, V3<R>{R(0), R(0), R(-1)});
@@ -549,7 +573,8 @@ template <class R> struct Sim {
     if (dmg >= R(1)) return j - push * down;
     if (advance > R(0)) {
       R delta = push * advance;
-      work[0] += delta;
+      delta_out = delta;
+      worked = true;
       dmg = std::min(R(1), dmg + delta / R(c.threshold));
     }
     return j;
@@ -558,7 +583,13 @@ template <class R> struct Sim {
   // blade_at (contact.py:235-329) of knife k: knife 0 keeps its friction history in row 1 of
   // `histories` and its damage in `damage`; knife k >= 1 (extra_blade, contact.py:500-566) in
   // extra_histories and extra_damage.  The reaction and torque go to knife k's slots.
-  V3<R> blade_at(int64_t p, int tool_model, int cutting, int64_t k = 0) {
+  // One particle's contribution to a knife's step sums.
+  struct KnifeSums {
+    V3<R> react{R(0), R(0), R(0)}, torq{R(0), R(0), R(0)};
+    R work = R(0);
+    bool touched = false, worked = false;
+  };
+  V3<R> blade_at(int64_t p, int tool_model, int cutting, int64_t k, KnifeSums& sums) {
     V3<R> hf = half();
     R dt = R(c.dt);
     V3<R> xp = g3(x, p), vp = g3(v, p);
@@ -598,19 +629,28 @@ template <class R> struct Sim {
     penalty(distance - offset, relative, nrm, me, g3(hist, row), R(c.sn), j, h);
     s3(hist, row, h);
     if (dot(j, j) > R(0)) {
-      if (cutting != 0) j = resist_cut(p, j, origin, tvel, q, tom, offset, cutting, dmg);
+      if (cutting != 0) j = resist_cut(p, j, origin, tvel, q, tom, offset, cutting, dmg, sums.work, sums.worked);
       result = j;
-      add3(reaction, k, -j);
-      add3(torque, k, -cross(lever, j));
+      sums.touched = true;
+      sums.react = -j;
+      sums.torq = -cross(lever, j);
     }
     return result;
   }
 
-  // rigid_and_speed / rigid_at (contact.py:332-497)
+  // rigid_and_speed / rigid_at (contact.py:332-497), then extra_blade (contact.py:500-566) for
+  // knives 1 .. K-1 after all of knife 0.  The per-particle contacts are independent; the
+  // board, reaction, torque and work sums are added afterwards in particle order (knife by
+  // knife for the work), the order of a sequential pass.
   void rigid_contacts(int tool_model, int cutting) {
     R dt = R(c.dt);
+    std::vector<V3<R>> board_j(n);
+    std::vector<char> board_hit(n, 0);
+    std::vector<KnifeSums> sums(size_t(n) * size_t(knives));
+    R spd = speed[0];
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1) reduction(max : spd)
     for (int64_t p = 0; p < n; ++p) {
-      speed[0] = std::max(speed[0], length(g3(v, p)));
+      spd = std::max(spd, length(g3(v, p)));
       V3<R> xp = g3(x, p);
       V3<R> total = {R(0), R(0), R(-9.81) * R(c.mass) * dt};
       R gap = xp.z - R(c.radius);
@@ -621,14 +661,38 @@ template <class R> struct Sim {
       s3(histories, p, hb);
       if (dot(jb, jb) > R(0)) {
         total = total + jb;
-        add3(board, 0, jb);
+        board_j[p] = jb;
+        board_hit[p] = 1;
       }
-      total = total + blade_at(p, tool_model, cutting);
+      total = total + blade_at(p, tool_model, cutting, 0, sums[size_t(p) * knives]);
       s3(impulses, p, total);
     }
-    // extra_blade (contact.py:500-566) for knives 1 .. K-1, after all of knife 0.
-    for (int64_t k = 1; k < knives; ++k)
-      for (int64_t p = 0; p < n; ++p) add3(impulses, p, blade_at(p, tool_model, cutting, k));
+    for (int64_t k = 1; k < knives; ++k) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+      for (int64_t p = 0; p < n; ++p) add3(impulses, p, blade_at(p, tool_model, cutting, k, sums[size_t(p) * knives + k]));
+    }
+    speed[0] = spd;
+    for (int64_t p = 0; p < n; ++p)
+      if (board_hit[p]) add3(board, 0, board_j[p]);
+    for (int64_t k = 0; k < knives; ++k)
+      for (int64_t p = 0; p < n; ++p) {
+        const KnifeSums& t = sums[size_t(p) * knives + k];
+        if (t.worked) work[0] += t.work;
+        if (t.touched) {
+          add3(reaction, k, t.react);
+          add3(torque, k, t.torq);
+        }
+      }
+  }
+
+  // True when pair_at(p, q) returns at one of its first tests, which have no effect: beyond
+  // contact reach this step, or compatible particles that were neighbours in the reference.
+  bool pair_inert(int64_t p, int64_t q) const {
+    R dt = R(c.dt), rad = R(c.radius);
+    V3<R> delta = g3(x, p) - g3(x, q);
+    R dist = length(delta);
+    if (dist >= R(2) * rad + dt * length(g3(v, p) - g3(v, q))) return true;
+    return compatible(codes[p], codes[q]) && length(g3(ref, p) - g3(ref, q)) < R(2) * R(c.spacing);
   }
 
   // pair_at (contact.py:584-640)
@@ -688,29 +752,42 @@ template <class R> struct Sim {
       cell_of(p, cc);
       bins[pack(cc[0], cc[1], cc[2])].push_back(p);
     }
-    std::vector<int64_t> cand;
-    for (int64_t p = 0; p < n; ++p) {
-      int64_t cc[3];
-      cell_of(p, cc);
-      cand.clear();
-      for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dz = -1; dz <= 1; ++dz) {
-            auto it = bins.find(pack(cc[0] + dx, cc[1] + dy, cc[2] + dz));
-            if (it == bins.end()) continue;
-            for (int64_t q : it->second)
-              if (q > p) cand.push_back(q);
-          }
-      std::sort(cand.begin(), cand.end());
-      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-      for (int64_t q : cand) {
-        V3<R> d = g3(x, q) - g3(x, p);
-        R d2 = dot(d, d);
-        R r = R(reach);
-        if (d2 >= r * r * R(1.0001) + R(1e-30)) continue;
-        pair_at(p, q);
+    // Candidates (p, q > p) within reach, in (p, q) order: found in parallel over contiguous
+    // particle ranges (one per thread, concatenated in order), then pair_at in that order.
+    // Pairs that pair_at would leave without any effect (too far for this step, or
+    // compatible neighbours in the reference) are dropped in the parallel pass.
+    const int chunks = std::max(1, threads);
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> found(chunks);
+#pragma omp parallel for schedule(static, 1) num_threads(threads) if (threads > 1)
+    for (int t = 0; t < chunks; ++t) {
+      std::vector<int64_t> cand;
+      const int64_t first = n * t / chunks, last = n * (t + 1) / chunks;
+      for (int64_t p = first; p < last; ++p) {
+        int64_t cc[3];
+        cell_of(p, cc);
+        cand.clear();
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+              auto it = bins.find(pack(cc[0] + dx, cc[1] + dy, cc[2] + dz));
+              if (it == bins.end()) continue;
+              for (int64_t q : it->second)
+                if (q > p) cand.push_back(q);
+            }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (int64_t q : cand) {
+          V3<R> d = g3(x, q) - g3(x, p);
+          R d2 = dot(d, d);
+          R r = R(reach);
+          if (d2 >= r * r * R(1.0001) + R(1e-30)) continue;
+          if (pair_inert(p, q)) continue;
+          found[t].emplace_back(p, q);
+        }
       }
     }
+    for (const auto& list : found)
+      for (const auto& pq : list) pair_at(pq.first, pq.second);
   }
 
   // ---------------------------------------------------------------- material
@@ -742,7 +819,21 @@ template <class R> struct Sim {
       links[a] = heads[node];
       heads[node] = int32_t(a);
     }
+    // Field slot -> its (particle, corner) entries in particle order (counting sort).
+    slot_start.assign(active_count + 1, 0);
+    for (int64_t i = 0; i < 8 * n; ++i)
+      if (slots[i] >= 0) slot_start[slots[i] + 1] += 1;
+    for (int64_t a = 0; a < active_count; ++a) slot_start[a + 1] += slot_start[a];
+    slot_items.assign(slot_start[active_count], 0);
+    std::vector<int64_t> fill(slot_start.begin(), slot_start.end() - 1);
+    for (int64_t i = 0; i < 8 * n; ++i)
+      if (slots[i] >= 0) slot_items[fill[slots[i]]++] = i;
     topology_dirty = false;
+  }
+
+  // The field slot lists match the current slots (not after host writes to `slots`).
+  bool gather_ready() const {
+    return threads > 1 && int64_t(slot_start.size()) == active_count + 1;
   }
 
   // p2g (material.py:127-152)
@@ -752,6 +843,21 @@ template <class R> struct Sim {
     std::fill(momentum.begin(), momentum.begin() + 3 * active_count, R(0));
     std::fill(grid_impulse.begin(), grid_impulse.begin() + 3 * active_count, R(0));
     std::fill(remapped.begin(), remapped.begin() + 3 * active_count, R(0));
+    if (gather_ready()) {
+      // Each slot sums its entries in particle order: the same sums as the scatter below.
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+      for (int64_t s = 0; s < active_count; ++s)
+        for (int64_t e = slot_start[s]; e < slot_start[s + 1]; ++e) {
+          int64_t i = slot_items[e], p = i >> 3;
+          int a = int(i & 7);
+          R w = weights[i];
+          V3<R> g = {gradients[24 * p + 3 * a], gradients[24 * p + 3 * a + 1], gradients[24 * p + 3 * a + 2]};
+          grid_mass[s] += mass * w;
+          add3(momentum, s, (mass * w) * g3(v, p));
+          add3(grid_impulse, s, w * g3(impulses, p) - mulv(scale(dt * vol, g9(stress, p)), g));
+        }
+      return;
+    }
     for (int64_t p = 0; p < n; ++p) {
       M3<R> P = g9(stress, p);
       V3<R> vp = g3(v, p), imp = g3(impulses, p);
@@ -769,6 +875,7 @@ template <class R> struct Sim {
 
   // couple_and_velocity_at (material.py:252-285)
   void couple_and_velocity() {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (int64_t a = 0; a < active_count; ++a) {
       if (grid_keys[a] < 0 || grid_mass[a] <= R(0)) continue;
       int64_t node = grid_keys[a] % c.node_count;
@@ -793,13 +900,23 @@ template <class R> struct Sim {
   // couple_at (material.py:155-179)
   void couple(const std::vector<R>& state, std::vector<R>& result) {
     couple_fields<R>(grid_keys.data(), heads.data(), links.data(), grid_mass.data(), state.data(), result.data(),
-                     active_count, c.node_count);
+                     active_count, c.node_count, threads);
   }
 
   // g2p_remap (material.py:343-401)
   void g2p_remap() {
     R mass = R(c.mass);
-    g2p_fields<R>(x.data(), v.data(), slots.data(), weights.data(), old_v.data(), new_v.data(), R(c.dt), R(c.flip), n);
+    g2p_fields<R>(x.data(), v.data(), slots.data(), weights.data(), old_v.data(), new_v.data(), R(c.dt), R(c.flip), n,
+                  threads);
+    if (gather_ready()) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+      for (int64_t s = 0; s < active_count; ++s)
+        for (int64_t e = slot_start[s]; e < slot_start[s + 1]; ++e) {
+          int64_t i = slot_items[e];
+          add3(remapped, s, (mass * weights[i]) * g3(v, i >> 3));
+        }
+      return;
+    }
     for (int64_t p = 0; p < n; ++p) {
       V3<R> vp = g3(v, p);
       for (int a = 0; a < 8; ++a) add3(remapped, slots[8 * p + a], (mass * weights[8 * p + a]) * vp);
@@ -825,7 +942,7 @@ template <class R> struct Sim {
     M3<R> f = mul(g9(F, p), pinv);
     R J = det(f);
     if (J <= R(0) || !std::isfinite(double(J))) {
-      errors[1] += 1;
+      count_error(1);
       s9(stress, p, zero3<R>());
       return;
     }
@@ -839,7 +956,7 @@ template <class R> struct Sim {
   bool plastic_at(int64_t p, M3<R>& U, V3<R>& sg, M3<R>& V) {
     svd3(mul(g9(F, p), inverse(g9(Fp, p))), U, sg, V);
     if (std::min(std::min(sg.x, sg.y), sg.z) <= R(0)) {
-      errors[1] += 1;
+      count_error(1);
       return false;
     }
     V3<R> e = {std::log(sg.x), std::log(sg.y), std::log(sg.z)};
@@ -877,7 +994,7 @@ template <class R> struct Sim {
     V3<R> sm;
     svd3(mul(f, inverse(Fv)), Um, sm, Vm);
     if (std::min(std::min(sm.x, sm.y), sm.z) <= R(0)) {
-      errors[1] += 1;
+      count_error(1);
       return zero3<R>();
     }
     V3<R> e = {std::log(sm.x), std::log(sm.y), std::log(sm.z)};
@@ -920,7 +1037,7 @@ template <class R> struct Sim {
       M3<R> f = mul(g9(F, p), inverse(g9(Fp, p)));
       R J = det(f);
       if (J <= R(0) || !std::isfinite(double(J))) {
-        errors[1] += 1;
+        count_error(1);
         s9(stress, p, zero3<R>());
         return;
       }
@@ -973,6 +1090,7 @@ template <class R> struct Sim {
   // update_F_plastic (material.py:445-475), reading the coupled re-scattered momentum.
   void update_F() {
     R dt = R(c.dt);
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (int64_t p = 0; p < n; ++p) {
       M3<R> rate = zero3<R>();
       for (int a = 0; a < 8; ++a) {
@@ -1277,6 +1395,8 @@ template <class R> struct Sim {
     Q<R> q = rot(k);
     V3<R> pos = g3(tool_pos, k), tvel = g3(tool_vel, k), tom = g3(tool_omega, k);
     R dt = R(c.dt), spacing = R(c.spacing), rad = R(c.radius);
+    bool relabelled = false;
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1) reduction(|| : relabelled)
     for (int64_t p = 0; p < n; ++p) {
       if ((codes[p] / digit) % 3 != 0) continue;
       V3<R> xp = g3(x, p);
@@ -1295,9 +1415,10 @@ template <class R> struct Sim {
       if (std::fabs(local.x) <= R(6) * spacing && std::fabs(local.y) <= hf.y && (behind || pinched)) {
         int64_t side = local.x < R(0) ? 1 : 2;
         codes[p] += digit * side;
-        topology_dirty = true;
+        relabelled = true;
       }
     }
+    if (relabelled) topology_dirty = true;
   }
 
   // One substep in the order of _step_kernels (solver.py:920-940).
@@ -1649,6 +1770,20 @@ int oracle_substep(void* handle, const double* target, const double* target_q, i
     s->substep({target[0], target[1], target[2]}, {target_q[0], target_q[1], target_q[2], target_q[3]}, cutting, digit);
   }
   return 0;
+}
+
+// Threads of the per-particle and per-field loops (1: sequential).  Returns the count in
+// effect, 1 when the library was built without OpenMP.
+int oracle_set_threads(void* handle, int threads) {
+  Handle* h = static_cast<Handle*>(handle);
+#ifdef _OPENMP
+  int t = threads < 1 ? 1 : threads;
+#else
+  int t = 1;
+#endif
+  if (h->precision == 32) static_cast<Sim<float>*>(h->sim)->threads = t;
+  else static_cast<Sim<double>*>(h->sim)->threads = t;
+  return t;
 }
 
 // Stage-level entry points used by the parity tests (each one stage of the substep).

@@ -33,6 +33,17 @@ wrong sign or index fails:
                    with damage at zero yield stress, F returned to the yield surface
   blade constraint face particles leave a held-blade substep without velocity into the
                    faces beyond their gap (tests/test_blade_constraint.py:57-101)
+  Maxwell branch   a held deformation relaxes as the closed-form exp(-n dt / tau) (float64
+                   1e-12; float32 2e-5; tau = 3.5e5 steps within 1 %); the instantaneous
+                   response is the elastic model's; a volume change does not relax; Fv stays
+                   isochoric (tests/test_maxwell.py:90-136)
+  viscosity        node velocities G X + b on a block at uniform F: no impulse at interior
+                   nodes, -dt P N A on each face with P = eta J (L + L^T) F^-T, L = G F^-1; a
+                   rigid spin gives none; a viscous material step conserves momentum
+  extra knives     mirror-image knives stay mirror images and label their own digits; slot
+                   swaps swap results; a knife off the block falls as -g t; each knife pays
+                   its own cutting work (tests/test_multi_tool.py)
+  threads          the OpenMP loops of the CPU baseline give the one-thread results bit for bit
 
 Parity unpinned: the exact float32 bits of the reference's wp.svd3 and of Warp's quaternion
 code generation (no Warp in this environment); the oracle's SVD is evaluated in float64.
@@ -928,3 +939,29 @@ def test_each_knife_pays_its_own_cutting_work(first):
     o.stage("rigid", 1)
     np.testing.assert_allclose(o.get("reaction").reshape(2, 3)[1 - first], reaction[first], rtol=1e-12)
     assert [o.get("damage")[0], o.get("extra_damage")[0]][1 - first] == 1
+
+
+@pytest.mark.parametrize("tools", [1, 2])
+def test_threaded_oracle_equals_one_thread(tools):
+    """The OpenMP loops (CPU baseline) keep every sum's order: a cutting run with pair
+    contacts, relabelling, plasticity and one or two knives is bit-identical on 1 and 4
+    threads."""
+    extra = dict(free_blade=True, tool_count=2) if tools == 2 else {}
+    cfg = Config(size=(0.048, 0.024, 0.018), grid_spacing=0.006, dt=0.0002, cutting_energy=0.05,
+                 yield_stress=2000.0, **extra)
+    runs = []
+    for threads in (1, 4):
+        o, ref = sim(cfg, 32)
+        assert o.set_threads(threads) == threads
+        rng = np.random.default_rng(0)
+        o.set("v", rng.normal(0, 0.01, (o.n, 3)))
+        hz = cfg.blade_half_size[2]
+        o.set("tool_pos", [[x, 0.0, cfg.size[2] + hz - 0.002] for x in (0.0, 0.012)[:tools]])
+        o.set("tool_vel", [[0.0, 0.0, -0.05]] * tools)
+        for _ in range(12):
+            o.substep(cutting=True)
+        runs.append({name: o.get(name) for name in ("x", "v", "F", "Fp", "stress", "damage", "histories", "codes",
+                                                    "tool_pos", "tool_vel", "reaction", "work", "errors")})
+    assert np.count_nonzero(runs[0]["codes"]) > 0
+    for name in runs[0]:
+        np.testing.assert_array_equal(runs[0][name], runs[1][name], err_msg=name)
