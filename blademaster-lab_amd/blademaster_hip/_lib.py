@@ -106,14 +106,15 @@ class MlsParams(ctypes.Structure):
         ("plastic", ctypes.c_int64),
         ("origin", ctypes.c_double * 3),
     ] + [(name, ctypes.c_double) for name in
-         "dx dt mass volume mu lam damaged_shear softening compressed stretched".split()]
+         "dx dt mass volume mu lam damaged_shear softening compressed stretched".split()] + [
+        ("groups", ctypes.c_int64)]
 
 
 MLS_STATE_FIELDS = [
     (name, c_f32p) for name in
     "x v C F damage damage_reference yield_stress impulse grid_m grid_p grid_v interval_impulse "
     "interval_angular".split()
-] + [("inversions", c_i32p)]
+] + [("inversions", c_i32p), ("perm", c_i32p)]
 
 
 class MlsState(ctypes.Structure):

@@ -30,6 +30,33 @@ class TopoCutParams:
     headroom: float = 0.06  # height above the material, m
 
 
+def p2g_groups(positions, origin, dx, brick=3, slots=256):
+    """P2G workgroups of the particles: grouped by bricks of brick^3 stencil base cells (the
+    base cell of cell_base, mlsmpm.py:54-61, at these positions), each brick's particles in
+    base-cell order, at most `slots` per group (a brick with more takes several).  Returns
+    the particle of each slot, -1 for an empty one, `slots` per group."""
+    n = len(positions)
+    if n == 0:
+        return np.full(slots, -1, dtype=np.int32)
+    base = np.floor((np.asarray(positions, np.float64) - origin) / dx - 0.5).astype(np.int64)
+    base -= base.min(0)
+    span = base.max(0) + 1
+    brick_of = base // brick
+    bspan = brick_of.max(0) + 1
+    brick_id = (brick_of[:, 0] * bspan[1] + brick_of[:, 1]) * bspan[2] + brick_of[:, 2]
+    cell_id = (base[:, 0] * span[1] + base[:, 1]) * span[2] + base[:, 2]
+    order = np.lexsort((np.arange(n), cell_id, brick_id))
+    b_sorted = brick_id[order]
+    starts = np.flatnonzero(np.r_[True, b_sorted[1:] != b_sorted[:-1]])
+    ends = np.r_[starts[1:], n]
+    out = []
+    for s0, s1 in zip(starts, ends):
+        for g in range(s0, s1, slots):
+            chunk = order[g:min(g + slots, s1)]
+            out.append(np.pad(chunk.astype(np.int32), (0, slots - len(chunk)), constant_values=-1))
+    return np.concatenate(out)
+
+
 class MlsSolver:
     """damage_from_start=True applies Eq. 6 to J of the reference configuration from the
     first step; False leaves damage off until `start_damage` (mlsmpm.py:232-235)."""
@@ -94,6 +121,9 @@ class MlsSolver:
             rigid_partial=z(max(1, -(-n // 256)) * 12),
             counters=z(4, dtype=torch.int32),
         )
+        groups = p2g_groups(reference, self.origin, dx)
+        self.p2g_groups = len(groups) // 256
+        T["perm"] = torch.tensor(groups, dtype=torch.int32, device=dev)
         for name in ("x", "v", "C", "F", "damage", "tool_pos", "tool_vel", "tool_rot", "tool_omega", "tool_mobility",
                      "impulse", "angular_impulse", "inversions", "damage_reference", "yield_stress"):
             setattr(self, name, DArray(T[name]))
@@ -155,6 +185,8 @@ class MlsSolver:
                             ("interval_angular", "angular_impulse")]:
             setattr(s, field, self._ptr(name, f))
         s.inversions = self._ptr("inversions", _lib.c_i32p)
+        m.groups = self.p2g_groups
+        s.perm = self._ptr("perm", _lib.c_i32p)
         return m, s
 
     def _stream(self):

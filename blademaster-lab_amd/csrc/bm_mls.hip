@@ -59,40 +59,59 @@ __device__ __forceinline__ bool finite3(f3 v) { return isfinite(v.x) && isfinite
 
 // p2g (mlsmpm.py:75-119).
 //
-// The workgroup's particles are consecutive in index order, which the sampling makes
-// neighbours in space (z fastest): their 27-node stencils overlap heavily.  Instead of 27 x 4
-// float atomics per particle, the workgroup
-//   1. computes each particle's stencil base cell, momentum and APIC affine term into LDS,
-//   2. sorts its particles by base cell inside the box of base cells they occupy (a counting
-//      sort: one LDS atomic per particle and a scan),
-//   3. lets each node of the box gather the contributions of the particles in the 27 base
-//      cells whose stencils reach it, in registers, and adds the node's 4 sums to the grid
-//      with one global atomic each.
-// A workgroup whose particles spread over more than kMaxCells base cells (large motion) adds
-// every contribution with a global atomic instead.  Sums agree up to float order.
-constexpr int kMaxCells = 1024;
-
-// Particle record of the gather.  15 floats: the odd stride spreads the lanes' scattered reads
-// over the LDS banks (a padded 16-float record with 16-byte reads measured 1.5x slower).
-struct P2gParticle {
-  float fx, fy, fz, mx, my, mz;  // offset from the base node (cells), momentum
-  float a[9];                    // affine term: scale * P F^T + m C
-};
+// Instead of 27 x 4 float atomics per particle, a workgroup takes the particles of one brick
+// of 3 x 3 x 3 stencil base cells (`perm`, grouped on the host from the reference positions;
+// particles that have moved are still handled, since the cells are recomputed here) and:
+//   1. computes each particle's APIC terms into LDS, in a form where the contribution to node
+//      offset (i, j, k) is
+//        w_i w_j w_k (q + a_0 i + a_1 j + a_2 k),  q = p - a f,  a = dx (scale P F^T + m C)
+//      (f: the particle's offset from its base node, in cells; w from f);
+//   2. finds the box of base cells its particles occupy (wavefront min / max: LDS atomics
+//      on six shared words cost 126 us on the 1M block) and counting-sorts the particles
+//      by base cell, with one scan inside each wavefront;
+//   3. sums, for every occupied cell and node offset, the contributions of the cell's
+//      particles: the lane of (cell, j, k) walks the cell's particles, which every lane of
+//      the cell reads at the same LDS address, and keeps the 3 offsets i in registers;
+//   4. lets each node of the box gather the <= 27 cell sums that reach it and add each
+//      component to the grid with one global atomic.
+// Cell sums are made kMaxSumCells occupied cells at a time (more only when particles have
+// moved across cells: lattice particles that sit on base-cell boundaries split a brick over
+// up to 4 x 4 x 4 cells).  A workgroup whose cell box exceeds kMaxCells adds every
+// contribution with global atomics instead.  Sums agree with the reference's up to float
+// order.  The LDS budget keeps four workgroups per CU: a 21-float record with the weights,
+// or 64 cells of sums, left three and cost 40-50 us more.
+constexpr int kMaxCells = 512;     // box of base cells (2 per thread in the scan)
+constexpr int kMaxSumCells = 52;   // occupied cells whose offset sums fit in LDS at once
+constexpr int kRec = 15;           // floats per particle record: f 3, q 3, a 9
 
 __device__ __forceinline__ float spline_at(float f, int i) {
   return i == 0 ? 0.5f * (1.5f - f) * (1.5f - f) : (i == 1 ? 0.75f - (f - 1.0f) * (f - 1.0f) : 0.5f * (f - 0.5f) * (f - 0.5f));
 }
 
 __global__ void __launch_bounds__(kBlock) k_mls_p2g(const bm_mls_params M, bm_mls_state S) {
-  __shared__ P2gParticle part[kBlock];
-  __shared__ int cell_start[kMaxCells + 1];
+  // 40.5 KB in all, so that four workgroups fit in a CU's LDS.
+  __shared__ float rec[kBlock * kRec];
+  __shared__ __attribute__((aligned(16))) float cell_sum[kMaxSumCells * 27 * 4];
+  // The per-cell counts live in cell_sum's space: they are read by the scan before any sum
+  // is written.
+  int* cell_count = reinterpret_cast<int*>(cell_sum);
+  __shared__ unsigned short cell_start[kMaxCells + 1];
+  __shared__ short occ_of[kMaxCells];  // occupied-cell index, -1 for an empty cell
+  __shared__ int wave_total[kBlock / 64];
   __shared__ unsigned short sorted[kBlock];
   __shared__ int lo[3], hi[3];
+  __shared__ int box_part[kBlock / 64][6];
   const int t = threadIdx.x;
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + t;
+  int64_t p = -1;
+  if (S.perm != nullptr) {
+    p = S.perm[int64_t(blockIdx.x) * blockDim.x + t];
+  } else {
+    p = int64_t(blockIdx.x) * blockDim.x + t;
+    if (p >= M.n) p = -1;
+  }
   Base b = Base{0, 0, 0};
   f3 fx = f3{0.f, 0.f, 0.f};
-  bool live = p < M.n;
+  bool live = p >= 0;
   f3 x = live ? ld3(S.x, p) : f3{0.f, 0.f, 0.f};
   live = live && finite3(x) && stencil_base(M, x, b, fx);
   const float dx = float(M.dx), mass = float(M.mass);
@@ -115,28 +134,38 @@ __global__ void __launch_bounds__(kBlock) k_mls_p2g(const bm_mls_params M, bm_ml
     for (int k = 0; k < 9; ++k) affine.a[k] = scale * stress.a[k] + mass * C.a[k];
     mom = mass * ld3(S.v, p) + ld3(S.impulse, p);
   }
-  if (t < 3) {
-    lo[t] = INT_MAX;
-    hi[t] = INT_MIN;
+  // Box of the base cells: a min / max over each wavefront, then over the wavefronts.
+  {
+    int v[6] = {live ? b.x : INT_MAX, live ? b.y : INT_MAX, live ? b.z : INT_MAX,
+                live ? b.x : INT_MIN, live ? b.y : INT_MIN, live ? b.z : INT_MIN};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        int o = __shfl_xor(v[a], off, 64);
+        v[a] = a < 3 ? min(v[a], o) : max(v[a], o);
+      }
+    if ((t & 63) == 0)
+#pragma unroll
+      for (int a = 0; a < 6; ++a) box_part[t >> 6][a] = v[a];
+    __syncthreads();
+    if (t < 6) {
+      int r = box_part[0][t];
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) r = t < 3 ? min(r, box_part[w][t]) : max(r, box_part[w][t]);
+      if (t < 3) lo[t] = r;
+      else hi[t - 3] = r;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (live) {
-    atomicMin(&lo[0], b.x);
-    atomicMin(&lo[1], b.y);
-    atomicMin(&lo[2], b.z);
-    atomicMax(&hi[0], b.x);
-    atomicMax(&hi[1], b.y);
-    atomicMax(&hi[2], b.z);
-  }
-  __syncthreads();
   const bool any = hi[0] >= lo[0];
+  if (!any) return;
   const int cx = hi[0] - lo[0] + 1, cy = hi[1] - lo[1] + 1, cz = hi[2] - lo[2] + 1;
-  const int64_t cells64 = any ? int64_t(cx) * cy * cz : 0;
+  const int64_t cells64 = int64_t(cx) * cy * cz;
   const int64_t dz = M.dims[2];
   const int64_t plane = M.dims[0] * M.dims[1] * M.dims[2];
-  if (!any) return;
-  if (cells64 > kMaxCells) {
-    // Scattered particles: every contribution goes to the grid directly.
+  // Direct atomics: this workgroup's particles (when its box of cells is too large) or,
+  // below, after counting its occupied cells.
+  auto scatter = [&]() {
     if (!live) return;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
@@ -151,82 +180,140 @@ __global__ void __launch_bounds__(kBlock) k_mls_p2g(const bm_mls_params M, bm_ml
           atomicAdd(S.grid_p + 2 * plane + node, c.z);
           atomicAdd(S.grid_m + node, weight * mass);
         }
+  };
+  if (cells64 > kMaxCells) {
+    scatter();
     return;
   }
   const int cells = int(cells64);
-  // 1. particle data and counting sort by base cell
-  for (int c = t; c <= cells; c += blockDim.x) cell_start[c] = 0;
-  if (live) {
-    P2gParticle q;
-    q.fx = fx.x;
-    q.fy = fx.y;
-    q.fz = fx.z;
-    q.mx = mom.x;
-    q.my = mom.y;
-    q.mz = mom.z;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) q.a[k] = affine.a[k];
-    part[t] = q;
-  }
-  __syncthreads();
+  // 1. particle records; counts per cell
+  for (int c = t; c < kMaxCells; c += blockDim.x) cell_count[c] = 0;
   int my_cell = -1, rank = 0;
   if (live) {
+    float* r = rec + t * kRec;
+    r[0] = fx.x;
+    r[1] = fx.y;
+    r[2] = fx.z;
+#pragma unroll
+    for (int row = 0; row < 3; ++row) {
+      float a0 = dx * affine.a[3 * row], a1 = dx * affine.a[3 * row + 1], a2 = dx * affine.a[3 * row + 2];
+      r[6 + 3 * row] = a0;
+      r[7 + 3 * row] = a1;
+      r[8 + 3 * row] = a2;
+      r[3 + row] = comp(mom, row) - (a0 * fx.x + a1 * fx.y + a2 * fx.z);
+    }
     my_cell = ((b.x - lo[0]) * cy + (b.y - lo[1])) * cz + (b.z - lo[2]);
-    rank = atomicAdd(&cell_start[my_cell + 1], 1);
   }
   __syncthreads();
-  // Inclusive scan of the counts in cell_start[1..cells] (one workgroup, cells <= 1024).
-  for (int off = 1; off < cells; off <<= 1) {
-    int v[kMaxCells / kBlock];
-#pragma unroll
-    for (int r = 0; r < kMaxCells / kBlock; ++r) {
-      int c = 1 + t + r * kBlock;
-      v[r] = (c <= cells && c - off >= 1) ? cell_start[c - off] : 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kMaxCells / kBlock; ++r) {
-      int c = 1 + t + r * kBlock;
-      if (c <= cells) cell_start[c] += v[r];
-    }
-    __syncthreads();
-  }
-  if (live) sorted[cell_start[my_cell] + rank] = (unsigned short)t;
+  if (live) rank = atomicAdd(&cell_count[my_cell], 1);
   __syncthreads();
-  // 2. gather per node of the node box (cells + 2 per axis)
+  // 2. exclusive scan of the counts and of the occupied flags (packed: count in the low 16
+  //    bits, flag in the high bits), thread t holding cells 2t and 2t + 1: a scan inside the
+  //    wavefront, then the wavefronts' totals.
+  {
+    const int c0 = 2 * t, c1 = 2 * t + 1;
+    const int a0 = c0 < cells ? cell_count[c0] : 0, a1 = c1 < cells ? cell_count[c1] : 0;
+    const int mine = (a0 + a1) | ((a0 > 0 ? 1 : 0) + (a1 > 0 ? 1 : 0)) << 16;
+    int incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int v = __shfl_up(incl, off, 64);
+      if ((t & 63) >= off) incl += v;
+    }
+    if ((t & 63) == 63) wave_total[t >> 6] = incl;
+    __syncthreads();
+    int before = incl - mine;
+    for (int w = 0; w < (t >> 6); ++w) before += wave_total[w];
+    const int start = before & 0xFFFF, occ = before >> 16;
+    if (c0 < cells) {
+      cell_start[c0] = (unsigned short)start;
+      occ_of[c0] = short(a0 > 0 ? occ : -1);
+    }
+    if (c1 < cells) {
+      cell_start[c1] = (unsigned short)(start + a0);
+      occ_of[c1] = short(a1 > 0 ? occ + (a0 > 0 ? 1 : 0) : -1);
+    }
+    if (t == int(blockDim.x) - 1) cell_start[cells] = (unsigned short)((before + mine) & 0xFFFF);
+  }
+  int occupied = 0;
+  for (int w = 0; w < int(blockDim.x >> 6); ++w) occupied += wave_total[w] >> 16;
+  __syncthreads();
+  if (live) sorted[int(cell_start[my_cell]) + rank] = (unsigned short)t;
+  __syncthreads();
   const int ex = cx + 2, ey = cy + 2, ez = cz + 2;
   const int nodes = ex * ey * ez;
-  for (int n = t; n < nodes; n += blockDim.x) {
-    int nk = n % ez, nj = (n / ez) % ey, ni = n / (ez * ey);
-    float sx = 0.f, sy = 0.f, sz = 0.f, sm = 0.f;
-    for (int i = 0; i < 3; ++i) {
-      int bi = ni - i;
-      if (bi < 0 || bi >= cx) continue;
-      for (int j = 0; j < 3; ++j) {
-        int bj = nj - j;
-        if (bj < 0 || bj >= cy) continue;
-        for (int k = 0; k < 3; ++k) {
-          int bk = nk - k;
-          if (bk < 0 || bk >= cz) continue;
-          int c = (bi * cy + bj) * cz + bk;
-          for (int e = cell_start[c]; e < cell_start[c + 1]; ++e) {
-            const P2gParticle& q = part[sorted[e]];
-            float weight = spline_at(q.fx, i) * spline_at(q.fy, j) * spline_at(q.fz, k);
-            float px = (float(i) - q.fx) * dx, py = (float(j) - q.fy) * dx, pz = (float(k) - q.fz) * dx;
-            sx += weight * (q.mx + q.a[0] * px + q.a[1] * py + q.a[2] * pz);
-            sy += weight * (q.my + q.a[3] * px + q.a[4] * py + q.a[5] * pz);
-            sz += weight * (q.mz + q.a[6] * px + q.a[7] * py + q.a[8] * pz);
-            sm += weight * mass;
+  // Occupied cells kMaxSumCells at a time (almost always one pass).
+  for (int first = 0; first < occupied; first += kMaxSumCells) {
+    const int last = min(occupied, first + kMaxSumCells);
+    // 3. cell sums: lane (occupied cell, j, k) over the cell's particles, offsets i in
+    //    registers.  Every lane of a cell reads the same particle record.
+    for (int item = t; item < cells * 9; item += blockDim.x) {
+      const int c = item / 9, jk = item - 9 * c;
+      const int occ = occ_of[c];
+      if (occ < first || occ >= last) continue;
+      const int start = cell_start[c], end = cell_start[c + 1];
+      const int j = jk / 3, k = jk - 3 * j;
+      const float fj = float(j), fk = float(k);
+      float acc[3][4];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[i][q] = 0.0f;
+      for (int e = start; e < end; ++e) {
+        const float* r = rec + int(sorted[e]) * kRec;
+        const float wjk = spline_at(r[1], j) * spline_at(r[2], k);
+        float base[3];
+#pragma unroll
+        for (int row = 0; row < 3; ++row) base[row] = r[3 + row] + r[7 + 3 * row] * fj + r[8 + 3 * row] * fk;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const float w = spline_at(r[0], i) * wjk;
+#pragma unroll
+          for (int row = 0; row < 3; ++row) acc[i][row] += w * (base[row] + r[6 + 3 * row] * float(i));
+          acc[i][3] += w * mass;
+        }
+      }
+      float* dst = cell_sum + (occ - first) * 27 * 4;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[((i * 3 + j) * 3 + k) * 4 + q] = acc[i][q];
+    }
+    __syncthreads();
+    // 4. per node of the node box: the cell sums reaching it (the 4 components of an offset
+    //    are one 16-byte read), then one atomic per component.
+    const float4* sums = reinterpret_cast<const float4*>(cell_sum);
+    for (int n = t; n < nodes; n += blockDim.x) {
+      const int nk = n % ez, nj = (n / ez) % ey, ni = n / (ez * ey);
+      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int i = 0; i < 3; ++i) {
+        const int bi = ni - i;
+        if (bi < 0 || bi >= cx) continue;
+        for (int j = 0; j < 3; ++j) {
+          const int bj = nj - j;
+          if (bj < 0 || bj >= cy) continue;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const int bk = nk - k;
+            if (bk < 0 || bk >= cz) continue;
+            const int occ = occ_of[(bi * cy + bj) * cz + bk];
+            if (occ >= first && occ < last) {
+              const float4 v = sums[(occ - first) * 27 + (i * 3 + j) * 3 + k];
+              sum.x += v.x;
+              sum.y += v.y;
+              sum.z += v.z;
+              sum.w += v.w;
+            }
           }
         }
       }
+      const int64_t node = (int64_t(lo[0] + ni) * M.dims[1] + (lo[1] + nj)) * dz + (lo[2] + nk);
+      if (sum.x != 0.0f) atomicAdd(S.grid_p + node, sum.x);
+      if (sum.y != 0.0f) atomicAdd(S.grid_p + plane + node, sum.y);
+      if (sum.z != 0.0f) atomicAdd(S.grid_p + 2 * plane + node, sum.z);
+      if (sum.w != 0.0f) atomicAdd(S.grid_m + node, sum.w);
     }
-    if (sm == 0.0f && sx == 0.0f && sy == 0.0f && sz == 0.0f) continue;
-    int64_t node = (int64_t(lo[0] + ni) * M.dims[1] + (lo[1] + nj)) * dz + (lo[2] + nk);
-    atomicAdd(S.grid_p + node, sx);
-    atomicAdd(S.grid_p + plane + node, sy);
-    atomicAdd(S.grid_p + 2 * plane + node, sz);
-    atomicAdd(S.grid_m + node, sm);
+    __syncthreads();
   }
 }
 
@@ -366,7 +453,10 @@ extern "C" {
 int bm_mls_p2g(const bm_mls_params* m, const bm_mls_state* s, void* stream) {
   if (!mls_ok(m, s)) return -1;
   if (m->n == 0) return 0;
-  hipLaunchKernelGGL(k_mls_p2g, dim3(blocks_for(m->n)), dim3(kBlock), 0, (hipStream_t)stream, *m, *s);
+  // One workgroup per group of `perm` (kBlock slots each), or per kBlock particles in index order.
+  const int groups = s->perm != nullptr ? int(m->groups) : blocks_for(m->n);
+  if (s->perm != nullptr && m->groups < 1) return -1;
+  hipLaunchKernelGGL(k_mls_p2g, dim3(groups), dim3(kBlock), 0, (hipStream_t)stream, *m, *s);
   return status();
 }
 

@@ -369,6 +369,7 @@ typedef struct bm_mls_params {
   int64_t plastic;         /* yield_stress > 0 */
   double origin[3];
   double dx, dt, mass, volume, mu, lam, damaged_shear, softening, compressed, stretched;
+  int64_t groups;          /* workgroups of P2G when perm is set (kBlock = 256 slots each) */
 } bm_mls_params;
 
 typedef struct bm_mls_state {
@@ -376,6 +377,10 @@ typedef struct bm_mls_state {
   float *grid_m, *grid_p, *grid_v;
   float *interval_impulse, *interval_angular;
   int32_t *inversions;
+  /* [groups][256] particle of each P2G slot, -1 for none: particles grouped by bricks of
+   * 3 x 3 x 3 stencil base cells, so a workgroup's particles share nodes.  Null: P2G takes
+   * particles in index order.  Any grouping gives the same sums up to float order. */
+  int32_t *perm;
 } bm_mls_state;
 
 /* Quadratic B-spline APIC P2G with the corotated stress (mlsmpm.py:75-119). */

@@ -320,3 +320,24 @@ def test_multiply_high_division_constants_are_exact():
         m = (2**64 - 1) // d + 1
         for x in [0, d - 1, d, 2**32 - 1] + [int(v) for v in rng.integers(0, 2**32, 20)]:
             assert (x * m) >> 64 == x // d
+
+
+@pytest.mark.parametrize("shape, size, spacing", [("block", (0.06, 0.03, 0.024), 0.003),
+                                                  ("banana", BANANA_DEMO_SIZE_M, 0.006)])
+def test_mls_p2g_groups_cover_every_particle_once(shape, size, spacing):
+    """The MLS P2G workgroups hold every particle exactly once, 256 slots per group, and a
+    group's particles lie in one brick of 3 x 3 x 3 stencil base cells."""
+    from blademaster_hip.mlsmpm import p2g_groups
+
+    ref, _, _ = points(size, spacing, shape, 0)
+    origin = ref.min(0) - 0.01
+    groups = p2g_groups(ref, origin, spacing)
+    assert len(groups) % 256 == 0
+    used = groups[groups >= 0]
+    assert np.array_equal(np.sort(used), np.arange(len(ref)))
+    base = np.floor((ref - origin) / spacing - 0.5).astype(int)
+    for g in groups.reshape(-1, 256):
+        members = g[g >= 0]
+        assert len(members) > 0
+        bricks = np.unique((base[members] - base.min(0)) // 3, axis=0)
+        assert len(bricks) == 1
