@@ -169,8 +169,11 @@ def keyframes(name):
 
 def _scene(name):
     """Material, geometry and knife of a scene (scenarios.py:197-284)."""
+    # The reference's field table has no cap (one hash slot per code and node); crossing cuts
+    # put up to 9 codes (3 sides of each of two cuts) on a node near the crossing, which 4 field
+    # slots cannot hold: these scenes take 16.
     common = dict(external_tool=True, grid_storage="compact", blade_contact_radius=0.0,
-                  blade_bevel_height=DEMO_KNIFE_BEVEL_HEIGHT_M)
+                  blade_bevel_height=DEMO_KNIFE_BEVEL_HEIGHT_M, fields_per_node=16)
     if name == "fig4":
         return Config(size=(2 * FIG4_RADIUS, 2 * FIG4_RADIUS, FIG4_HEIGHT), shape="cylinder", grid_spacing=0.01,
                       dt=0.002, normal_scale=1.0, board_normal_scale=1.0, young_modulus=10000.0, poisson_ratio=0.4,
@@ -185,11 +188,11 @@ def _scene(name):
                       yield_stress=2000.0, cutting_energy=10.0, blade_half_size=FIG8_KNIFE_HALF_SIZE, **common)
     if name == "three-cuts":
         return Config(size=LOG_SIZE, shape="horizontal_cylinder", grid_spacing=0.003, dt=40.0e-6,
-                      young_modulus=2.0e5, poisson_ratio=0.35, fields_per_node=4, cache_topology=True,
-                      blade_half_size=LOG_KNIFE_HALF_SIZE, **common)
+                      young_modulus=2.0e5, poisson_ratio=0.35, cache_topology=True,
+                      blade_half_size=LOG_KNIFE_HALF_SIZE, **dict(common, fields_per_node=4))
     return Config(size=(0.06, 0.03, 0.03), grid_spacing=0.006, dt=0.00025, young_modulus=6000, poisson_ratio=0.35,
                   external_tool=True, blade_half_size=DEMO_KNIFE_HALF_SIZE_M,
-                  blade_bevel_height=DEMO_KNIFE_BEVEL_HEIGHT_M)
+                  blade_bevel_height=DEMO_KNIFE_BEVEL_HEIGHT_M, fields_per_node=16)
 
 
 def scene_config(name, dt=None, normal_scale=None):

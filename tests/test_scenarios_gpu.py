@@ -39,3 +39,20 @@ def test_three_sequential_cuts_leave_four_pieces(tmp_path):
 def test_reference_scene_runs_through_the_port(tmp_path):
     meta = sc.run(tmp_path / "run", "fig8", duration=0.5, log=lambda *a: None)
     assert meta["status"] == "completed" and meta["completed_sim_s"] == pytest.approx(0.5)
+
+
+@pytest.mark.parametrize("scene", ["fig7-shallow", "fig7-deep", "fig8", "fig4", "cross-shallow"])
+def test_crossing_cut_scenes_stay_within_the_field_slots(tmp_path, scene):
+    """The field table holds fields_per_node codes per node (16 in these scenes), where the
+    reference's hash table has no cap: with 4, the crossing cuts of Fig. 7 and Fig. 4 overflow
+    (2 to 28 nodes).  The crossing cuts of Fig. 7, the slices of Fig. 8, the two radial cuts and
+    lift of Fig. 4 and the crossed demo cuts run their whole keyframed duration without a
+    storage overflow (the port raises on any device error count), and the material stays
+    valid."""
+    meta = sc.run(tmp_path / "run", scene, log=lambda *a: None)
+    keys = sc.keyframes(scene)
+    assert meta["status"] == "completed", meta.get("error")
+    assert meta["completed_sim_s"] == pytest.approx(keys[-1][0], abs=2 * max(0.002, meta["config"]["dt"]))
+    assert meta["min_det_F"] > 0
+    sides = load_states(tmp_path / "run" / "states.npz")["sides"][-1]
+    assert np.count_nonzero(sides) > 0  # the knife labelled the material it crossed
