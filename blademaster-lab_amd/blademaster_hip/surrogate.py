@@ -8,11 +8,13 @@ contact impulses.  Widening the exchange to a window of W robot steps needs the 
 step of the window before the material has run it.  Without a learned model the last measured
 wrench is held over the window ("hold"); here a small MLP predicts it ("mlp").
 
-One window (`CutRun.window`):
-1. For k = 0..W-1 the predictor gives the wrench w^_k from the arm state at step k, the last
-   measured wrench and the occupancy/damage patch (both from the window start, k steps old)
-   and the material; the arm steps with w^_k, and the material interval k is enqueued from the
-   arm state at its start (the TLMPM runs while the host predicts the next step).
+One window (`CutRun.window_step`):
+1. The material interval 0 is enqueued from the arm state at the window start.  While it runs,
+   the predictor gives the wrenches w^_k of all W steps in one inference, from the arm state
+   k steps into the window (along the path the held wrench would give), the last measured
+   wrench and the occupancy/damage patch (both from the window start, k steps old) and the
+   material.  Then for k = 0..W-1 the arm steps with w^_k and interval k + 1 is enqueued from
+   the arm state at its start.
 2. One readback returns the W measured wrenches w_k: the full two-way TLMPM coupling along the
    same knife path judges the prediction.  The residual of an instance is
    |mean_k (F^_k - F_k)| / max(|mean_k F_k|, floor) on the force.
@@ -48,6 +50,8 @@ from .presets import banana_config
 INTERVAL = 0.001
 INPUTS, HIDDEN, OUTPUTS = 96, 256, 6
 THRESHOLD = 0.05
+# Predicted force changes below this (N) are taken as zero (chosen on the validation tasks).
+SHRINK = 0.15
 PREV = slice(12, 18)
 PATCH_FEATURES = 70
 # Smallest standard deviation used to standardise each input: the physical resolution of the
@@ -221,11 +225,29 @@ class CutRun:
         self.wall_s += time.perf_counter() - begin
         return w
 
-    def _predict(self, net, age):
-        x = torch.from_numpy(self.features(age)).pin_memory()
+    def _predict_window(self, net, targets):
+        """The network's wrenches for the W steps of a window, from one inference: the inputs
+        of step k take the arm state k steps into the window along the path the held wrench
+        would give (the arm's motion over a window depends little on the wrench it feels).
+        A predicted change of the force below SHRINK newtons is taken as zero (hold): on
+        validation tasks the network's error in steady contact is of that size."""
+        saved = self.arm.save()
+        rows = []
+        for k, (origin, orientation) in enumerate(targets):
+            rows.append(self.features(k))
+            self.arm.step(origin, orientation, self.prev)
+        self.arm.load(saved)
+        x = np.concatenate(rows)
+        if getattr(self, "_x_host", None) is None or self._x_host.shape != x.shape:
+            self._x_host = torch.empty(x.shape, dtype=torch.float32, pin_memory=True)
+        self._x_host.copy_(torch.from_numpy(x))
+        x = self._x_host
         with torch.cuda.stream(self._mlp_stream):
-            y = net.infer(x.to(net.device, non_blocking=True))
-            return self.prev + y.cpu().numpy().astype(np.float64)
+            y = net.infer(x.to(net.device, non_blocking=True)).cpu().numpy().astype(np.float64)
+        y = y.reshape(len(targets), self.count, OUTPUTS)
+        small = np.linalg.norm(y[..., :3], axis=-1) < SHRINK
+        y[small] = 0.0
+        return self.prev[None] + y
 
     def window_step(self, predictor="mlp", net=None, threshold=THRESHOLD, floor=1.0):
         """One exchange window of `window` robot steps (see the module docstring)."""
@@ -235,12 +257,18 @@ class CutRun:
             raise ValueError(f"The network predicts at most {MAX_AGE} steps ahead")
         saved = self.arm.save()
         self._measured_at = self.arm.p.copy()
+        plan = [self.task.at((self.steps + k) * INTERVAL) for k in range(W)]
+        guesses = None
         preds, starts, targets = [], [], []
         for k in range(W):
-            origin, orientation, cutting, cut = self.task.at((self.steps + k) * INTERVAL)
-            w_hat = self.prev.copy() if predictor == "hold" else self._predict(net, k)
+            origin, orientation, cutting, cut = plan[k]
             starts.append(self.arm.p.copy())
             self.port.submit_records(self.arm.records(), INTERVAL, cutting=cutting, cut_index=cut, slot=k)
+            if k == 0 and predictor != "hold":
+                # The window's first interval starts from the arm's state now: it runs on the
+                # GPU while the host predicts the window.
+                guesses = self._predict_window(net, [(o, q) for o, q, _, _ in plan])
+            w_hat = self.prev.copy() if guesses is None else guesses[k]
             self.arm.step(origin, orientation, w_hat)
             preds.append(w_hat)
             targets.append((origin, orientation))
@@ -506,6 +534,14 @@ def _evaluate_command(args):
         report[name] = run.run(predictor, net if predictor == "mlp" else None, steps=args.steps)
         log(name, json.dumps(report[name]))
     report["speedup_vs_plain"] = report["mlp_batch"]["sim_per_wall"] / report["plain_single"]["sim_per_wall"]
+    # Fallback rate against the force error felt by the arm, as the residual threshold moves.
+    report["threshold_sweep"] = []
+    for threshold in (0.03, 0.05, 0.08):
+        for predictor in ("hold", "mlp"):
+            run = CutRun(args.instances, seed=args.seed, window=args.window)
+            r = run.run(predictor, net if predictor == "mlp" else None, steps=args.steps, threshold=threshold)
+            report["threshold_sweep"].append(dict(threshold=threshold, predictor=predictor, **r))
+            log("sweep", threshold, predictor, json.dumps(r))
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)

@@ -85,7 +85,16 @@ def test_trained_network_beats_holding_the_wrench_on_unseen_tasks():
     mlp = S.CutRun(8, seed=32, window=4).run("mlp", net, steps=1100)
     assert mlp["predictor_error"] < 0.95 * hold["predictor_error"], (mlp, hold)
     assert mlp["mean_residual"] < hold["mean_residual"], (mlp, hold)
-    # The impulse the arm receives over each exchange window stays within 2 % of the judged
-    # TLMPM impulse, and the force of single steps within 3 %.
-    assert mlp["window_force_error"] <= 0.02, mlp
-    assert mlp["force_error"] <= 0.03, mlp
+    # At the same 5 % threshold the network falls back less often, at the same speed, with the
+    # force felt by the arm within 2 % of the judged TLMPM force.
+    assert mlp["fallback_rate"] < hold["fallback_rate"], (mlp, hold)
+    assert mlp["sim_per_wall"] >= 0.9 * hold["sim_per_wall"], (mlp, hold)
+    assert mlp["force_error"] <= 0.02, mlp
+    # It accepts windows that hold rejects, so at the same threshold it feels more error; the
+    # fair comparison is at equal fallback rate.  Holding reaches the network's fallback rate
+    # only at a looser threshold, and there it feels at least the network's force error.
+    curve = [S.CutRun(8, seed=32, window=4).run("hold", steps=1100, threshold=t) for t in (0.06, 0.07, 0.08, 0.1)]
+    reach = [r for r in [hold] + curve if r["fallback_rate"] >= mlp["fallback_rate"]]
+    assert reach, (mlp, curve)
+    closest = min(reach, key=lambda r: r["fallback_rate"])
+    assert mlp["force_error"] <= closest["force_error"], (mlp, closest)
