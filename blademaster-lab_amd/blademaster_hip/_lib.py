@@ -92,6 +92,7 @@ STATE_FIELDS = [
     ("bc_sums", c_f32p), ("history_mask", ctypes.POINTER(ctypes.c_uint64)),
     ("viscous", ctypes.POINTER(ctypes.c_double)), ("viscous_impulse", c_f32p),
     ("extra_histories", c_f32p), ("extra_damage", c_f32p), ("extra_partial", c_f32p),
+    ("multi_nodes", c_i32p), ("multi_stamp", c_i32p), ("multi_count", c_i32p), ("field_idx_prev", c_u8p),
 ]
 
 

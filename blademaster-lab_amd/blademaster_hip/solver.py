@@ -152,6 +152,10 @@ class Solver:
         F_, N_ = self.fields, self.node_count
         T["field_code"] = t((F_, N_), i64, -1)
         T["node_nf"] = t((N_,), i32)
+        # Multi-field node list of the node kernels (appended by P2G, emptied every substep).
+        T["multi_nodes"] = t((N_,), i32)
+        T["multi_stamp"] = t((N_,), i32)
+        T["multi_count"] = torch.tensor([0, 1], dtype=i32, device=dev)
         T["grid_mass"] = t((F_, N_))
         for name in ("momentum", "grid_impulse", "old_v", "new_v", "remapped", "grid_vel"):
             T[name] = t((F_, N_, 3))
@@ -188,6 +192,7 @@ class Solver:
         T["fp_identity"] = t((1,), i32, 1)
         T["update_pending"] = t((1,), i32)
         T["update_done"] = t((max(n, 1),), i32)
+        T["field_idx_prev"] = t((max(n, 1), 8), torch.uint8)
         # pair contacts
         P_ = self.pair_capacity
         self.list_buckets = _pow2_at_least(2 * n)

@@ -345,6 +345,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
                                                   int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
+  // A new substep: empty the node kernels' multi-field node list.
+  if (p == 0 && S.multi_count != nullptr) {
+    S.multi_count[0] = 0;
+    S.multi_count[1] += 1;
+  }
   const float dt = float(P.dt);
   Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
@@ -1228,6 +1233,13 @@ __global__ void k_check_pair_list(const bm_params P, bm_state S) {
 // its 8 nodes, claiming the first empty slot when the code is new there.  Used slots stay a
 // prefix of the node's F slots; a node needing more than F fields counts errors[0].
 __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p) {
+  // While the particle's F update waits for the next P2G (update_pending 2), that update needs
+  // the slots of the grid velocities it was computed on: the first registration of the step
+  // keeps them in field_idx_prev and marks the particle.
+  const uint2 before = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  if (S.field_idx_prev != nullptr && S.update_pending != nullptr && S.update_pending[0] == 2 &&
+      atomicExch(S.update_done + p, 1) == 0)
+    reinterpret_cast<uint2*>(S.field_idx_prev)[p] = before;
   int64_t code = S.transfer_codes[p];
   int32_t base = S.base_node[p];
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
@@ -1511,9 +1523,16 @@ __device__ __forceinline__ void gather_cells(const bm_params& P, const Window& w
 template <bool kMx>
 __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_t p, const m3& rate,
                                    bool store_stress);
-template <bool kRemap, bool kMx>
+template <bool kMx>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx, bool store_stress);
+
+// Appends a node with several fields to the node kernels' list, once per epoch (many window
+// threads of neighbouring bricks see the same node).
+__device__ __forceinline__ void note_multi_node(const bm_state& S, int64_t node) {
+  const int epoch = S.multi_count[1];
+  if (atomicExch(S.multi_stamp + node, epoch) != epoch) S.multi_nodes[atomicAdd(S.multi_count, 1)] = int32_t(node);
+}
 
 template <bool kMx>
 __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
@@ -1544,6 +1563,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   bool live = p >= 0;
   int64_t my_node = int(threadIdx.x) < w.W ? window_node(P, w, D, int(threadIdx.x)) : -1;
   int my_nf = my_node >= 0 ? S.node_nf[my_node] : 0;
+  if (my_nf >= 2 && S.multi_nodes != nullptr) note_multi_node(S, my_node);
   if (F > 1) {
     for (int k = 0; k < 7; ++k)
       for (int i = threadIdx.x; i < FW; i += blockDim.x) win[k * FW + i] = 0.0f;
@@ -1557,20 +1577,23 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   if (live_mask != 0) {
     f3 vp = f3{0.f, 0.f, 0.f}, imp = vp, lp = vp;
     m3 Pk = mzero();
-    uint2 fidx = make_uint2(0u, 0u);
+    uint2 fidx = make_uint2(0u, 0u), ufidx = fidx;
     bool update = false;
     if (live) {
       vp = ld3(S.v, p);
       imp = ld3(S.impulses, p);
       fidx = reinterpret_cast<const uint2*>(S.field_idx)[p];
-      // pending 2: the particles marked in update_done were updated at the end of the last
-      // substep; their marks are cleared here.
-      const bool done = pending == 2 && S.update_done[p] != 0;
-      update = pending && !done;
-      if (done) S.update_done[p] = 0;
+      update = pending != 0;
+      // pending 2: a particle marked in update_done registered again this step; its update
+      // reads the grid velocities of its earlier slots.  The marks are cleared here.
+      ufidx = fidx;
+      if (pending == 2 && S.update_done[p] != 0) {
+        ufidx = reinterpret_cast<const uint2*>(S.field_idx_prev)[p];
+        S.update_done[p] = 0;
+      }
     }
     // The separable path needs no local coordinates: the slot gives them.
-    const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y) != 0u) == 0;
+    const bool separable = P.regular && __ballot(live && (fidx.x | fidx.y | ufidx.x | ufidx.y) != 0u) == 0;
     if (live && !separable) lp = ld3(S.local, p);
     const f3 sg = gradient_signs(lane, inv_h);
     if (separable) {
@@ -1583,7 +1606,7 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
         if (update) Pk = finish_update<kMx>(P, S, p, rate, false);
       }
     } else if (update) {
-      Pk = update_particle<false, kMx>(P, S, D, p, lp, fidx, false);
+      Pk = update_particle<kMx>(P, S, D, p, lp, ufidx, false);
     }
     if (live) {
       if (!update) Pk = ld9(S.stress, p);
@@ -1715,12 +1738,147 @@ __device__ __forceinline__ void node_coords(const bm_params& P, const Divisors& 
   nz = int(rem - int64_t(ny) * dz);
 }
 
+// Nodes of 2 .. kMultiFields fields are exchanged by the first kMultiBlocks workgroups of
+// each node kernel: four lanes per listed node, one per field.  The lanes load the list entry
+// (with the count), then their field's window sums, code and mass in one round, and trade
+// them by cross-lane reads within the four.  These workgroups are dispatched first and run
+// beside the node threads, which skip such nodes; a node thread that met one would otherwise
+// wait for one more round of loads per field, and on the few multi-field nodes of a cut that
+// chain, not bandwidth, sets the kernel's duration.
+constexpr int kMultiBlocks = 8;
+constexpr int kMultiFields = 4;
+
+__device__ __forceinline__ bool multi_listed(const bm_state& S, int nf) {
+  return S.multi_nodes != nullptr && nf >= 2 && nf <= kMultiFields;
+}
+__device__ __forceinline__ int64_t quad_read64(int64_t v, int src) {
+  const int lo = __shfl(int(uint32_t(uint64_t(v))), src, 64), hi = __shfl(int(uint64_t(v) >> 32), src, 64);
+  return int64_t((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo)));
+}
+
+// The exchange of couple_and_velocity (below) for the listed nodes; field f of a node is lane
+// f of its four, and the sums run over the fields in the same order as a node thread's.
+__device__ void multi_velocity(const bm_params& P, const bm_state& S, const Divisors& D) {
+  if (S.multi_nodes == nullptr) return;
+  const int f = int(threadIdx.x) & 3, base = int(threadIdx.x) & 60;
+  const int64_t N = P.node_count;
+  constexpr int kStride = kMultiBlocks * kBlock / 4;
+  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += kStride) {
+    const int count = S.multi_count[0];
+    const int64_t node = q < N ? S.multi_nodes[q] : 0;  // in bounds whatever the count
+    if (q >= count) return;
+    int nx, ny, nz;
+    node_coords(P, D, node, nx, ny, nz);
+    float g[7];
+    int64_t code = 0;
+    if (f < int(P.fields)) {
+      gather_node<7>(P, D, S.p2g_partial, nx, ny, nz, f, g);
+      code = S.field_code[int64_t(f) * N + node];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) g[c] = 0.0f;
+    }
+    const int nf = S.node_nf[node];
+    if (nf < 2 || nf > kMultiFields) continue;  // the same for the four lanes
+    float mb[kMultiFields], pb[kMultiFields][3], jb[kMultiFields][3];
+    int64_t cb[kMultiFields];
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b) {
+      mb[b] = __shfl(g[0], base + b, 64);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        pb[b][k] = __shfl(g[1 + k], base + b, 64);
+        jb[b][k] = __shfl(g[4 + k], base + b, 64);
+      }
+      cb[b] = quad_read64(code, base + b);
+    }
+    float total = 0.0f;
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b)
+      if (b < nf) total += mb[b];
+    if (f >= nf) continue;
+    const int64_t sa = int64_t(f) * N + node;
+    const float ma = g[0];
+    S.grid_mass[sa] = ma;
+    if (!(ma > 0.0f)) continue;
+    f3 pa = f3{g[1], g[2], g[3]}, ja = f3{g[4], g[5], g[6]};
+    f3 dm = f3{0.f, 0.f, 0.f}, di = dm;
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b) {
+      if (b == f || b >= nf || !compatible(code, cb[b])) continue;
+      dm = dm + (ma * f3{pb[b][0], pb[b][1], pb[b][2]} - mb[b] * pa);
+      di = di + (ma * f3{jb[b][0], jb[b][1], jb[b][2]} - mb[b] * ja);
+    }
+    f3 pm = pa + dm / total, jm = ja + di / total;
+    st3(S.old_v, sa, pm / ma);
+    st3(S.new_v, sa, (pm + jm) / ma);
+  }
+}
+
+// The exchange of couple_remapped (below) for the listed nodes.
+__device__ void multi_remapped(const bm_params& P, const bm_state& S, const Divisors& D) {
+  if (S.multi_nodes == nullptr) return;
+  const int f = int(threadIdx.x) & 3, base = int(threadIdx.x) & 60;
+  const int64_t N = P.node_count;
+  constexpr int kStride = kMultiBlocks * kBlock / 4;
+  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += kStride) {
+    const int count = S.multi_count[0];
+    const int64_t node = q < N ? S.multi_nodes[q] : 0;
+    if (q >= count) return;
+    int nx, ny, nz;
+    node_coords(P, D, node, nx, ny, nz);
+    float g[3], m = 0.0f;
+    int64_t code = 0;
+    if (f < int(P.fields)) {
+      gather_node<3>(P, D, S.remap_partial, nx, ny, nz, f, g);
+      m = S.grid_mass[int64_t(f) * N + node];
+      code = S.field_code[int64_t(f) * N + node];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[c] = 0.0f;
+    }
+    const int nf = S.node_nf[node];
+    if (nf < 2 || nf > kMultiFields) continue;
+    float mb[kMultiFields], xb[kMultiFields][3];
+    int64_t cb[kMultiFields];
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b) {
+      mb[b] = __shfl(m, base + b, 64);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xb[b][k] = __shfl(g[k], base + b, 64);
+      cb[b] = quad_read64(code, base + b);
+    }
+    float total = 0.0f;
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b)
+      if (b < nf) total += mb[b];
+    if (f >= nf) continue;
+    const int64_t sa = int64_t(f) * N + node;
+    if (m <= 0.0f) {
+      st3(S.grid_vel, sa, f3{0.f, 0.f, 0.f});
+      continue;
+    }
+    f3 xa = f3{g[0], g[1], g[2]};
+    f3 ex = f3{0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < kMultiFields; ++b) {
+      if (b == f || b >= nf || !compatible(code, cb[b])) continue;
+      ex = ex + (m * f3{xb[b][0], xb[b][1], xb[b][2]} - mb[b] * xa);
+    }
+    st3(S.grid_vel, sa, (xa + ex / total) / m);
+  }
+}
+
 // Compatible-field exchange of one node (Eq. 8, material.py:252-285) on the gathered P2G
 // sums: x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b; then
 // old_v = p / m, new_v = (p + j) / m.
 
 __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S, const Divisors D) {
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (int(blockIdx.x) < kMultiBlocks) {
+    multi_velocity(P, S, D);
+    return;
+  }
+  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
   // Every P2G block has read update_pending by now.
   if (i == 0 && S.update_pending != nullptr) S.update_pending[0] = 0;
   if (i >= active_node_count(P)) return;
@@ -1745,6 +1903,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
     }
     return;
   }
+  if (multi_listed(S, nf)) return;  // exchanged by the multi-field workgroups
   // Several fields: gather all, then exchange.  Up to kMaxLocal fields stay in registers;
   // more go through the grid arrays.
   if (nf <= kMaxLocal) {
@@ -2038,22 +2197,24 @@ __global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int
     out[int64_t(i) * 3 + 2] = sum[2];
   }
 }
-template <bool kRemap, bool kMx>
-__device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks);
+__device__ void defer_update(const bm_state& S, int block);
 
 // couple (material.py:155-179) of the gathered re-scattered momentum, then the field
 // velocity p / m read by the F update (0 on massless fields, which the update skips).
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
-// update to the next P2G (deferred_update), so no separate F-update launch follows.
-template <bool kMx>
+// update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
                                                             int update_blocks) {
-  const int node_blocks = int(gridDim.x) - update_blocks;
-  if (int(blockIdx.x) >= node_blocks) {
-    deferred_update<true, kMx>(P, S, D, int(blockIdx.x) - node_blocks, update_blocks);
+  if (int(blockIdx.x) < kMultiBlocks) {
+    multi_remapped(P, S, D);
     return;
   }
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int node_end = int(gridDim.x) - update_blocks;
+  if (int(blockIdx.x) >= node_end) {
+    defer_update(S, int(blockIdx.x) - node_end);
+    return;
+  }
+  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
   int64_t N = P.node_count;
@@ -2073,6 +2234,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
     st3(S.grid_vel, node, vel);
     return;
   }
+  if (multi_listed(S, nf)) return;  // exchanged by the multi-field workgroups
   constexpr int kMaxLocal = 4;
   if (nf <= kMaxLocal) {
     float g[kMaxLocal][3], m[kMaxLocal];
@@ -2138,36 +2300,6 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
     }
     st3(S.grid_vel, sa, (xa + ex / total) / ma);
   }
-}
-
-// The velocity k_couple_remapped stores for field a of a node, computed here from the
-// re-scattered partials with the same operations in the same order, for an F update that
-// runs beside that kernel.
-__device__ f3 remapped_velocity(const bm_params& P, const bm_state& S, const Divisors& D, int64_t node, int a) {
-  const int64_t N = P.node_count;
-  int nx, ny, nz;
-  node_coords(P, D, node, nx, ny, nz);
-  const int nf = S.node_nf[node];
-  float g[3];
-  gather_node<3>(P, D, S.remap_partial, nx, ny, nz, a, g);
-  const int64_t sa = int64_t(a) * N + node;
-  const float ma = S.grid_mass[sa];
-  const f3 xa = f3{g[0], g[1], g[2]};
-  if (nf <= 1) return ma > 0.0f ? xa / ma : f3{0.f, 0.f, 0.f};
-  if (ma <= 0.0f) return f3{0.f, 0.f, 0.f};
-  float total = 0.0f;
-  for (int f = 0; f < nf; ++f) total += S.grid_mass[int64_t(f) * N + node];
-  const int64_t ca = S.field_code[sa];
-  f3 ex = f3{0.f, 0.f, 0.f};
-  for (int b = 0; b < nf; ++b) {
-    if (b == a) continue;
-    const int64_t sb = int64_t(b) * N + node;
-    if (!compatible(ca, S.field_code[sb])) continue;
-    float gb[3];
-    gather_node<3>(P, D, S.remap_partial, nx, ny, nz, b, gb);
-    ex = ex + (ma * f3{gb[0], gb[1], gb[2]} - S.grid_mass[sb] * xa);
-  }
-  return (xa + ex / total) / ma;
 }
 
 // Lame constants of particle p: its instance's for batched instances, else the solver's.
@@ -2397,96 +2529,54 @@ __device__ inline m3 finish_update(const bm_params& P, const bm_state& S, int64_
 }
 
 // update_F_plastic (material.py:445-475) for one particle: F += dt Σ_a v_a ⊗ ∇w_a over its
-// 8 corners, then the stress.  The corner velocities of its field come straight from
-// grid_vel (a node array small enough to stay in L2), or with kRemap from the re-scattered
-// partials (remapped_velocity), for an update that runs beside k_couple_remapped.  Stores F
-// (and Fp after plastic flow) and the stress, and returns the stress.
-template <bool kRemap, bool kMx>
+// 8 corners, then the stress.  The corner velocities of the field slots `fidx` come straight
+// from grid_vel (a node array small enough to stay in L2).  Stores F (and Fp after plastic
+// flow) and, with store_stress, the stress, and returns the stress.
+template <bool kMx>
 __device__ inline m3 update_particle(const bm_params& P, const bm_state& S, const Divisors& D, int64_t p, f3 lp,
                                      uint2 fidx, bool store_stress) {
+  (void)D;
   int32_t base = S.base_node[p];
   int64_t N = P.node_count;
   int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
   float inv_h = 1.0f / float(P.spacing);
   m3 rate = mzero();
-  if constexpr (kRemap) {
-    // One corner at a time: each velocity is a gather over bricks and fields.
-#pragma unroll 1
-    for (int a = 0; a < 8; ++a) {
-      f3 v = remapped_velocity(P, S, D, base + corner_offset(a, dyz, dz), field_of(fidx, a));
-      float wt;
-      f3 g;
-      stencil_corner(lp, inv_h, a, wt, g);
-      rate.a[0] += v.x * g.x; rate.a[1] += v.x * g.y; rate.a[2] += v.x * g.z;
-      rate.a[3] += v.y * g.x; rate.a[4] += v.y * g.y; rate.a[5] += v.y * g.z;
-      rate.a[6] += v.z * g.x; rate.a[7] += v.z * g.y; rate.a[8] += v.z * g.z;
-    }
-  } else {
-    f3 vel[8];
+  f3 vel[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
+  for (int a = 0; a < 8; ++a) vel[a] = ld3(S.grid_vel, int64_t(field_of(fidx, a)) * N + (base + corner_offset(a, dyz, dz)));
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      float wt;
-      f3 g;
-      stencil_corner(lp, inv_h, a, wt, g);
-      rate.a[0] += vel[a].x * g.x; rate.a[1] += vel[a].x * g.y; rate.a[2] += vel[a].x * g.z;
-      rate.a[3] += vel[a].y * g.x; rate.a[4] += vel[a].y * g.y; rate.a[5] += vel[a].y * g.z;
-      rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
-    }
+  for (int a = 0; a < 8; ++a) {
+    float wt;
+    f3 g;
+    stencil_corner(lp, inv_h, a, wt, g);
+    rate.a[0] += vel[a].x * g.x; rate.a[1] += vel[a].x * g.y; rate.a[2] += vel[a].x * g.z;
+    rate.a[3] += vel[a].y * g.x; rate.a[4] += vel[a].y * g.y; rate.a[5] += vel[a].y * g.z;
+    rate.a[6] += vel[a].z * g.x; rate.a[7] += vel[a].z * g.y; rate.a[8] += vel[a].z * g.z;
   }
   return finish_update<kMx>(P, S, p, rate, store_stress);
 }
 
-// The deferred F update of a substep, run by `blocks` workgroups (this is workgroup `block`
-// of them): with kRemap extra workgroups of k_couple_remapped, else a k_update_F launch after
-// it.  The update is left to the next P2G, which updates
-// each particle and uses its stress at once (update_pending tells that P2G), for every
-// particle whose field slots stay as they are: all of them without a label change; after
-// relabelling only, all but the particles of the 27 cells around each relabelled one, which
-// register again before that P2G and are updated here (once each, marked in update_done).
-// A full re-registration (rare: more relabels than the changed list holds) updates every
-// particle here.  Running beside the node kernel (kRemap), these updates read their corner
-// velocities through remapped_velocity.
-template <bool kRemap, bool kMx>
-__device__ void deferred_update(const bm_params& P, const bm_state& S, const Divisors& D, int block, int blocks) {
+// The deferred F update of a substep (workgroup `block` of the extra workgroups of
+// k_couple_remapped, or of a k_update_F launch after it): the update of every particle is
+// left to the next P2G, which updates each particle and uses its stress at once.
+// update_pending tells that P2G; it is 2 when this step relabelled particles, so that the next
+// registration keeps the earlier field slots of the particles it registers again
+// (insert_particle) for their updates.
+__device__ void defer_update(const bm_state& S, int block) {
   const int topo = S.topology_dirty[0];
-  const int64_t first = int64_t(block) * blockDim.x + threadIdx.x, stride = int64_t(blocks) * blockDim.x;
-  const uint2* fidx = reinterpret_cast<const uint2*>(S.field_idx);
-  if (topo & kTopoFull) {
-    for (int64_t p = first; p < P.n; p += stride) update_particle<kRemap, kMx>(P, S, D, p, ld3(S.local, p), fidx[p], true);
-    return;
-  }
-  // 1: every particle waits for the next P2G; 2: some were updated here (update_done).
-  if (first == 0) S.update_pending[0] = (topo & kTopoChanged) ? 2 : 1;
-  if ((topo & kTopoChanged) == 0) return;
-  int64_t count = min(S.changed_count[0], int(P.changed_capacity));
-  int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
-  const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
-  const int64_t cpb = int64_t(BX) * BY * BZ;
-  for (int64_t a = first; a < count * 27 * 8; a += stride) {
-    int64_t base = S.base_node[S.changed[a / (27 * 8)]];
-    int k = int((a >> 3) % 27);
-    int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
-    if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
-      continue;
-    int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
-    int32_t q = S.perm[8 * (brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ) + (a & 7)];
-    if (q >= 0 && atomicExch(S.update_done + q, 1) == 0)
-      update_particle<kRemap, kMx>(P, S, D, q, ld3(S.local, q), fidx[q], true);
-  }
+  if (block == 0 && threadIdx.x == 0) S.update_pending[0] = (topo & (kTopoChanged | kTopoFull)) ? 2 : 1;
 }
 
 // The F update of a substep, one lane per particle slot of a brick; with `defer`, the
 // deferred update above.  The workgroups stride over the bricks (at most kUpdateThreads
 // threads in all, in a multiple of 8 workgroups, so a workgroup keeps the bricks of its XCD).
 constexpr int kUpdateThreads = 1 << 20;
-constexpr int kDeferredUpdateBlocks = 128;
+constexpr int kDeferredUpdateBlocks = 1;
 template <bool kMx>
 __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S, const Divisors D, int defer,
                                                    int nbricks) {
   if (defer) {
-    deferred_update<false, kMx>(P, S, D, int(blockIdx.x), int(gridDim.x));
+    defer_update(S, int(blockIdx.x));
     return;
   }
   int cpb = cells_per_brick(P), CS = 8 * cpb;
@@ -2508,7 +2598,7 @@ __global__ void __launch_bounds__(1024) k_update_F(const bm_params P, bm_state S
       m3 rate = separable_rate(vc, sg);
       if (live) finish_update<kMx>(P, S, p, rate, true);
     } else if (live) {
-      update_particle<false, kMx>(P, S, D, p, ld3(S.local, p), fidx, true);
+      update_particle<kMx>(P, S, D, p, ld3(S.local, p), fidx, true);
     }
   }
 }
@@ -3159,12 +3249,11 @@ static int brick_threads(const bm_params* p) {
 }
 // The node kernel of the re-scattered momentum (with `update_blocks` deferred-update
 // workgroups among `blocks`) and the F update, in the instantiation of the material model.
+// `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
                                    int update_blocks, hipStream_t st) {
-  if (p->maxwell)
-    hipLaunchKernelGGL(k_couple_remapped<true>, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
-  else
-    hipLaunchKernelGGL(k_couple_remapped<false>, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
+  blocks += kMultiBlocks;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
                             int defer, int nbricks, hipStream_t st) {
@@ -3190,7 +3279,8 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
     hipLaunchKernelGGL(k_p2g<true>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
   else
     hipLaunchKernelGGL(k_p2g<false>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
-  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s, D);
+  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s,
+                     D);
   // Viscosity: the impulse of the coupled old velocities, coupled and added to new_v
   // (solver.py:433-458).
   if (p->viscosity > 0.0) {
@@ -3205,18 +3295,16 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   if (p->tools > 1 && (g2p_flags & 1))
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, digit, 1, 1);
   const int node_blocks = int(blocks_for(node_launch(p)));
-  defer = defer && s->update_pending && s->update_done;
-  if (defer && s->instance == nullptr) {
-    // The F update is deferred to the next P2G: a few extra workgroups of the node kernel set
-    // that up (and make the early updates), and no F-update launch follows.
+  defer = defer && s->update_pending && s->update_done && s->field_idx_prev;
+  if (defer) {
+    // The F update is deferred to the next P2G: one extra workgroup of the node kernel sets
+    // that up, and no F-update launch follows.
     launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st);
     return;
   }
-  // Batched instances relabel in many instances each substep: their early updates read
-  // grid_vel after the node kernel, in a small deferred launch.
   launch_couple_remapped(p, s, D, node_blocks, 0, st);
-  int ub = defer ? kDeferredUpdateBlocks : update_blocks(tb);
-  launch_update_F(p, s, D, nb < ub ? nb : ub, tb, defer ? 1 : 0, nb, st);
+  const int ub = update_blocks(tb);
+  launch_update_F(p, s, D, nb < ub ? nb : ub, tb, 0, nb, st);
 }
 }  // namespace
 
@@ -3258,7 +3346,8 @@ int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_couple_velocity, dim3(blocks_for(node_launch(p))), dim3(kBlock), 0, (hipStream_t)stream, *p,
+  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p))), dim3(kBlock), 0,
+                     (hipStream_t)stream, *p,
                      *s, divisors_of(p));
   return launch_status();
 }

@@ -99,9 +99,10 @@ typedef struct bm_state {
   int32_t *errors, *topology_dirty;
   int32_t *fp_identity;    /* [1] 1 while every Fp is exactly I (no plastic flow, no host write) */
   int32_t *update_pending; /* [1] 1 while the F update of the last substep waits for the next
-                            * P2G (bm_substep_deferred); null disables the deferral */
-  int32_t *update_done;    /* [n] 1 for a particle whose deferred update already ran (it is
-                            * registered again before the next P2G); zero between substeps */
+                            * P2G (bm_substep_deferred), 2 when also some particles register
+                            * again before it; null disables the deferral */
+  int32_t *update_done;    /* [n] 1 for a particle registered again while its update waits: its
+                            * earlier field slots are in field_idx_prev; zero between substeps */
   /* batched instances (null for a single solver): the instance of each particle and its Lame
    * constants; tool_pos/vel/rot/omega, mobility, reaction, torque, board, work and the
    * interval sums are then per instance ([instances][...]) */
@@ -176,6 +177,16 @@ typedef struct bm_state {
   float *extra_histories;
   float *extra_damage;
   float *extra_partial;
+  /* Nodes with 2..4 fields, for the node kernels' exchange lanes: the transfer kernel appends
+   * each window node it finds with node_nf >= 2 once per epoch; the contact kernel empties
+   * the list and advances the epoch at the start of a substep.  Stale or repeated entries
+   * are harmless (the node kernels re-read node_nf).  Null: no list, the node threads do
+   * the exchange themselves. */
+  int32_t *multi_nodes;    /* [N] */
+  int32_t *multi_stamp;    /* [N] epoch of a node's last append */
+  int32_t *multi_count;    /* [2] entries, epoch (starts at 1) */
+  uint8_t *field_idx_prev; /* [n][8] field slots before a registration that ran while the
+                            * particle's F update waited (marked in update_done) */
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */
