@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(kBlock) k_extra_blades(const bm_params P, bm_s
 }
 
 __device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
-__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
 
 // Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
 // the single-thread work of the step: apply the sums, list reach, knife update.  Other
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     return;
   }
   if (blockIdx.x > 0) {
-    // Relabelled particles with a current pair list: mark the lists to rebuild this step.
+    // Relabelled particles with a current pair list: add their new candidate pairs.
     if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
     int count = min(S.changed_count[0], int(P.changed_capacity));
     int first = int(blockIdx.x) - 1 - insert_blocks, blocks = int(gridDim.x) - 1 - insert_blocks;
@@ -706,7 +706,11 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     for (int i = first * kWaves + int(threadIdx.x >> 6); i < count; i += blocks * kWaves) {
       int32_t d = S.changed[i];
       ListRegion L = list_region(P, S, instance_of(S, d));
-      if (!L.build) mark_pair_neighbours(P, S, d, lane, L);  // a list rebuilt in full needs no marks
+      // A particle relabelled by two knives in one step is listed twice: one walk is enough.
+      int seen = 0;
+      if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
+      seen = __shfl(seen, 0, 64);
+      if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
     }
     return;
   }
@@ -1058,9 +1062,6 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
-  // The particle's two list words are loaded together up front: in the common case (no
-  // rebuild, most lists empty) the thread waits for one round of loads.
-  const int32_t mark_p = S.pair_mark[p];
   const int32_t listed = S.partner_count[p];
   // Registration, second half (the anchors claimed their slots in k_prep).
   if (topology && (S.topology_dirty[0] & kTopoFull) != 0) insert_particle(P, S, p);
@@ -1083,10 +1084,17 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   int lanes = int(P.list_lanes);
   int count;
   bool dirty = L.build;
-  // A full rebuild uses the current positions; a rebuild after relabelling only (this
-  // particle or a neighbour got a new code) keeps the positions of the last full build, so
-  // every list stays valid for the same skin.
-  bool rebuild = dirty || mark_p == S.pair_epoch[0] + 1;
+  // A rebuild uses the current positions.  Relabelled particles do not rebuild: the contact
+  // kernel added their new pairs to the lists (add_relabelled_pairs), and their build codes
+  // are brought up to date here.
+  if (P.pair_mode == kPairList) {
+    const int count = min(S.changed_count[0], int(P.changed_capacity));
+    for (int64_t i = p; i < count; i += int64_t(gridDim.x) * blockDim.x) {
+      const int32_t d = S.changed[i];
+      S.built_codes[d] = S.codes[d];
+    }
+  }
+  const bool rebuild = dirty;
   const float* bx = dirty ? S.x : S.built_x;
   if (rebuild) {
     f3 xp = ld3(bx, p);
@@ -1183,31 +1191,60 @@ __global__ void k_pairs_done(const bm_params P, bm_state S) {
   S.pair_epoch[0] += 1;
 }
 
-// Marks relabelled particle d and every particle within the list reach of d at the build
-// positions: the pairs whose candidate status can change with d's code.  One wave per
-// particle: the lanes take the cells of the search cube (those near enough to matter) and walk
-// their bucket chains side by side; every mark written is the same value.
-__device__ void mark_pair_neighbours(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
-  int mark = S.pair_epoch[0] + 1;
-  if (lane == 0) S.pair_mark[d] = mark;
-  int tag = L.tag;
-  float reach = S.list_reach[0];
-  const float reach2 = reach * reach;
-  float cell = list_cell_size(P);
-  int rings = list_rings(P, reach);
-  f3 xd = ld3(S.built_x, d);
-  Cell home = cell_of(xd, list_cell_inv(P));
+// After particle d's code changed, adds the pairs (d, q) that pair_at no longer skips to the
+// partner lists, so the lists need no rebuild: q within the list reach of d at the build
+// positions, the pair now a candidate (codes incompatible or references apart).  A pair goes
+// to the list of its lower index unless it is there already; when q was relabelled too (its
+// code differs from its build code), the walk of the lower of the two adds it.  A list that
+// fills up turns into a bucket search (-1).  Pairs that become compatible stay listed and
+// pair_at skips them, as with any listed pair.  Candidates only grow by a relabel of the
+// knives (a code gains a digit), and the lists keep the Verlet guarantee of the last build.
+// One wave per relabelled particle: the lanes take the cells of the search cube (those near
+// enough to matter) and walk their bucket chains side by side.
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
+  const int tag = L.tag;
+  const float reach = S.list_reach[0];
+  const float reach2 = reach * reach, two_h = 2.0f * float(P.spacing);
+  const float cell = list_cell_size(P), inv = list_cell_inv(P);
+  const int rings = list_rings(P, reach);
+  const int lanes = int(P.list_lanes);
+  const f3 xd = ld3(S.built_x, d), rd = ld3(S.ref, d);
+  const int64_t cd = S.codes[d];
+  const Cell home = cell_of(xd, inv);
   CellCull cull(xd, home, cell, reach);
-  int side = 2 * rings + 1;
+  const int side = 2 * rings + 1;
   for (int k = lane; k < side * side * side; k += 64) {
-    int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
+    const int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
     if (!cull.near(dx, dy, dz)) continue;
-    Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+    const Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
     int64_t step = 0;  // a chain holds each particle at most once
     for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), tag); q >= 0 && step < P.n;
          q = S.list_links[q], ++step) {
-      f3 dd = ld3(S.built_x, q) - xd;
-      if (dot(dd, dd) < reach2) S.pair_mark[q] = mark;
+      if (q == d) continue;
+      const f3 xq = ld3(S.built_x, q);
+      const Cell cq = cell_of(xq, inv);
+      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+      const f3 dd = xq - xd;
+      if (!(dot(dd, dd) < reach2)) continue;
+      const int64_t cq_code = S.codes[q];
+      if (compatible(cd, cq_code) && len(rd - ld3(S.ref, q)) < two_h) continue;
+      if (q < d && cq_code != S.built_codes[q]) continue;  // q's own walk adds the pair
+      const int32_t owner = min(d, q), other = max(d, q);
+      int32_t* count = S.partner_count + owner;
+      const int32_t* list = S.partners + int64_t(owner) * lanes;
+      int32_t n = *count;
+      bool listed = false;
+      for (int i = 0; i < n && i < lanes; ++i) listed = listed || list[i] == other;
+      // Append (the other walks append other partners to the same list concurrently).
+      while (!listed && n >= 0) {
+        const int32_t next = n < lanes ? n + 1 : -1;
+        const int32_t seen = atomicCAS(count, n, next);
+        if (seen == n) {
+          if (next > 0) S.partners[int64_t(owner) * lanes + n] = other;
+          break;
+        }
+        n = seen;
+      }
     }
   }
 }
