@@ -74,6 +74,32 @@ def interval_schedule(steps, warmup, per=INTERVAL_SUBSTEPS):
     return warm, timed
 
 
+_SLEEP_CYCLES_PER_MS = []
+
+
+def hold_stream(stream, ms=3.0):
+    """Keep `stream` busy for about `ms` milliseconds with a bounded spin kernel.
+
+    Enqueued just before a timed region's start event, it lets the host enqueue the first
+    timed interval (tool-state staging, graph launch) while the device still spins, so the
+    events time the device's work and not the host's first enqueue.  Later intervals are
+    enqueued while earlier ones run (the host is many times faster than an interval).
+    """
+    import torch
+
+    if not _SLEEP_CYCLES_PER_MS:
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        probe = 1_000_000
+        torch.cuda._sleep(probe)
+        a.record(stream)
+        torch.cuda._sleep(probe)
+        b.record(stream)
+        torch.cuda.synchronize()
+        _SLEEP_CYCLES_PER_MS.append(probe / max(a.elapsed_time(b), 1e-3))
+    torch.cuda._sleep(int(ms * _SLEEP_CYCLES_PER_MS[0]))
+
+
 def workload(seed=0, n_side=(0.3, 0.15, 0.075)):
     from blademaster_hip import Config
 
@@ -139,13 +165,12 @@ def run_gpu(args):
     start_codes = int(torch.count_nonzero(s._t["codes"]).item())
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    wall0 = time.perf_counter()
+    hold_stream(stream)
     start.record(stream)
     for length in timed:
         last = submit(length)
     stop.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
     gpu_ms = start.elapsed_time(stop)
     steps_timed = sum(timed)
     reaction = port.collect(last, timed[-1] * cfg.dt, 2)
@@ -207,7 +232,6 @@ def run_gpu(args):
         E=E,
         rate=rate,
         ms_per_step=ms_per_step,
-        wall=wall,
         gpu_ms=gpu_ms,
         steps=steps_timed,
         warmup=sum(warm),
@@ -284,6 +308,7 @@ def run_batch(args):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    hold_stream(stream)
     a.record(stream)
     for length in timed:
         last = submit(length)
@@ -363,6 +388,7 @@ def run_mls(args):
         # Each interval ends with the reaction's readback: time the device work of each
         # interval and add them (the domain check runs once, after the timed intervals).
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        hold_stream(stream)
         a.record(stream)
         advance(length, check=False)
         b.record(stream)
@@ -524,7 +550,7 @@ def main():
             "brick": [int(v) for v in s.brick],
         },
         "hbm_peak_frac": SUBSTEP_BYTES * r["rate"] / (peak * 1e9),
-        "sim_per_wall": r["steps"] * cfg.dt / r["wall"],
+        "sim_per_device_s": r["steps"] * cfg.dt / (r["gpu_ms"] * 1e-3),
         "roofline": {
             "bound": "hbm",
             "kernel": r["dominant"],
