@@ -178,6 +178,7 @@ SIGNATURES = {
     "bm_finish_external": [P, S, VP],
     "bm_accumulate_reaction": [P, S, VP],
     "bm_assign": [P, S, I64, I32, VP],
+    "bm_mark_relabelled": [P, S, VP],
     "bm_substep": [P, S, I32, I32, I64, I32, I32, VP, VP, I32, VP],
     "bm_substep_deferred": [P, S, I32, I32, I64, I32, I32, VP, VP, I32, VP],
     "bm_load_external_state": [P, S, VP, VP],

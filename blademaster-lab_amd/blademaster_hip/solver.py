@@ -242,7 +242,9 @@ class Solver:
         nb = int(np.prod(self.bricks))
         T["p2g_partial"] = t((max(nb * F_ * window * 7, 1),))
         T["remap_partial"] = t((max(nb * F_ * window * 3, 1),))
-        self.rigid_blocks = max(1, -(-n // 256))
+        # Rows of contact sums: one per contact-kernel workgroup, then 16 the pair kernel reduces
+        # them to.
+        self.rigid_blocks = max(1, -(-n // 256)) + 16
         T["rigid_partial"] = t((self.rigid_blocks * 12,))
         if self.tools > 1:
             # Knives 1 .. K-1: friction history row, damage and per-block contact sums of each.

@@ -289,14 +289,20 @@ struct ListRegion {
 };
 constexpr int kPairList = 0, kPairHash = 1, kPairAll = 2;  // bm_params.pair_mode
 // The search lists of the particles of instance inst; `build` when they are rebuilt this step
-// (always in the hash mode, which queries the cell buckets every step).
+// (always in the hash mode, which queries the cell buckets every step).  The build count is
+// advanced once the step's pair kernel is done, so a list rebuilt this step carries the tag
+// count + 1 from its binning on.
 __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_state& S, int inst) {
   const bool every_step = P.pair_mode == kPairHash;
-  if (S.instance == nullptr)
-    return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0], every_step || S.pair_dirty[0] != 0};
-  return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, S.inst_pair_builds[inst],
-                    every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0};
+  if (S.instance == nullptr) {
+    const bool build = every_step || S.pair_dirty[0] != 0;
+    return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0] + (build ? 1 : 0), build};
+  }
+  const bool build = every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0;
+  return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1,
+                    S.inst_pair_builds[inst] + (build ? 1 : 0), build};
 }
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
 
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
                                              const ListRegion& L);
@@ -337,10 +343,29 @@ __device__ inline f3 resist_cut(f3 j, f3 xp, f3 vp, f3 origin, f3 tvel, quat q, 
   return j;
 }
 
+// Particles relabelled this step (assign) add their new candidate pairs to the partner lists
+// that the next step's pair kernel uses (add_relabelled_pairs), unless those lists are rebuilt
+// anyway; one wave per relabelled particle: wave `wave` of `waves`.
+__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves) {
+  if (S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
+  const int count = min(S.changed_count[0], int(P.changed_capacity));
+  const int lane = int(threadIdx.x) & 63;
+  for (int i = wave; i < count; i += waves) {
+    const int32_t d = S.changed[i];
+    ListRegion L = list_region(P, S, instance_of(S, d));
+    // A particle relabelled by two knives in one step is listed twice: one walk is enough.
+    int seen = 0;
+    if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
+    seen = __shfl(seen, 0, 64);
+    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
+  }
+}
+
 // rigid_and_speed (contact.py:332-497): gravity, board and blade contact of every particle.
-// Per-block sums go to rigid_partial; k_prep reduces them.  In a substep the same pass also
-// empties the pair slots of the new table, bins particles for a pair-list rebuild and clears
-// the field table for a re-registration.
+// Per-block sums go to rigid_partial (reduced by the pair kernel's first workgroups and the
+// node kernel's last).  In a substep (kPairPrep) the same pass also empties the pair slots of
+// the new table and resets its insertion count, bins particles for a pair-list rebuild and
+// clears the field table for a re-registration.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
                                                   int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -350,6 +375,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     S.multi_count[0] = 0;
     S.multi_count[1] += 1;
   }
+  // The pair kernel inserts into the new table from an empty count.
+  if (p == 0 && (mode & kPairPrep) != 0) S.counters[1 + parity_new] = 0u;
   const float dt = float(P.dt);
   Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
   f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
@@ -378,7 +405,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     }
   } else if (topo & kTopoChanged) {
     // Only the stencil nodes of the particles relabelled last step can change their codes:
-    // empty those (the particles around them register again in k_prep).
+    // empty those (the particles around them register again in the pair kernel).
     int64_t count = min(S.changed_count[0], int(P.changed_capacity));
     int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
     for (int64_t a = p; a < count * 8; a += stride) {
@@ -531,8 +558,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     float acc = wsum[0][k];
     for (int w = 1; w < kBlock / 64; ++w) acc = (k == 10) ? fmaxf(acc, wsum[w][k]) : acc + wsum[w][k];
     if (S.instance != nullptr && k < 10) {
-      // One add per sum and block into a slot of the block's instance (k_prep reads only the
-      // speed of rigid_partial for batched instances).
+      // One add per sum and block into a slot of the block's instance (the step's finish reads
+      // only the speed of rigid_partial for batched instances).
       if (acc != 0.0f) atomicAdd(S.inst_sums + (int64_t(block_inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10 + k, acc);
       acc = 0.0f;
     }
@@ -633,94 +660,63 @@ __global__ void __launch_bounds__(kBlock) k_extra_blades(const bm_params P, bm_s
   if (live) st3(S.impulses, p, imp);
 }
 
-__device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p);
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
-
-// Block 0: reduction of the contact kernel's block sums in block order (deterministic), then
-// the single-thread work of the step: apply the sums, list reach, knife update.  Other
-// blocks: registration of every particle's field slots when the topology changed.
-constexpr int kPrepBlock = 1024;  // k_prep: block 0 reduces the partials with 16 waves
-
-__global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt,
-                                                 int parity_new, int insert_blocks) {
-  if (blockIdx.x > 0 && int(blockIdx.x) <= insert_blocks) {
-    const int topo = S.topology_dirty[0];
-    if ((mode & kTopologyPrep) == 0 || topo == 0) return;
-    if ((topo & kTopoFull) == 0) {
-      // After relabelling only: every particle of the 27 cells around a relabelled particle's
-      // cell touches a node emptied in k_rigid and registers again; the other particles' nodes
-      // and slots are unchanged.  A particle met twice registers twice, finding its slots.
-      int64_t count = min(S.changed_count[0], int(P.changed_capacity));
-      int64_t dz = P.dims[2], dyz = P.dims[1] * P.dims[2];
-      const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
-      const int64_t cpb = int64_t(BX) * BY * BZ;
-      int64_t stride = int64_t(insert_blocks) * blockDim.x;
-      // One thread per (relabelled particle, neighbouring cell, particle slot).
-      for (int64_t a = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; a < count * 27 * 8; a += stride) {
-        int64_t base = S.base_node[S.changed[a / (27 * 8)]];
-        int k = int((a >> 3) % 27);
-        int64_t cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= P.bricks[0] * BX || cy >= P.bricks[1] * BY || cz >= P.bricks[2] * BZ)
-          continue;
-        int64_t brick = ((cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
-        int64_t cell = brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ;
-        int32_t q = S.perm[8 * cell + (a & 7)];
-        if (q >= 0) insert_particle(P, S, q);
-      }
-      return;
+// register (material.py:50-65): the field slot of code `code` at node `node`, claiming the
+// first empty slot when the code is new there.  Used slots stay a prefix of the node's F
+// slots; a node needing more than F fields counts errors[0] and takes slot 0.
+__device__ __forceinline__ int claim_slot(const bm_params& P, const bm_state& S, int64_t node, int64_t code) {
+  const int64_t N = P.node_count;
+  const int F = int(P.fields);
+  for (int f = 0; f < F; ++f) {
+    int64_t* slot = S.field_code + int64_t(f) * N + node;
+    int64_t cur = *slot;
+    if (cur == -1) {
+      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)(-1LL),
+                                          (unsigned long long)code);
+      cur = int64_t(prev) == -1 ? code : int64_t(prev);
+      if (int64_t(prev) == -1) atomicMax(S.node_nf + node, f + 1);
     }
-    // Registration, first half: one anchor particle per cell with even coordinates in its
-    // brick (its first particle) claims its code at its 8 nodes.  With even brick sides every
-    // node is a corner of exactly one such cell, so these claims meet no contention; the
-    // other particles register in k_pairs and mostly find their code in place, reading it
-    // instead of competing for the same slots.  Anchors register again there, finding their
-    // own slots.
-    const int ax = int((P.brick[0] + 1) / 2), ay = int((P.brick[1] + 1) / 2), az = int((P.brick[2] + 1) / 2);
-    const int per = ax * ay * az;
-    const int64_t nbricks = P.active_bricks > 0 ? P.active_bricks : P.bricks[0] * P.bricks[1] * P.bricks[2];
-    const int64_t cpb = P.brick[0] * P.brick[1] * P.brick[2];
-    int64_t stride = int64_t(insert_blocks) * blockDim.x;
-    for (int64_t a = int64_t(blockIdx.x - 1) * blockDim.x + threadIdx.x; a < nbricks * per; a += stride) {
-      int b = active_brick(S, int(a / per));
-      int k = int(a % per);
-      int cz = 2 * (k % az), cy = 2 * ((k / az) % ay), cx = 2 * (k / (az * ay));
-      int64_t cell = int64_t(b) * cpb + (cx * P.brick[1] + cy) * P.brick[2] + cz;
-      const int32_t* slots = S.perm + 8 * cell;
-      for (int s = 0; s < 8; ++s) {
-        int32_t q = slots[s];
-        if (q >= 0) {
-          insert_particle(P, S, q);
-          break;
-        }
-      }
-    }
-    return;
+    if (cur == code) return f;
   }
-  if (blockIdx.x > 0) {
-    // Relabelled particles with a current pair list: add their new candidate pairs.
-    if ((mode & kPairPrep) == 0 || S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
-    int count = min(S.changed_count[0], int(P.changed_capacity));
-    int first = int(blockIdx.x) - 1 - insert_blocks, blocks = int(gridDim.x) - 1 - insert_blocks;
-    constexpr int kWaves = kPrepBlock / 64;
-    int lane = int(threadIdx.x) & 63;
-    for (int i = first * kWaves + int(threadIdx.x >> 6); i < count; i += blocks * kWaves) {
-      int32_t d = S.changed[i];
-      ListRegion L = list_region(P, S, instance_of(S, d));
-      // A particle relabelled by two knives in one step is listed twice: one walk is enough.
-      int seen = 0;
-      if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
-      seen = __shfl(seen, 0, 64);
-      if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
-    }
-    return;
+  atomicAdd(S.errors + 0, 1);
+  return 0;
+}
+
+// While the particle's F update waits for the next P2G (update_pending 2), that update needs
+// the slots of the grid velocities it was computed on: the first registration of the step
+// keeps them in field_idx_prev and marks the particle.  Runs before any of the particle's
+// slots of this registration are written.
+__device__ __forceinline__ void keep_previous_slots(const bm_state& S, int64_t p) {
+  const uint2 before = reinterpret_cast<const uint2*>(S.field_idx)[p];
+  if (S.field_idx_prev != nullptr && S.update_pending != nullptr && S.update_pending[0] == 2 &&
+      atomicExch(S.update_done + p, 1) == 0)
+    reinterpret_cast<uint2*>(S.field_idx_prev)[p] = before;
+}
+
+// The field slot of the particle's transfer code at each of its 8 nodes, in field_idx.
+__device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p) {
+  keep_previous_slots(S, p);
+  int64_t code = S.transfer_codes[p];
+  int32_t base = S.base_node[p];
+  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+  uint32_t packed[2] = {0u, 0u};
+  for (int a = 0; a < 8; ++a) {
+    const int found = claim_slot(P, S, corner_node(base, a, dyz, dz), code);
+    packed[a >> 2] |= uint32_t(found & 0xff) << (8 * (a & 3));
   }
-  __shared__ float part[kPrepBlock / 64][kRigidSums];
+  reinterpret_cast<uint2*>(S.field_idx)[p] = make_uint2(packed[0], packed[1]);
+}
+
+
+// Sum (the speed, k = 10: maximum) of `count` rows of kRigidSums contact sums, in row order
+// within each thread's stride and in wave order across the block, so that the result does not
+// depend on timing.  Every thread of the block calls it; the sums end in part[0].
+__device__ void reduce_rigid_rows(const float* rows, int count, float (*part)[kRigidSums]) {
   float acc[kRigidSums];
 #pragma unroll
   for (int k = 0; k < kRigidSums; ++k) acc[k] = 0.0f;
 #pragma unroll 4
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    const float4* src = reinterpret_cast<const float4*>(S.rigid_partial + int64_t(b) * kRigidSums);
+  for (int b = threadIdx.x; b < count; b += blockDim.x) {
+    const float4* src = reinterpret_cast<const float4*>(rows + int64_t(b) * kRigidSums);
     float4 q0 = src[0], q1 = src[1], q2 = src[2];
     float v[kRigidSums] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
@@ -741,16 +737,17 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     part[0][k] = a;
   }
   __syncthreads();
+}
+
+// The step's contact sums t (board 3, reaction 3, torque 3, work, speed) applied: stored or
+// added (kOverwrite), then finish_external_step (kFinishExternal) or the interval sums and
+// tool.step (kAccumulateInterval, kToolStep).  Batched instances: each thread finishes some
+// instances from their contact-sum slots, which it clears for the next step.  Every thread of
+// the workgroup calls it.
+__device__ void finish_rigid_step(const bm_params& P, const bm_state& S, const float* t, int mode,
+                                  const ToolTarget& tt) {
   if (S.instance != nullptr) {
-    // Batched instances: each thread finishes the step of some instances (their sums were
-    // accumulated by k_rigid) and counts the builds of their pair lists; thread 0 also keeps
-    // the shared pair-list scalars.
-    int built = 0;
     for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
-      if ((mode & kPairPrep) != 0 && (P.pair_mode == kPairHash || S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0)) {
-        S.inst_pair_builds[i] += 1;
-        built = 1;
-      }
       // This step's sums of instance i: its slots, which are cleared for the next step.
       float sum[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float* slots = S.inst_sums + int64_t(i) * kInstSlots * 10;
@@ -773,26 +770,10 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
         st3(S.torque, i, tau);
       }
     }
-    built = __syncthreads_or(built);
-    if (threadIdx.x != 0) return;
-    const float dt = float(P.dt);
-    float speed = part[0][10];
-    S.speed[0] = speed;
-    if (mode & kPairPrep) {
-      S.pair_count[parity_new] = 0;
-      if (built) {
-        float r = float(P.radius), skin = float(P.pair_skin);
-        float value = 2.0f * r + skin;
-        if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
-        S.list_reach[0] = value;
-        S.pair_builds[0] += 1;
-      }
-    }
+    if (threadIdx.x == 0) S.speed[0] = t[10];
     return;
   }
   if (threadIdx.x != 0) return;
-  const float* t = part[0];
-  const float dt = float(P.dt);
   // A load's wait also waits for this thread's earlier stores, so everything the serial tail
   // reads is loaded before anything is written.
   f3 board = f3{t[0], t[1], t[2]}, react = f3{t[3], t[4], t[5]}, torq = f3{t[6], t[7], t[8]};
@@ -804,24 +785,11 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
     speed = fmaxf(speed, S.speed[0]);
   }
   float work = S.work[0] + t[9];
-  int dirty = (mode & kPairPrep) ? (P.pair_mode == kPairHash || S.pair_dirty[0] != 0) : 0;
-  int builds = (mode & kPairPrep) ? S.pair_builds[0] : 0;
   bool finish = (mode & kFinishExternal) != 0;
   ExternalTool ext;
   if (finish) ext = load_external_tool(P, S);
   // stores
   S.work[0] = work;
-  if (mode & kPairPrep) {
-    // set_pair_reach (contact.py:749-761) for a rebuild in this step; reset the new table count.
-    S.pair_count[parity_new] = 0;
-    if (dirty) {
-      float r = float(P.radius), skin = float(P.pair_skin);
-      float value = 2.0f * r + skin;
-      if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
-      S.list_reach[0] = value;
-      S.pair_builds[0] = builds + 1;
-    }
-  }
   if (finish) {
     // finish_external_step: the step sums are consumed and reset here.
     finish_external_regs(P, S, ext, react, torq);
@@ -833,6 +801,79 @@ __global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state
   S.speed[0] = speed;
   if (mode & kAccumulateInterval) accumulate_reaction_dev(S);
   if (mode & kToolStep) tool_step_dev(P, S, tt.targets, tt.rotations, tt.index);
+}
+
+// Registration of the field slots when the labels changed (topology_dirty), by workgroup
+// `block` of `blocks`, after the contact kernel emptied the affected nodes.
+__device__ void register_changed(const bm_params& P, const bm_state& S, int block, int blocks) {
+  const int topo = S.topology_dirty[0];
+  if (topo == 0) return;
+  const int64_t stride = int64_t(blocks) * blockDim.x;
+  const int64_t first = int64_t(block) * blockDim.x + threadIdx.x;
+  if ((topo & kTopoFull) == 0) {
+    // After relabelling only: every particle of the 27 cells around a relabelled particle's
+    // cell touches a node emptied in k_rigid and registers again; the other particles' nodes
+    // and slots are unchanged.  A particle met twice registers twice, finding its slots.
+    // One wavefront per (relabelled particle, neighbouring cell), a lane per (particle slot,
+    // corner): the 8 lanes of a particle claim its 8 nodes side by side.
+    const int count = min(S.changed_count[0], int(P.changed_capacity));
+    const int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
+    const int BX = int(P.brick[0]), BY = int(P.brick[1]), BZ = int(P.brick[2]);
+    const int cpb = BX * BY * BZ;
+    const int corner = int(threadIdx.x) & 7, slot = (int(threadIdx.x) >> 3) & 7;
+    const int waves = blocks * int(blockDim.x >> 6);
+    for (int w = block * int(blockDim.x >> 6) + int(threadIdx.x >> 6); w < count * 27; w += waves) {
+      const int base = S.base_node[S.changed[w / 27]];
+      const int k = w % 27;
+      const int cx = base / dyz + k / 9 - 1, cy = (base % dyz) / dz + (k / 3) % 3 - 1, cz = base % dz + k % 3 - 1;
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= int(P.bricks[0]) * BX || cy >= int(P.bricks[1]) * BY ||
+          cz >= int(P.bricks[2]) * BZ)
+        continue;
+      const int64_t brick = (int64_t(cx / BX) * P.bricks[1] + cy / BY) * P.bricks[2] + cz / BZ;
+      const int64_t cell = brick * cpb + ((cx % BX) * BY + cy % BY) * BZ + cz % BZ;
+      const int32_t q = S.perm[8 * cell + slot];
+      if (q < 0) continue;
+      // The particle's first lane keeps its earlier slots before any lane writes a new one.
+      if (corner == 0) keep_previous_slots(S, q);
+      const int f = claim_slot(P, S, corner_node(S.base_node[q], corner, dyz, dz), S.transfer_codes[q]);
+      S.field_idx[8 * int64_t(q) + corner] = uint8_t(f);
+    }
+    return;
+  }
+  // Every particle registers in the pair kernel's particle threads.  These workgroups, which
+  // the launch dispatches first, let one anchor particle per cell with even coordinates in its
+  // brick (its first particle) claim its code at its 8 nodes.  With even brick sides every
+  // node is a corner of exactly one such cell, so the claims meet no contention, and most
+  // particles find their code in place, reading it instead of competing for the same slots.
+  // A particle that registers twice finds its own slots.
+  const int ax = int((P.brick[0] + 1) / 2), ay = int((P.brick[1] + 1) / 2), az = int((P.brick[2] + 1) / 2);
+  const int per = ax * ay * az;
+  const int64_t nbricks = P.active_bricks > 0 ? P.active_bricks : P.bricks[0] * P.bricks[1] * P.bricks[2];
+  const int64_t cpb = P.brick[0] * P.brick[1] * P.brick[2];
+  for (int64_t a = first; a < nbricks * per; a += stride) {
+    int b = active_brick(S, int(a / per));
+    int k = int(a % per);
+    int cz = 2 * (k % az), cy = 2 * ((k / az) % ay), cx = 2 * (k / (az * ay));
+    int64_t cell = int64_t(b) * cpb + (cx * P.brick[1] + cy) * P.brick[2] + cz;
+    const int32_t* slots = S.perm + 8 * cell;
+    for (int s = 0; s < 8; ++s) {
+      int32_t q = slots[s];
+      if (q >= 0) {
+        insert_particle(P, S, q);
+        break;
+      }
+    }
+  }
+}
+
+// The contact kernel's block sums reduced in block order and applied (finish_rigid_step), for
+// a standalone rigid-contact call.  One workgroup.
+constexpr int kPrepBlock = 1024;  // 16 waves reduce the rows
+
+__global__ void __launch_bounds__(kPrepBlock) k_prep(const bm_params P, bm_state S, int nblocks, int mode, ToolTarget tt) {
+  __shared__ float part[kPrepBlock / 64][kRigidSums];
+  reduce_rigid_rows(S.rigid_partial, nblocks, part);
+  finish_rigid_step(P, S, part[0], mode, tt);
 }
 
 // The step sums of knives 1 .. tools-1 from extra_partial (nblocks blocks each, in block order):
@@ -936,6 +977,14 @@ __device__ __forceinline__ int bucket_of_point(f3 x, float inv, int mask) { retu
 // whose reach is larger (fast particles) searches ceil(reach / cell) rings of cells.
 __device__ __forceinline__ float list_cell_size(const bm_params& P) { return 2.0f * float(P.radius) + float(P.pair_skin); }
 __device__ __forceinline__ float list_cell_inv(const bm_params& P) { return 1.0f / list_cell_size(P); }
+// set_pair_reach (contact.py:749-761): the search reach of a list built at the largest
+// particle speed `speed`.
+__device__ __forceinline__ float list_reach_of(const bm_params& P, float speed) {
+  const float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt);
+  float value = 2.0f * r + skin;
+  if (dt * speed > 0.5f * skin) value += 2.0f * dt * speed;
+  return value;
+}
 // At most 4 rings: only a diverged run (non-finite or absurd speeds) asks for more, and its
 // contacts are void anyway (the run fails its state checks).
 __device__ __forceinline__ int list_rings(const bm_params& P, float reach) {
@@ -976,7 +1025,7 @@ __global__ void k_clear_pair_slots(bm_state S, int64_t P, int parity_new) {
 // written by an earlier build carries an older tag and reads as an empty bucket.
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
                                              const ListRegion& L) {
-  int tag = L.tag + 1;
+  int tag = L.tag;
   int b = L.base + bucket_of_point(xp, list_cell_inv(P), L.mask);
   unsigned long long entry = (uint64_t(uint32_t(tag)) << 32) | uint64_t(uint32_t(p));
   unsigned long long old = atomicExch(reinterpret_cast<unsigned long long*>(S.list_heads + b), entry);
@@ -995,20 +1044,41 @@ __global__ void k_list_bin(const bm_params P, bm_state S) {
   if (L.build) bin_particle(P, S, p, ld3(S.x, p), L);
 }
 __global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
-  float r = float(P.radius), skin = float(P.pair_skin), dt = float(P.dt), sp = S.speed[0];
-  float value = 2.0f * r + skin;
-  if (dt * sp > 0.5f * skin) value += 2.0f * dt * sp;
-  S.list_reach[0] = value;
-  S.pair_count[parity_new] = 0;
-  const bool every_step = P.pair_mode == kPairHash;
-  bool built = every_step || S.pair_dirty[0] != 0;
-  for (int i = 0; S.instance != nullptr && i < int(P.instances); ++i) {
-    if (every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[i] != 0) {
-      S.inst_pair_builds[i] += 1;
-      built = true;
+  // A standalone pair call takes the reach of the speed of the last rigid-contact call.
+  const float speed = fmaxf(S.speed[0], 0.0f);
+  S.list_reach[0] = list_reach_of(P, speed);
+  S.counters[0] = __float_as_uint(speed);
+  S.counters[1 + parity_new] = 0u;
+}
+
+// The step's pair-list bookkeeping once its pair kernel is done (every reader of these flags
+// and counts has finished): the build counts advance for the lists built this step, whose
+// reach is kept; the insertion counts become the tables' counts; the rebuild requests, the
+// relabelled list and the step's speed are reset.  Every thread of one workgroup calls it.
+__device__ void finish_pair_step(const bm_params& P, const bm_state& S) {
+  const bool all = P.pair_mode == kPairHash || S.pair_dirty[0] != 0;
+  int built = all ? 1 : 0;
+  if (S.instance != nullptr) {
+    for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) {
+      if (all || S.inst_pair_dirty[i] != 0) {
+        S.inst_pair_builds[i] += 1;
+        built = 1;
+      }
+      S.inst_pair_dirty[i] = 0;
     }
+    built = __syncthreads_or(built);
   }
-  if (built) S.pair_builds[0] += 1;
+  if (threadIdx.x != 0) return;
+  if (built) {
+    S.pair_builds[0] += 1;
+    S.list_reach[0] = list_reach_of(P, __uint_as_float(S.counters[0]));
+  }
+  S.pair_count[0] = int32_t(S.counters[1]);
+  S.pair_count[1] = int32_t(S.counters[2]);
+  S.counters[0] = 0u;
+  S.pair_dirty[0] = 0;
+  S.changed_count[0] = 0;
+  S.pair_epoch[0] += 1;
 }
 
 struct PairCtx {
@@ -1063,24 +1133,86 @@ __device__ void pair_at(const bm_params& P, const bm_state& S, const PairCtx& T,
 // binned by the contact kernel: partners within reach that pair_at does not skip as
 // compatible reference neighbours, at most `lanes`, else -1 and a bucket search at every
 // step (pairs_listed, contact.py:825-925).
-__global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology) {
+// In a substep the pair kernel's last workgroups do other work of the step, which needs the
+// contact kernel's results and must end before the transfers: after the `particles`
+// workgroups of the pair contacts, `reduce` workgroups sum the contact kernel's `rows` block
+// sums into rows `rows` .. `rows + reduce - 1` of rigid_partial (for the step's finish in the
+// node kernel), then `insert` workgroups register the field slots after a label change.
+struct PairExtras {
+  int particles, reduce, insert, rows;
+};
+constexpr int kReduceBlocks = 16;
+
+__global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology,
+                                                  PairExtras X) {
+  if (int(blockIdx.x) >= X.particles) {
+    const int block = int(blockIdx.x) - X.particles;
+    if (block < X.reduce) {
+      __shared__ float part[kBlock / 64][kRigidSums];
+      const int chunk = (X.rows + X.reduce - 1) / X.reduce;
+      const int first = block * chunk;
+      reduce_rigid_rows(S.rigid_partial + int64_t(first) * kRigidSums, max(0, min(chunk, X.rows - first)), part);
+      if (threadIdx.x < kRigidSums)
+        S.rigid_partial[int64_t(X.rows + block) * kRigidSums + threadIdx.x] = part[0][threadIdx.x];
+    } else {
+      register_changed(P, S, block - X.reduce, X.insert);
+    }
+    return;
+  }
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  const int32_t listed = S.partner_count[p];
-  // Registration, second half (the anchors claimed their slots in k_prep).
+  const bool live = p < P.n;
+  const int32_t listed = live ? S.partner_count[p] : 0;
+  if (P.pair_mode == kPairList) {
+    // The relabelled particles' build codes (their new pairs are in the lists already).
+    const int count = min(S.changed_count[0], int(P.changed_capacity));
+    for (int64_t i = p; i < count; i += int64_t(X.particles) * blockDim.x) {
+      const int32_t d = S.changed[i];
+      S.built_codes[d] = S.codes[d];
+    }
+  }
+  // This step's largest particle speed, for lists rebuilt this step: the maximum of the
+  // contact kernel's block maxima (rows of rigid_partial), taken by each workgroup that
+  // rebuilds, which keeps it in counters[0] for the step's bookkeeping.
+  // A standalone pair call (no rows) finds it in counters[0].
+  float build_speed = 0.0f;
+  if (P.pair_mode != kPairAll) {
+    // Uniform over a single solver's particles; batched instances rebuild their own lists.
+    const int build = S.instance == nullptr ? int(P.pair_mode == kPairHash || S.pair_dirty[0] != 0)
+                                            : __syncthreads_or(live && list_region(P, S, instance_of(S, p)).build);
+    if (build && X.rows > 0) {
+      __shared__ float wave_top[kBlock / 64];
+      float top = 0.0f;
+      for (int b = int(threadIdx.x); b < X.rows; b += int(blockDim.x))
+        top = fmaxf(top, S.rigid_partial[int64_t(b) * kRigidSums + 10]);
+      top = wave_max(top);
+      if (lane0()) wave_top[threadIdx.x >> 6] = top;
+      __syncthreads();
+      build_speed = wave_top[0];
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) build_speed = fmaxf(build_speed, wave_top[w]);
+      if (threadIdx.x == 0) S.counters[0] = __float_as_uint(build_speed);  // the same value in every such block
+    } else if (build) {
+      build_speed = __uint_as_float(S.counters[0]);
+    }
+  }
+  if (!live) return;
+  // Registration of every particle (after the anchors of the first workgroups).
   if (topology && (S.topology_dirty[0] & kTopoFull) != 0) insert_particle(P, S, p);
+  // The new table's insertion count (copied to pair_count once the kernel is done).
+  int32_t* const filled = reinterpret_cast<int32_t*>(S.counters + 2 - parity_old);
   if (P.pair_mode == kPairAll) {
     // pairs_all (contact.py:929-993): every pair q > p, no lists.
     int parity_new = 1 - parity_old;
     int64_t cap = P.pair_capacity;
     PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
-              S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, S.pair_count + parity_new};
+              S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, filled};
     for (int64_t q = p + 1; q < P.n; ++q) pair_at(P, S, T, p, q);
     return;
   }
   ListRegion L = list_region(P, S, instance_of(S, p));
   int tag = L.tag;
-  float reach = S.list_reach[0];
+  // A list rebuilt this step takes the reach of this step's largest speed (set_pair_reach,
+  // contact.py:749-761); the others keep the reach of their build.
+  float reach = L.build ? list_reach_of(P, build_speed) : S.list_reach[0];
   const float reach2 = reach * reach;
   float cell = list_cell_size(P), inv = list_cell_inv(P);
   int rings = list_rings(P, reach);
@@ -1089,15 +1221,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   int count;
   bool dirty = L.build;
   // A rebuild uses the current positions.  Relabelled particles do not rebuild: the contact
-  // kernel added their new pairs to the lists (add_relabelled_pairs), and their build codes
-  // are brought up to date here.
-  if (P.pair_mode == kPairList) {
-    const int count = min(S.changed_count[0], int(P.changed_capacity));
-    for (int64_t i = p; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-      const int32_t d = S.changed[i];
-      S.built_codes[d] = S.codes[d];
-    }
-  }
+  // kernel added their new pairs to the lists (add_relabelled_pairs); their build codes are
+  // brought up to date at the start of this kernel.
   const bool rebuild = dirty;
   const float* bx = dirty ? S.x : S.built_x;
   if (rebuild) {
@@ -1163,7 +1288,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   int parity_new = 1 - parity_old;
   int64_t cap = P.pair_capacity;
   PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
-            S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, S.pair_count + parity_new};
+            S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, filled};
   if (count > 0) {
     for (int k = 0; k < count; ++k) pair_at(P, S, T, p, S.partners[p * lanes + k]);
     return;
@@ -1188,12 +1313,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
         }
       }
 }
-__global__ void k_pairs_done(const bm_params P, bm_state S) {
-  S.pair_dirty[0] = 0;
-  for (int i = 0; S.instance != nullptr && i < int(P.instances); ++i) S.inst_pair_dirty[i] = 0;
-  S.changed_count[0] = 0;
-  S.pair_epoch[0] += 1;
-}
+__global__ void k_pairs_done(const bm_params P, bm_state S) { finish_pair_step(P, S); }
 
 // After particle d's code changed, adds the pairs (d, q) that pair_at no longer skips to the
 // partner lists, so the lists need no rebuild: q within the list reach of d at the build
@@ -1269,49 +1389,6 @@ __global__ void k_check_pair_list(const bm_params P, bm_state S) {
 }
 
 // ------------------------------------------------------------------------ topology
-
-// register (material.py:50-65): the field slot of the particle's transfer code at each of
-// its 8 nodes, claiming the first empty slot when the code is new there.  Used slots stay a
-// prefix of the node's F slots; a node needing more than F fields counts errors[0].
-__device__ void insert_particle(const bm_params& P, const bm_state& S, int64_t p) {
-  // While the particle's F update waits for the next P2G (update_pending 2), that update needs
-  // the slots of the grid velocities it was computed on: the first registration of the step
-  // keeps them in field_idx_prev and marks the particle.
-  const uint2 before = reinterpret_cast<const uint2*>(S.field_idx)[p];
-  if (S.field_idx_prev != nullptr && S.update_pending != nullptr && S.update_pending[0] == 2 &&
-      atomicExch(S.update_done + p, 1) == 0)
-    reinterpret_cast<uint2*>(S.field_idx_prev)[p] = before;
-  int64_t code = S.transfer_codes[p];
-  int32_t base = S.base_node[p];
-  int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
-  int64_t N = P.node_count;
-  int F = int(P.fields);
-  uint32_t packed[2] = {0u, 0u};
-  for (int a = 0; a < 8; ++a) {
-    int64_t node = corner_node(base, a, dyz, dz);
-    int found = -1;
-    for (int f = 0; f < F; ++f) {
-      int64_t* slot = S.field_code + int64_t(f) * N + node;
-      int64_t cur = *slot;
-      if (cur == -1) {
-        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)(-1LL),
-                                            (unsigned long long)code);
-        cur = int64_t(prev) == -1 ? code : int64_t(prev);
-        if (int64_t(prev) == -1) atomicMax(S.node_nf + node, f + 1);
-      }
-      if (cur == code) {
-        found = f;
-        break;
-      }
-    }
-    if (found < 0) {
-      atomicAdd(S.errors + 0, 1);
-      found = 0;
-    }
-    packed[a >> 2] |= uint32_t(found & 0xff) << (8 * (a & 3));
-  }
-  reinterpret_cast<uint2*>(S.field_idx)[p] = make_uint2(packed[0], packed[1]);
-}
 
 __global__ void k_topo_clear(const bm_params P, bm_state S) {
   int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -1580,17 +1657,9 @@ __global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
-  if (flags && blockIdx.x == 0 && threadIdx.x == 0) {
-    // Registration (bit 0) and the pair-list rebuild (bit 1) of this step are complete.
-    if (flags & 1) S.topology_dirty[0] = 0;
-    if (flags & 2) {
-      S.pair_dirty[0] = 0;
-      S.changed_count[0] = 0;
-      S.pair_epoch[0] += 1;
-    }
-  }
-  if ((flags & 2) && S.instance != nullptr && blockIdx.x == 0)
-    for (int i = int(threadIdx.x); i < int(P.instances); i += int(blockDim.x)) S.inst_pair_dirty[i] = 0;
+  // Registration (bit 0) and the pair kernel (bit 1) of this step are complete.
+  if ((flags & 1) && blockIdx.x == 0 && threadIdx.x == 0) S.topology_dirty[0] = 0;
+  if ((flags & 2) && blockIdx.x == 0) finish_pair_step(P, S);
   Window w = window_of(P, D, b);
   int F = int(P.fields);
   int FW = F * w.W;
@@ -1914,7 +1983,24 @@ __device__ void multi_remapped(const bm_params& P, const bm_state& S, const Divi
 // sums: x_a += sum_{b compatible, b != a} (m_a x_b - m_b x_a) / sum_all m_b; then
 // old_v = p / m, new_v = (p + j) / m.
 
-__global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S, const Divisors D) {
+// The step's finish (finish_rigid_step) in a substep, by the node kernel's last workgroup:
+// `mode` of the contact sums (-1: none), whose block sums the pair kernel reduced to rows
+// `rows` .. `rows + kReduceBlocks - 1` of rigid_partial; tt: the knife's targets.  Nothing
+// between the contact kernel and here reads what it writes, and the knife's new pose is
+// first read by G2P's labelling.
+struct StepFinish {
+  int mode, rows;
+  ToolTarget tt;
+};
+
+__global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, bm_state S, const Divisors D,
+                                                            StepFinish SF) {
+  if (SF.mode >= 0 && blockIdx.x == gridDim.x - 1) {
+    __shared__ float part[kBlock / 64][kRigidSums];
+    reduce_rigid_rows(S.rigid_partial + int64_t(SF.rows) * kRigidSums, kReduceBlocks, part);
+    finish_rigid_step(P, S, part[0], SF.mode, SF.tt);
+    return;
+  }
   if (int(blockIdx.x) < kMultiBlocks) {
     multi_velocity(P, S, D);
     return;
@@ -2245,12 +2331,18 @@ __device__ void defer_update(const bm_state& S, int block);
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
 // update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
-                                                            int update_blocks) {
+                                                            int update_blocks, int mark_blocks) {
   if (int(blockIdx.x) < kMultiBlocks) {
     multi_remapped(P, S, D);
     return;
   }
-  const int node_end = int(gridDim.x) - update_blocks;
+  const int node_end = int(gridDim.x) - update_blocks - mark_blocks;
+  if (int(blockIdx.x) >= node_end + update_blocks) {
+    // The last workgroups: the relabelled particles' new pairs into the partner lists.
+    mark_relabelled(P, S, (int(blockIdx.x) - node_end - update_blocks) * (kBlock / 64) + int(threadIdx.x >> 6),
+                    mark_blocks * (kBlock / 64));
+    return;
+  }
   if (int(blockIdx.x) >= node_end) {
     defer_update(S, int(blockIdx.x) - node_end);
     return;
@@ -2925,6 +3017,9 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 }
 
 // assign_sides (solver.py:866-890) for knives first .. tools-1: knife k labels digit * 3^k.
+__global__ void __launch_bounds__(kBlock) k_mark_relabelled(const bm_params P, bm_state S) {
+  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64));
+}
 __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting,
                                                    int first) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -3214,7 +3309,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   if (p->tools > 1)
     hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
-                     ToolTarget{nullptr, nullptr, 0}, 0, 0);
+                     ToolTarget{nullptr, nullptr, 0});
   if (p->tools > 1)
     hipLaunchKernelGGL(k_extra_tools, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb,
                        int(kAccumulate), ToolTarget{nullptr, nullptr, 0});
@@ -3292,9 +3387,9 @@ static int brick_threads(const bm_params* p) {
 // workgroups among `blocks`) and the F update, in the instantiation of the material model.
 // `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
-                                   int update_blocks, hipStream_t st) {
-  blocks += kMultiBlocks;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
+                                   int update_blocks, hipStream_t st, int mark_blocks = 0) {
+  blocks += kMultiBlocks + mark_blocks;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
                             int defer, int nbricks, hipStream_t st) {
@@ -3313,15 +3408,18 @@ static void launch_register(const bm_params* p, const bm_state* s, hipStream_t s
 // pair-list staleness check (2) on the moved particles; defer: leave the F update to the
 // next P2G where the field slots allow it.
 static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flags, int64_t digit, int g2p_flags,
-                      hipStream_t st, int defer = 0) {
+                      hipStream_t st, int defer = 0, StepFinish sf = StepFinish{-1, 0, ToolTarget{nullptr, nullptr, 0}}) {
   int nb = int(brick_launch(p)), tb = brick_threads(p);
   const Divisors D = divisors_of(p);
   if (p->maxwell)
     hipLaunchKernelGGL(k_p2g<true>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
   else
     hipLaunchKernelGGL(k_p2g<false>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
-  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p))), dim3(kBlock), 0, st, *p, *s,
-                     D);
+  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p)) + (sf.mode >= 0 ? 1 : 0)),
+                     dim3(kBlock), 0, st, *p, *s, D, sf);
+  // Knives 1 .. tools-1: their sums and tool steps after knife 0's.
+  if (sf.mode >= 0 && p->tools > 1)
+    hipLaunchKernelGGL(k_extra_tools, dim3(1), dim3(kPrepBlock), 0, st, *p, *s, sf.rows, sf.mode, sf.tt);
   // Viscosity: the impulse of the coupled old velocities, coupled and added to new_v
   // (solver.py:433-458).
   if (p->viscosity > 0.0) {
@@ -3336,14 +3434,18 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   if (p->tools > 1 && (g2p_flags & 1))
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, digit, 1, 1);
   const int node_blocks = int(blocks_for(node_launch(p)));
+  // In a substep the relabelled particles' new pairs go to the partner lists in the node
+  // kernel's last workgroups: one wave per relabelled particle, 64 workgroups of four waves.
+  const int mark_blocks = sf.mode < 0 || !(g2p_flags & 1) ? 0
+                          : int(p->changed_capacity < 256 ? (p->changed_capacity + 3) / 4 : 64);
   defer = defer && s->update_pending && s->update_done && s->field_idx_prev;
   if (defer) {
     // The F update is deferred to the next P2G: one extra workgroup of the node kernel sets
     // that up, and no F-update launch follows.
-    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st);
+    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks);
     return;
   }
-  launch_couple_remapped(p, s, D, node_blocks, 0, st);
+  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks);
   const int ub = update_blocks(tb);
   launch_update_F(p, s, D, nb < ub ? nb : ub, tb, 0, nb, st);
 }
@@ -3359,7 +3461,8 @@ int bm_pair_contacts(const bm_params* p, const bm_state* s, int32_t parity, void
   // Binning tags the bucket heads with the build count after this build's increment.
   hipLaunchKernelGGL(k_list_bin, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s);
   hipLaunchKernelGGL(k_list_reach, dim3(1), dim3(1), 0, st, *p, *s, new_p);
-  hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p, 0);
+  hipLaunchKernelGGL(k_pairs, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, old_p, 0,
+                     PairExtras{int(blocks_for(p->n)), 0, 0, 0});
   hipLaunchKernelGGL(k_pairs_done, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
 }
@@ -3388,8 +3491,7 @@ int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
   hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p))), dim3(kBlock), 0,
-                     (hipStream_t)stream, *p,
-                     *s, divisors_of(p));
+                     (hipStream_t)stream, *p, *s, divisors_of(p), StepFinish{-1, 0, ToolTarget{nullptr, nullptr, 0}});
   return launch_status();
 }
 
@@ -3447,6 +3549,15 @@ int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutt
   if (p->n == 0 || !cutting) return 0;
   hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, digit,
                      int(cutting), 0);
+  // The relabelled particles' new pairs into the partner lists, as in a substep.
+  if (pair_args_ok(p, s)) hipLaunchKernelGGL(k_mark_relabelled, dim3(64), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
+  return launch_status();
+}
+
+int bm_mark_relabelled(const bm_params* p, const bm_state* s, void* stream) {
+  if (!pair_args_ok(p, s)) return -1;
+  if (p->n == 0) return 0;
+  hipLaunchKernelGGL(k_mark_relabelled, dim3(64), dim3(kBlock), 0, (hipStream_t)stream, *p, *s);
   return launch_status();
 }
 
@@ -3454,16 +3565,11 @@ namespace {
 static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int32_t cutting, int64_t digit, int32_t parity,
             int32_t accumulate, const float* targets, const float* rotations, int32_t index, void* stream, int defer) {
   if (!brick_args_ok(p, s) || !pair_args_ok(p, s) || !s->rigid_partial || !s->counters || digit < 1) return -1;
-  if (p->rigid_blocks < blocks_for(p->n)) return -2;
   bool external = p->tool_model < 0;
   if (!external && (!targets || !rotations || index < 0)) return -1;
   if (p->n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   int old_p = parity & 1, new_p = 1 - old_p;
-  // 1. contacts with the knife and the board, plus the step's preparation: empty the new
-  //    pair table's slots, bin particles for a stale pair list, clear a stale field table.
-  // 2. block 0 reduces the contact sums and moves the knife (its update reads only this
-  //    step's rigid reaction, solver.py:824-864); the other blocks re-register the fields.
   int mode = kOverwrite | kPairPrep | kTopologyPrep;
   if (external) {
     // With a held blade the grid constraint adds to the reaction first (finish runs last).
@@ -3472,26 +3578,27 @@ static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int3
     mode |= kToolStep | (accumulate ? kAccumulateInterval : 0);
   }
   int nb = blocks_for(p->n);
-  // Registration and marking blocks exit at once on most substeps; keep their count modest:
-  // registration at most 512 blocks (two per CU), marking one wave per relabelled particle
-  // in at most 16 blocks.
-  int insert_blocks = int((p->n + kPrepBlock - 1) / kPrepBlock);
-  insert_blocks = insert_blocks < 512 ? insert_blocks : 512;
-  int mark_blocks = int((p->changed_capacity + kPrepBlock / 64 - 1) / (kPrepBlock / 64));
-  mark_blocks = mark_blocks < 16 ? mark_blocks : 16;
+  if (p->rigid_blocks < nb + kReduceBlocks) return -2;
+  // Registration workgroups of the pair kernel, which exit at once on most substeps: 2048
+  // wavefronts, one per (relabelled particle, neighbouring cell).
+  const int insert_blocks = int(p->n < 512 * kBlock ? blocks_for(p->n) : 512);
+  // 1. contacts with the knife and the board, plus the step's preparation: empty the new
+  //    pair table's slots, bin particles for a stale pair list, clear a stale field table.
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
-  // Knives 1 .. tools-1: their contacts after knife 0's, their sums and tool steps after k_prep.
+  // Knives 1 .. tools-1: their contacts after knife 0's, their sums and tool steps after
+  // knife 0's step.
   if (p->tools > 1)
     hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0);
-  hipLaunchKernelGGL(k_prep, dim3(1 + insert_blocks + mark_blocks), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
-                     ToolTarget{targets, rotations, int(index)}, new_p, insert_blocks);
-  if (p->tools > 1)
-    hipLaunchKernelGGL(k_extra_tools, dim3(1), dim3(kPrepBlock), 0, st, *p, *s, nb, mode,
-                       ToolTarget{targets, rotations, int(index)});
-  // 3. particle pairs across cuts (partner lists rebuilt first when stale)
-  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, *p, *s, old_p, 1);
-  // 4. transfers; G2P labels the moved particles and checks the pair list's skin
-  launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st, defer);
+  // 2. particle pairs across cuts (partner lists rebuilt first when stale); its first
+  //    workgroups reduce the contact sums and register the fields of a label change.
+  hipLaunchKernelGGL(k_pairs, dim3(nb + kReduceBlocks + insert_blocks), dim3(kBlock), 0, st, *p, *s, old_p, 1,
+                     PairExtras{nb, kReduceBlocks, insert_blocks, nb});
+  // 3. transfers; the node kernel's last workgroup applies the contact sums and moves the
+  //    knife (its update reads only this step's rigid reaction, solver.py:824-864), which
+  //    G2P's labelling reads; G2P labels the moved particles and checks the pair list's skin,
+  //    and the last node kernel adds the relabelled particles' new pairs to the lists.
+  launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st, defer,
+                   StepFinish{mode, nb, ToolTarget{targets, rotations, int(index)}});
   if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
   return launch_status();
 }

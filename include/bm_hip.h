@@ -53,7 +53,9 @@ typedef struct bm_params {
   int64_t plastic;        /* yield_stress > 0 */
   int64_t brick[3];       /* cells per brick along x, y, z (particle bins of the P2G/G2P kernels) */
   int64_t bricks[3];      /* bricks along x, y, z */
-  int64_t rigid_blocks;   /* blocks of the contact kernel (partial sums per block) */
+  int64_t rigid_blocks;   /* rows of rigid_partial: at least one per contact-kernel workgroup
+                             (ceil(n / 256)); a substep needs 16 more, where its pair kernel
+                             reduces the rows to 16 */
   int64_t blade_hold;     /* external tool held over the interval: grid constraint active */
   int64_t changed_capacity; /* entries of the changed-code list (more: full pair-list rebuild) */
   int64_t instances;      /* independent cutting instances packed in this state (1: a single solver) */
@@ -148,7 +150,10 @@ typedef struct bm_state {
   float *p2g_partial;      /* [bricks][F][W][7] window sums of mass, momentum, impulse */
   float *remap_partial;    /* [bricks][F][W][3] window sums of the re-scattered momentum */
   float *rigid_partial;    /* [rigid_blocks][12] per-block contact sums */
-  uint32_t *counters;      /* [4] reserved */
+  uint32_t *counters;      /* [4]: [0] float bits of the step's largest particle speed (contact
+                            * kernel, atomicMax; zero between steps), [1 + t] the insertion
+                            * count of pair table t while a pair kernel fills it (copied to
+                            * pair_count when the step's pair work is done), [3] unused */
   /* held-blade grid constraint (blade_constraint.py), used when blade_hold is set */
   float *bc_side;          /* [n] face side of the previous step (0, +1, -1) */
   float *bc_normal;        /* [n][3] */
@@ -245,8 +250,17 @@ int bm_finish_external(const bm_params* p, const bm_state* s, void* stream);
 /* Interval sums for a driven knife (tool.py:6-16). */
 int bm_accumulate_reaction(const bm_params* p, const bm_state* s, void* stream);
 
-/* Side labels behind the blade edge; digit = 3^cut_index (contact.py:996-1039). */
+/* Side labels behind the blade edge; digit = 3^cut_index (contact.py:996-1039).  The
+ * relabelled particles are recorded in `changed`, and their new candidate pairs are added to
+ * the partner lists (bm_mark_relabelled). */
 int bm_assign(const bm_params* p, const bm_state* s, int64_t digit, int32_t cutting, void* stream);
+
+/* The partner-list side of a label change: each particle recorded in `changed` adds the pairs
+ * that its new code makes candidates to the lists (no counterpart in the reference, which
+ * rebuilds every list on any code change, contact.py:716-735).  bm_assign and bm_substep run
+ * it themselves; a caller that writes codes and `changed` directly runs it before the next
+ * pair contacts. */
+int bm_mark_relabelled(const bm_params* p, const bm_state* s, void* stream);
 
 /* One full substep in the reference order (solver.py:920-940).  external selects
  * finish_external_step instead of tool.step. */

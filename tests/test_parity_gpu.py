@@ -252,6 +252,7 @@ def test_relabelled_pair_lists_match_a_full_rebuild():
         T["topology_dirty"].fill_(1)
     a._t["changed"][: len(moved)] = torch.from_numpy(moved).to(a._t["changed"].device)
     a._t["changed_count"].fill_(len(moved))
+    a._call("bm_mark_relabelled")  # what assign does after recording the relabels
     b._t["pair_dirty"].fill_(1)
     builds = int(a.pair_list_builds.numpy()[0])
     for s in (a, b, control):
@@ -304,6 +305,7 @@ def test_incremental_registration_matches_a_full_one():
         T["codes"][torch.from_numpy(moved).long().to(T["codes"].device)] = 1
         T["changed"][: len(moved)] = torch.from_numpy(moved).to(T["changed"].device)
         T["changed_count"].fill_(len(moved))
+        s._call("bm_mark_relabelled")  # what assign does after recording the relabels
     a._t["topology_dirty"].fill_(2)  # only around the relabelled particles
     b._t["topology_dirty"].fill_(1)  # every particle
     for s in (a, b):
