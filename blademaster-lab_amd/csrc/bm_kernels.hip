@@ -1240,6 +1240,13 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
     constexpr int kChains = 9;
     CellCull cull(xp, home, cell, reach);
     int side = 2 * rings + 1, nyz = side * side;
+    // With at least 2^17 buckets, two cells share a bucket only when they are at least 16
+    // cells apart along some axis (bucket_of is the linear index x 2^13 + y 2^6 + z, wrapped).
+    // The cells of the search cube (at most 9 per axis) then fall into distinct buckets, so
+    // each particle met is in one searched cell's chain only, and a particle of another cell
+    // whose bucket aliases a searched one lies beyond the reach: the cell of each particle
+    // met is not checked.
+    const bool distinct = mask >= (1 << 17) - 1;
     for (int dx = -rings; dx <= rings; ++dx) {
       for (int g0 = 0; g0 < nyz; g0 += kChains) {
         int32_t cur[kChains];
@@ -1265,8 +1272,10 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
             more = more || next >= 0;
             if (q <= p || count < 0) continue;
             f3 xq = ld3(bx, q);
-            Cell cq = cell_of(xq, inv);
-            if (cq.x != cc[j].x || cq.y != cc[j].y || cq.z != cc[j].z) continue;
+            if (!distinct) {
+              Cell cq = cell_of(xq, inv);
+              if (cq.x != cc[j].x || cq.y != cc[j].y || cq.z != cc[j].z) continue;
+            }
             f3 d = xq - xp;
             if (!(dot(d, d) < reach2)) continue;
             if (compatible(cp, S.codes[q]) && len(rp - ld3(S.ref, q)) < two_h) continue;
