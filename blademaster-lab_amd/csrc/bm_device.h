@@ -30,6 +30,16 @@ __device__ __forceinline__ float len(f3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 __device__ __forceinline__ f3 ld3(const float* p, int64_t i) { return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+// A load whose address is the same for the whole wavefront, through the scalar cache: no
+// vector memory wait holds up the wavefront's other loads.  Only for data that no wavefront
+// of the running kernel writes (the scalar cache is not kept coherent with vector stores).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)(p);
+}
+__device__ __forceinline__ f3 uniform_ld3(const float* p, int64_t i) {
+  return f3{uniform_load(p + 3 * i), uniform_load(p + 3 * i + 1), uniform_load(p + 3 * i + 2)};
+}
 __device__ __forceinline__ void st3(float* p, int64_t i, f3 v) {
   p[3 * i] = v.x;
   p[3 * i + 1] = v.y;

@@ -370,6 +370,41 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
                                                   int parity_new) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
+  // The particle's own loads go first, ahead of the step's rare preparation loops below
+  // (whose stores the compiler cannot move them past).
+  // (A lane past the last particle loads particle 0 and is ignored.)
+  const f3 xp = ld3(S.x, live ? p : 0), vp = ld3(S.v, live ? p : 0);
+  // Friction histories that may be nonzero (bit p & 63 of the wavefront's word per row, one
+  // scalar load per row): a zero history is written only over one of them.
+  const int64_t mask_words = (P.n + 63) >> 6;
+  uint64_t may0 = ~0ull, may1 = ~0ull;
+  if (S.history_mask != nullptr) {
+    const int64_t word = ((int64_t(blockIdx.x) * blockDim.x) >> 6) + (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6);
+    if (word < mask_words) {
+      may0 = uniform_load(S.history_mask + word);
+      may1 = uniform_load(S.history_mask + mask_words + word);
+    }
+  }
+  const float dt = float(P.dt);
+  Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
+  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
+  // The knife of this particle's instance (for a single solver, knife 0 by scalar loads).
+  int inst = 0;
+  quat q;
+  f3 origin, tvel, tom;
+  if (S.instance == nullptr) {
+    q = quat{uniform_load(S.tool_rot), uniform_load(S.tool_rot + 1), uniform_load(S.tool_rot + 2),
+             uniform_load(S.tool_rot + 3)};
+    origin = uniform_ld3(S.tool_pos, 0);
+    tvel = uniform_ld3(S.tool_vel, 0);
+    tom = uniform_ld3(S.tool_omega, 0);
+  } else {
+    inst = live ? S.instance[p] : 0;
+    q = ldq(S.tool_rot + 4 * inst);
+    origin = ld3(S.tool_pos, inst);
+    tvel = ld3(S.tool_vel, inst);
+    tom = ld3(S.tool_omega, inst);
+  }
   // A new substep: empty the node kernels' multi-field node list.
   if (p == 0 && S.multi_count != nullptr) {
     S.multi_count[0] = 0;
@@ -377,24 +412,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   }
   // The pair kernel inserts into the new table from an empty count.
   if (p == 0 && (mode & kPairPrep) != 0) S.counters[1 + parity_new] = 0u;
-  const float dt = float(P.dt);
-  Contact c{dt, float(P.sn), float(P.mus), float(P.mud), float(P.gk), float(P.gc)};
-  f3 hf = f3{float(P.half[0]), float(P.half[1]), float(P.half[2])};
-  // The knife of this particle's instance (instance 0 for a single solver).
-  int inst = live ? instance_of(S, p) : 0;
-  quat q = ldq(S.tool_rot + 4 * inst);
-  f3 origin = ld3(S.tool_pos, inst), tvel = ld3(S.tool_vel, inst), tom = ld3(S.tool_omega, inst);
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
   int64_t stride = int64_t(gridDim.x) * blockDim.x;
   if ((mode & kPairPrep) != 0) {
     // clear_pair_slots (contact.py:643-652): empty the slots of the table used two steps ago.
-    int count = S.pair_count[parity_new];
+    const int count = uniform_load(S.pair_count + parity_new);
     int64_t* keys = S.pair_keys + int64_t(parity_new) * P.pair_capacity;
     const int32_t* used = S.pair_used + int64_t(parity_new) * P.pair_capacity;
     for (int64_t a = p; a < count; a += stride) keys[used[a]] = -1;
   }
-  const int topo = (mode & kTopologyPrep) != 0 ? S.topology_dirty[0] : 0;
+  const int topo = (mode & kTopologyPrep) != 0 ? uniform_load(S.topology_dirty) : 0;
   if (topo & kTopoFull) {
     // register, first half (solver.py:392-393): empty field table before the inserts.
     int64_t nodes = active_node_count(P);
@@ -406,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   } else if (topo & kTopoChanged) {
     // Only the stencil nodes of the particles relabelled last step can change their codes:
     // empty those (the particles around them register again in the pair kernel).
-    int64_t count = min(S.changed_count[0], int(P.changed_capacity));
+    int64_t count = min(uniform_load(S.changed_count), int(P.changed_capacity));
     int dz = int(P.dims[2]), dyz = int(P.dims[1] * P.dims[2]);
     for (int64_t a = p; a < count * 8; a += stride) {
       int64_t node = corner_node(S.base_node[S.changed[a >> 3]], int(a & 7), dyz, dz);
@@ -415,18 +443,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     }
   }
   f3 total = f3{0.f, 0.f, 0.f};
-  // Friction histories that may be nonzero (bit p & 63 of the wavefront's word per row): a zero
-  // history is written only over one of them.
-  const int64_t mask_words = (P.n + 63) >> 6;
-  uint64_t may0 = ~0ull, may1 = ~0ull;
-  if (S.history_mask != nullptr && live) {
-    may0 = S.history_mask[p >> 6];
-    may1 = S.history_mask[mask_words + (p >> 6)];
-  }
   const bool nonzero0 = (may0 >> (p & 63)) & 1ull, nonzero1 = (may1 >> (p & 63)) & 1ull;
   bool board_contact = false, blade_contact = false;
   if (live) {
-    f3 xp = ld3(S.x, p), vp = ld3(S.v, p);
     spd = len(vp);
     if ((mode & kPairPrep) != 0) {
       ListRegion L = list_region(P, S, inst);
@@ -1662,7 +1681,9 @@ __device__ __forceinline__ void note_multi_node(const bm_state& S, int64_t node)
 }
 
 template <bool kMx>
-__global__ void __launch_bounds__(1024) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
+// Elastic: six wavefronts per SIMD (at most 80 registers; the few spills are cheaper than the
+// occupancy they buy), 33.4 -> 32.6 us on the 1M block; the Maxwell branch keeps its registers.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(kMx ? 1 : 6, 8))) k_p2g(const bm_params P, bm_state S, int flags, const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
   int b = active_brick(S, brick_of_block(int(blockIdx.x), int(gridDim.x)));
@@ -2190,7 +2211,9 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 // reads its 8 corners and the re-scatter sums per corner (cell_sum).  In a substep the updated
 // particle is also labelled (assign, contact.py:996-1039) and checked against the pair list's
 // skin (check_pair_list, contact.py:716-735) while x and v are in registers.
-__global__ void __launch_bounds__(1024) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags,
+// Eight wavefronts per SIMD: 56 registers, no spills (the 1024-thread bound alone let it take
+// 89): 23.6 -> 22.0 us on the 1M block.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) k_g2p(const bm_params P, bm_state S, int64_t digit, int flags,
                                               const Divisors D) {
   extern __shared__ float lds[];
   __shared__ int wave_nf[16];
