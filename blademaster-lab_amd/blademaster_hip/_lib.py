@@ -64,6 +64,7 @@ class Params(ctypes.Structure):
         ("viscosity", ctypes.c_double),
         ("mu_maxwell", ctypes.c_double),
         ("maxwell_relax", ctypes.c_double),
+        ("mover_capacity", ctypes.c_int64),
     ]
 
 
@@ -93,6 +94,7 @@ STATE_FIELDS = [
     ("viscous", ctypes.POINTER(ctypes.c_double)), ("viscous_impulse", c_f32p),
     ("extra_histories", c_f32p), ("extra_damage", c_f32p), ("extra_partial", c_f32p),
     ("multi_nodes", c_i32p), ("multi_stamp", c_i32p), ("multi_count", c_i32p), ("field_idx_prev", c_u8p),
+    ("movers", c_i32p), ("displaced", c_i32p), ("refresh_mark", c_i32p), ("refresh_state", c_i32p),
 ]
 
 

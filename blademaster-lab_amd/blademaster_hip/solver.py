@@ -68,6 +68,10 @@ class Solver:
     # Relabelled particles listed per substep for the incremental re-registration; more in one
     # substep re-register every particle instead.
     CHANGED_CAPACITY = 65536
+    # Movers per step, and displaced particles per full build, of the pair-list refresh (more:
+    # every list is rebuilt).  The searches scan the displaced list, so it stays short.
+    MOVER_CAPACITY = 256
+    PAIR_REFRESH = True
 
     def __init__(self, config=Config(), device="cuda:0", brick_z=None, _layout=None):
         self.config = cfg = config
@@ -223,6 +227,16 @@ class Solver:
         T["list_links"] = t((max(n, 1),), i32, -1)
         self.changed_capacity = int(self.CHANGED_CAPACITY)
         T["changed"] = t((self.changed_capacity,), i32)
+        # Pair-list refresh of the particles that moved too far (one solver, list mode, at least
+        # 2^17 buckets so that a search cube's cells have distinct buckets): the other lists keep
+        # their build.  Otherwise every list is rebuilt.
+        refresh = (self.PAIR_REFRESH and not _layout and cfg.pair_search in ("list", "auto")
+                   and self.list_buckets >= 1 << 17 and PAIR_LIST_LANES <= 16)
+        self.mover_capacity = int(self.MOVER_CAPACITY) if refresh else 0
+        T["movers"] = t((max(self.mover_capacity, 1),), i32)
+        T["displaced"] = t((max(self.mover_capacity, 1),), i32)
+        T["refresh_mark"] = t((max(n, 1),), i32)
+        T["refresh_state"] = t((4,), i32)
         T["changed_count"] = t((1,), i32)
         T["pair_mark"] = t((max(n, 1),), i32)
         T["pair_epoch"] = t((1,), i32)
@@ -373,6 +387,7 @@ class Solver:
         p.tools = self.tools
         p.mu_maxwell, p.maxwell_relax = self.mu_maxwell, self.maxwell_relax
         p.viscosity = cfg.dynamic_viscosity
+        p.mover_capacity = self.mover_capacity
         return p
 
     def _make_state(self):

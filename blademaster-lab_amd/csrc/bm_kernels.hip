@@ -284,8 +284,11 @@ __device__ void finish_external_regs(const bm_params& P, const bm_state& S, cons
 // instances keep separate lists, each rebuilt when its own particles have moved too far (as
 // the separate runs of a sweep are), in a bucket region of its own.
 struct ListRegion {
-  int base, mask, tag;
-  bool build;
+  int base, mask;
+  int tag;   // build tag of the cell lists this step's pair kernel reads (now + bin)
+  int now;   // build tag of the cell lists before this step's contact kernel bins
+  bool build;  // every list of the region is rebuilt this step
+  bool bin;    // the contact kernel bins the particles this step (a rebuild)
 };
 constexpr int kPairList = 0, kPairHash = 1, kPairAll = 2;  // bm_params.pair_mode
 // The search lists of the particles of instance inst; `build` when they are rebuilt this step
@@ -296,13 +299,35 @@ __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_s
   const bool every_step = P.pair_mode == kPairHash;
   if (S.instance == nullptr) {
     const bool build = every_step || S.pair_dirty[0] != 0;
-    return ListRegion{0, int(P.list_buckets - 1), S.pair_builds[0] + (build ? 1 : 0), build};
+    const int now = S.pair_builds[0];
+    return ListRegion{0, int(P.list_buckets - 1), now + (build ? 1 : 0), now, build, build};
   }
   const bool build = every_step || S.pair_dirty[0] != 0 || S.inst_pair_dirty[inst] != 0;
-  return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1,
-                    S.inst_pair_builds[inst] + (build ? 1 : 0), build};
+  const int now = S.inst_pair_builds[inst];
+  return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, now + (build ? 1 : 0), now, build,
+                    build};
 }
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
+                                     bool refreshing);
+// Pair-list refresh (one solver, list mode): a particle that moved more than half the skin
+// from its build position (a mover) takes its current position as its build position and gets
+// a new partner list in the next pair kernel, while every other list keeps its build.
+// refresh_mark holds each particle's refresh time, 2 E + 3 for a refresh at pair epoch E
+// (times only grow), and refresh_state[3] the time of the last full build: a particle
+// refreshed since then is displaced (the cell lists hold it at a stale position, so searches
+// take it from the displaced list instead).  A pair belongs to the list of the particle
+// refreshed last, the smaller index on a tie; rank: its refresh time, or the build time when
+// it is not displaced.  A refresh then never edits another particle's list: the lists that
+// hold a mover skip it from then on.
+__device__ __forceinline__ int refresh_time(const bm_state& S) { return 2 * S.pair_epoch[0] + 3; }
+__device__ __forceinline__ int refresh_rank(const bm_state& S, int32_t q, int built) {
+  return max(S.refresh_mark[q], built);
+}
+// Whether the list of p (rank rp) holds the pair (p, q) (rank rq).
+__device__ __forceinline__ bool owns_pair(int rp, int32_t p, int rq, int32_t q) {
+  return rq < rp || (rq == rp && q > p);
+}
+
 
 __device__ __forceinline__ void bin_particle(const bm_params& P, const bm_state& S, int64_t p, f3 xp,
                                              const ListRegion& L);
@@ -343,21 +368,26 @@ __device__ inline f3 resist_cut(f3 j, f3 xp, f3 vp, f3 origin, f3 tvel, quat q, 
   return j;
 }
 
+__device__ void refresh_list_of(const bm_params& P, const bm_state& S, int32_t a, int time, int built, float reach,
+                                int* group_lds);
+constexpr int kRefreshLanes = 16;  // list_lanes at most this with refresh (LDS of a refresh wave)
+constexpr int kRefreshBlocks = 16;
 // Particles relabelled this step (assign) add their new candidate pairs to the partner lists
 // that the next step's pair kernel uses (add_relabelled_pairs), unless those lists are rebuilt
-// anyway; one wave per relabelled particle: wave `wave` of `waves`.
-__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves) {
+// anyway; one wave per relabelled particle: wave `wave` of `waves`.  Beside the refresh of the
+// movers' lists (refreshing), a mover's pairs are left to its refresh.
+__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves, bool refreshing) {
   if (S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
-  const int count = min(S.changed_count[0], int(P.changed_capacity));
   const int lane = int(threadIdx.x) & 63;
-  for (int i = wave; i < count; i += waves) {
+  const int relabels = min(S.changed_count[0], int(P.changed_capacity));
+  for (int i = wave; i < relabels; i += waves) {
     const int32_t d = S.changed[i];
     ListRegion L = list_region(P, S, instance_of(S, d));
     // A particle relabelled by two knives in one step is listed twice: one walk is enough.
     int seen = 0;
     if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
     seen = __shfl(seen, 0, 64);
-    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
+    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L, refreshing);  // a list rebuilt in full needs nothing
   }
 }
 
@@ -449,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     spd = len(vp);
     if ((mode & kPairPrep) != 0) {
       ListRegion L = list_region(P, S, inst);
-      if (L.build) bin_particle(P, S, p, xp, L);
+      if (L.bin) bin_particle(P, S, p, xp, L);
     }
     total = f3{0.f, 0.f, -9.81f * float(P.mass) * dt};
     // board: plane z = 0 inside its rectangle; penalty_impulse is zero (history zero) unless
@@ -1060,7 +1090,7 @@ __global__ void k_list_bin(const bm_params P, bm_state S) {
   int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
   ListRegion L = list_region(P, S, instance_of(S, p));
-  if (L.build) bin_particle(P, S, p, ld3(S.x, p), L);
+  if (L.bin) bin_particle(P, S, p, ld3(S.x, p), L);
 }
 __global__ void k_list_reach(const bm_params P, bm_state S, int parity_new) {
   // A standalone pair call takes the reach of the speed of the last rigid-contact call.
@@ -1091,6 +1121,18 @@ __device__ void finish_pair_step(const bm_params& P, const bm_state& S) {
   if (built) {
     S.pair_builds[0] += 1;
     S.list_reach[0] = list_reach_of(P, __uint_as_float(S.counters[0]));
+  }
+  if (P.mover_capacity > 0) {
+    // A full build bins every particle afresh: none is displaced any more (this step's marks
+    // included).  Otherwise the step's movers were refreshed (refresh_state[2] counts such
+    // steps).
+    if (all) {
+      S.refresh_state[1] = 0;
+      S.refresh_state[3] = refresh_time(S);
+    } else if (S.refresh_state[0] > 0) {
+      S.refresh_state[2] += 1;
+    }
+    S.refresh_state[0] = 0;
   }
   S.pair_count[0] = int32_t(S.counters[1]);
   S.pair_count[1] = int32_t(S.counters[2]);
@@ -1162,6 +1204,12 @@ struct PairExtras {
 };
 constexpr int kReduceBlocks = 16;
 
+// pair_at for the pair of a and q in index order (the order of the keys and the impulse).
+__device__ __forceinline__ void pair_ordered(const bm_params& P, const bm_state& S, const PairCtx& T, int32_t a,
+                                             int32_t q) {
+  pair_at(P, S, T, min(a, q), max(a, q));
+}
+
 __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S, int parity_old, int topology,
                                                   PairExtras X) {
   if (int(blockIdx.x) >= X.particles) {
@@ -1189,14 +1237,14 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
       S.built_codes[d] = S.codes[d];
     }
   }
-  // This step's largest particle speed, for lists rebuilt this step: the maximum of the
-  // contact kernel's block maxima (rows of rigid_partial), taken by each workgroup that
+  // This step's largest particle speed, for lists rebuilt or refreshed this step: the maximum
+  // of the contact kernel's block maxima (rows of rigid_partial), taken by each workgroup that
   // rebuilds, which keeps it in counters[0] for the step's bookkeeping.
   // A standalone pair call (no rows) finds it in counters[0].
   float build_speed = 0.0f;
   if (P.pair_mode != kPairAll) {
     // Uniform over a single solver's particles; batched instances rebuild their own lists.
-    const int build = S.instance == nullptr ? int(P.pair_mode == kPairHash || S.pair_dirty[0] != 0)
+    const int build = S.instance == nullptr ? int(list_region(P, S, 0).bin)
                                             : __syncthreads_or(live && list_region(P, S, instance_of(S, p)).build);
     if (build && X.rows > 0) {
       __shared__ float wave_top[kBlock / 64];
@@ -1233,6 +1281,11 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   // contact.py:749-761); the others keep the reach of their build.
   float reach = L.build ? list_reach_of(P, build_speed) : S.list_reach[0];
   const float reach2 = reach * reach;
+  // Displaced particles (refreshed lists): lists skip the pairs that another list owns, and
+  // searches take displaced particles from their list.
+  const bool displaced_lists =
+      S.instance == nullptr && P.mover_capacity > 0 && !L.build && S.refresh_state[1] > 0;
+  const int built = displaced_lists ? S.refresh_state[3] : 0;
   float cell = list_cell_size(P), inv = list_cell_inv(P);
   int rings = list_rings(P, reach);
   int base = L.base, mask = L.mask;
@@ -1313,16 +1366,25 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
     count = listed;
   }
   if (count == 0) return;
+  const int rp = displaced_lists ? refresh_rank(S, int32_t(p), built) : 0;
   int parity_new = 1 - parity_old;
   int64_t cap = P.pair_capacity;
   PairCtx T{S.pair_keys + parity_old * cap, S.pair_history + parity_old * cap * 3, S.pair_keys + parity_new * cap,
             S.pair_history + parity_new * cap * 3, S.pair_used + parity_new * cap, filled};
   if (count > 0) {
-    for (int k = 0; k < count; ++k) pair_at(P, S, T, p, S.partners[p * lanes + k]);
+    for (int k = 0; k < count; ++k) {
+      const int32_t q = S.partners[p * lanes + k];
+      if (!displaced_lists || owns_pair(rp, int32_t(p), refresh_rank(S, q, built), q))
+        pair_ordered(P, S, T, int32_t(p), q);
+    }
     return;
   }
   // Overflowing particle: search the buckets of the last build around its build position.
   // In a full build step built_x is being written by this kernel; the build positions are x.
+  // A displaced particle owns its pairs with the particles that are not displaced (any index)
+  // and some of the displaced ones (from their list); a particle that is not displaced owns
+  // none with displaced ones.
+  const bool displaced_p = rp > built;
   f3 xb = ld3(bx, p);
   Cell home = cell_of(xb, inv);
   CellCull cull(xb, home, cell, reach);
@@ -1334,14 +1396,101 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
         int64_t step = 0;
         for (int32_t q = bucket_head(S, base + bucket_of(c, mask), tag); q >= 0 && step < P.n;
              q = S.list_links[q], ++step) {
-          if (q <= p) continue;
+          if (displaced_p ? q == p : q <= p) continue;
+          if (displaced_lists && S.refresh_mark[q] > built) continue;
           Cell cq = cell_of(ld3(bx, q), inv);
           if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
-          pair_at(P, S, T, p, q);
+          pair_ordered(P, S, T, int32_t(p), q);
         }
       }
+  if (displaced_p) {
+    const int nd = min(S.refresh_state[1], int(P.mover_capacity));
+    for (int i = 0; i < nd; ++i) {
+      const int32_t q = S.displaced[i];
+      if (q != p && owns_pair(rp, int32_t(p), refresh_rank(S, q, built), q)) pair_ordered(P, S, T, int32_t(p), q);
+    }
+  }
 }
 __global__ void k_pairs_done(const bm_params P, bm_state S) { finish_pair_step(P, S); }
+
+// Refresh of mover a's partner list around its current position (its new build position), by
+// one workgroup, after the G2P that found it (time: its refresh time, built: the time of the
+// last full build).  The list holds the partners q whose pairs a owns once refreshed (every
+// particle refreshed before it, and the other movers q > a) that lie within reach + e_q of a
+// and are not compatible reference neighbours, where e_q is q's distance from its build
+// position (0 for a mover); more than list_lanes: -1 (a search at every step).  A pair left
+// out then has build positions at least `reach` apart and, with each particle within half the
+// skin of its build position and dt |v| + that distance <= skin / 2 (the staleness check),
+// stays beyond 2 r + dt |v_p - v_q| until either is refreshed or rebuilt again (the Verlet
+// guarantee of a full build at reach >= 2 r + skin).  The candidates come from the cell lists
+// of the last full build, which hold a particle that is not displaced at its build position,
+// within half the skin of its current one (search reach + skin, a thread per cell; at least
+// 2^17 buckets, so the cells searched have distinct buckets), and from the displaced list (the
+// movers included).  group_lds: 1 + list_lanes ints of LDS.
+__device__ void refresh_list_of(const bm_params& P, const bm_state& S, int32_t a, int time, int built, float reach,
+                                int* group_lds) {
+  const ListRegion L = list_region(P, S, 0);
+  const int tid = int(threadIdx.x), nt = int(blockDim.x);
+  const float two_h = 2.0f * float(P.spacing);
+  const f3 xa = ld3(S.x, a), ra = ld3(S.ref, a);
+  const int64_t ca = S.codes[a];
+  const int lanes = int(P.list_lanes);
+  const float search = reach + float(P.pair_skin);
+  const float cell = list_cell_size(P), inv = list_cell_inv(P);
+  const int rings = list_rings(P, search);
+  const Cell home = cell_of(xa, inv);
+  CellCull cull(xa, home, cell, search);
+  const int side = 2 * rings + 1;
+  const int nd = min(S.refresh_state[1], int(P.mover_capacity));
+  int* found = group_lds;
+  int* list = group_lds + 1;
+  if (tid == 0) *found = 0;
+  __syncthreads();
+  auto consider = [&](int32_t q, int mq, f3 xq) {
+    const float eq = mq == time ? 0.0f : len(xq - ld3(S.built_x, q));
+    if (!(len(xq - xa) < reach + eq)) return;
+    if (compatible(ca, S.codes[q]) && len(ra - ld3(S.ref, q)) < two_h) return;
+    const int k = atomicAdd(found, 1);
+    if (k < lanes) list[k] = q;
+  };
+  for (int k = tid; k < side * side * side; k += nt) {
+    const int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
+    if (!cull.near(dx, dy, dz)) continue;
+    const Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+    int64_t step = 0;  // a chain holds each particle at most once
+    for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), L.now); q >= 0 && step < P.n; ++step) {
+      // The next link, the mark and the position of q in one round of loads.
+      const int32_t next = S.list_links[q];
+      const int mq = S.refresh_mark[q];
+      const f3 xq = ld3(S.x, q);
+      if (mq <= built) consider(q, mq, xq);  // displaced particles (a included) come from their list
+      q = next;
+    }
+  }
+  // The displaced list, four entries per thread at a time, their loads issued together.
+  constexpr int kAhead = 4;
+  for (int i0 = tid; i0 < nd; i0 += kAhead * nt) {
+    int32_t qs[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) qs[u] = i0 + u * nt < nd ? S.displaced[i0 + u * nt] : -1;
+    int ms[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) ms[u] = qs[u] >= 0 ? S.refresh_mark[qs[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u)
+      if (qs[u] >= 0 && qs[u] != a && owns_pair(time, a, max(ms[u], built), qs[u])) consider(qs[u], ms[u], ld3(S.x, qs[u]));
+  }
+  __syncthreads();
+  const int n = *found;
+  const int count = n > lanes ? -1 : n;
+  for (int i = tid; i < count; i += nt) S.partners[int64_t(a) * lanes + i] = list[i];
+  if (tid == 0) {
+    S.partner_count[a] = count;
+    S.built_codes[a] = ca;
+    st3(S.built_x, a, xa);
+  }
+  __syncthreads();  // the LDS is taken again for the workgroup's next mover
+}
 
 // After particle d's code changed, adds the pairs (d, q) that pair_at no longer skips to the
 // partner lists, so the lists need no rebuild: q within the list reach of d at the build
@@ -1353,8 +1502,9 @@ __global__ void k_pairs_done(const bm_params P, bm_state S) { finish_pair_step(P
 // knives (a code gains a digit), and the lists keep the Verlet guarantee of the last build.
 // One wave per relabelled particle: the lanes take the cells of the search cube (those near
 // enough to matter) and walk their bucket chains side by side.
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
-  const int tag = L.tag;
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
+                                     bool refreshing) {
+  const int tag = L.now;  // the cell lists binned before this walk (none is binned meanwhile)
   const float reach = S.list_reach[0];
   const float reach2 = reach * reach, two_h = 2.0f * float(P.spacing);
   const float cell = list_cell_size(P), inv = list_cell_inv(P);
@@ -1365,23 +1515,48 @@ __device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int3
   const Cell home = cell_of(xd, inv);
   CellCull cull(xd, home, cell, reach);
   const int side = 2 * rings + 1;
-  for (int k = lane; k < side * side * side; k += 64) {
-    const int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
-    if (!cull.near(dx, dy, dz)) continue;
-    const Cell c = Cell{home.x + dx, home.y + dy, home.z + dz};
+  // Displaced particles (refresh) sit at stale positions in the cell lists: they are taken
+  // from the displaced list (after the cells, lanes over its entries).  A pair goes to the
+  // list of its owner.
+  const bool displaced_lists = S.instance == nullptr && P.mover_capacity > 0;
+  const int nd = displaced_lists ? min(S.refresh_state[1], int(P.mover_capacity)) : 0;
+  const int built = displaced_lists ? S.refresh_state[3] : 0;
+  const int rank_d = displaced_lists ? refresh_rank(S, d, built) : 0;
+  // A mover's list is refreshed beside this walk, with its new pairs.
+  const int mover = refreshing ? refresh_time(S) : -1;
+  if (rank_d == mover) return;
+  const int cube = side * side * side;
+  // Distinct buckets for the cells of a search cube (at least 2^17 buckets, as in the rebuild):
+  // a particle met is in one searched chain only, and needs no cell check.
+  const bool distinct = L.mask >= (1 << 17) - 1;
+  for (int k = lane; k < cube + nd; k += 64) {
+    int32_t first = -1;
+    Cell c = Cell{0, 0, 0};
+    if (k < cube) {
+      const int dx = k / (side * side) - rings, dy = (k / side) % side - rings, dz = k % side - rings;
+      if (!cull.near(dx, dy, dz)) continue;
+      c = Cell{home.x + dx, home.y + dy, home.z + dz};
+      first = bucket_head(S, L.base + bucket_of(c, L.mask), tag);
+    } else {
+      first = S.displaced[k - cube];
+    }
     int64_t step = 0;  // a chain holds each particle at most once
-    for (int32_t q = bucket_head(S, L.base + bucket_of(c, L.mask), tag); q >= 0 && step < P.n;
-         q = S.list_links[q], ++step) {
+    for (int32_t q = first; q >= 0 && step < P.n; q = k < cube ? S.list_links[q] : -1, ++step) {
       if (q == d) continue;
+      const int rq = displaced_lists ? refresh_rank(S, q, built) : 0;
+      if ((k < cube && rq > built) || rq == mover) continue;
       const f3 xq = ld3(S.built_x, q);
-      const Cell cq = cell_of(xq, inv);
-      if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+      if (k < cube && !distinct) {
+        const Cell cq = cell_of(xq, inv);
+        if (cq.x != c.x || cq.y != c.y || cq.z != c.z) continue;
+      }
       const f3 dd = xq - xd;
       if (!(dot(dd, dd) < reach2)) continue;
       const int64_t cq_code = S.codes[q];
       if (compatible(cd, cq_code) && len(rd - ld3(S.ref, q)) < two_h) continue;
       if (q < d && cq_code != S.built_codes[q]) continue;  // q's own walk adds the pair
-      const int32_t owner = min(d, q), other = max(d, q);
+      const bool mine = owns_pair(rank_d, d, rq, q);
+      const int32_t owner = mine ? d : q, other = mine ? q : d;
       int32_t* count = S.partner_count + owner;
       const int32_t* list = S.partners + int64_t(owner) * lanes;
       int32_t n = *count;
@@ -2298,10 +2473,29 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
     st3(S.x, p, xp);
     if (flags & 1) assign_dev(P, S, p, digit, xp, vp);
     if ((flags & 2) && len(xp - ld3(S.built_x, p)) + dt * len(vp) > 0.5f * float(P.pair_skin)) {
-      if (S.instance != nullptr)
+      if (S.instance != nullptr) {
         S.inst_pair_dirty[S.instance[p]] = 1;
-      else
+      } else if (P.mover_capacity > 0) {
+        // A mover: its list is refreshed next step (a mover with dt |v| > skin / 2, or too many
+        // movers or displaced particles, rebuild every list instead).  It takes the next
+        // refresh time and, unless displaced already, joins the displaced list.
+        const bool fast = dt * len(vp) > 0.5f * float(P.pair_skin);
+        const int k = fast ? int(P.mover_capacity) : atomicAdd(S.refresh_state, 1);
+        if (k < P.mover_capacity) {
+          S.movers[k] = int32_t(p);
+          if (atomicExch(S.refresh_mark + p, refresh_time(S)) <= S.refresh_state[3]) {
+            const int j = atomicAdd(S.refresh_state + 1, 1);
+            if (j < P.mover_capacity)
+              S.displaced[j] = int32_t(p);
+            else
+              S.pair_dirty[0] = 1;
+          }
+        } else {
+          S.pair_dirty[0] = 1;
+        }
+      } else {
         S.pair_dirty[0] = 1;
+      }
     }
   }
   __syncthreads();
@@ -2363,23 +2557,37 @@ __device__ void defer_update(const bm_state& S, int block);
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
 // update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
-                                                            int update_blocks, int mark_blocks) {
+                                                            int update_blocks, int mark_blocks, int refresh_blocks) {
   if (int(blockIdx.x) < kMultiBlocks) {
     multi_remapped(P, S, D);
+    return;
+  }
+  if (int(blockIdx.x) < kMultiBlocks + refresh_blocks) {
+    // The movers' list refreshes (one workgroup each), for the next step's pair kernel; these
+    // workgroups start early, beside the node work.
+    if (S.pair_dirty[0] != 0 || S.refresh_state[0] == 0) return;
+    __shared__ int group_lds[1 + kRefreshLanes];
+    const int n = min(S.refresh_state[0], int(P.mover_capacity));
+    // A refreshed list keeps the reach of the last build, at least 2 r + skin, which suffices
+    // while every mover has dt |v| <= skin / 2 (a faster one rebuilds every list).
+    const float reach = S.list_reach[0];
+    const int time = refresh_time(S), built = S.refresh_state[3];
+    for (int i = int(blockIdx.x) - kMultiBlocks; i < n; i += refresh_blocks)
+      refresh_list_of(P, S, S.movers[i], time, built, reach, group_lds);
     return;
   }
   const int node_end = int(gridDim.x) - update_blocks - mark_blocks;
   if (int(blockIdx.x) >= node_end + update_blocks) {
     // The last workgroups: the relabelled particles' new pairs into the partner lists.
     mark_relabelled(P, S, (int(blockIdx.x) - node_end - update_blocks) * (kBlock / 64) + int(threadIdx.x >> 6),
-                    mark_blocks * (kBlock / 64));
+                    mark_blocks * (kBlock / 64), refresh_blocks > 0);
     return;
   }
   if (int(blockIdx.x) >= node_end) {
     defer_update(S, int(blockIdx.x) - node_end);
     return;
   }
-  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
+  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks - refresh_blocks) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
   int64_t N = P.node_count;
@@ -3050,7 +3258,7 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 
 // assign_sides (solver.py:866-890) for knives first .. tools-1: knife k labels digit * 3^k.
 __global__ void __launch_bounds__(kBlock) k_mark_relabelled(const bm_params P, bm_state S) {
-  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64));
+  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64), false);
 }
 __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting,
                                                    int first) {
@@ -3355,7 +3563,10 @@ static bool pair_args_ok(const bm_params* p, const bm_state* s) {
          s->list_heads && s->list_links && s->changed && s->changed_count && s->pair_mark && s->pair_epoch &&
          p->changed_capacity >= 0 && p->pair_capacity > 0 && (p->pair_capacity & (p->pair_capacity - 1)) == 0 &&
          p->list_buckets > 0 && (s->instance != nullptr || (p->list_buckets & (p->list_buckets - 1)) == 0) &&
-         p->list_buckets <= (int64_t(1) << 28) && p->list_lanes >= 1 && p->n < (int64_t(1) << 31);
+         p->list_buckets <= (int64_t(1) << 28) && p->list_lanes >= 1 && p->n < (int64_t(1) << 31) &&
+         (p->mover_capacity <= 0 || (s->movers && s->displaced && s->refresh_mark && s->refresh_state &&
+                                     s->instance == nullptr && p->pair_mode == 0 &&
+                                     p->list_lanes <= kRefreshLanes && p->list_buckets >= (int64_t(1) << 17)));
 }
 static int64_t brick_count(const bm_params* p) { return p->bricks[0] * p->bricks[1] * p->bricks[2]; }
 // Workgroups of a brick kernel: one per brick with particles.
@@ -3419,9 +3630,10 @@ static int brick_threads(const bm_params* p) {
 // workgroups among `blocks`) and the F update, in the instantiation of the material model.
 // `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
-                                   int update_blocks, hipStream_t st, int mark_blocks = 0) {
-  blocks += kMultiBlocks + mark_blocks;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks);
+                                   int update_blocks, hipStream_t st, int mark_blocks = 0, int refresh_blocks = 0) {
+  blocks += kMultiBlocks + mark_blocks + refresh_blocks;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks,
+                     refresh_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
                             int defer, int nbricks, hipStream_t st) {
@@ -3470,14 +3682,18 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   // kernel's last workgroups: one wave per relabelled particle, 64 workgroups of four waves.
   const int mark_blocks = sf.mode < 0 || !(g2p_flags & 1) ? 0
                           : int(p->changed_capacity < 256 ? (p->changed_capacity + 3) / 4 : 64);
+  // In a substep with pair-list refresh, the node kernel's last workgroups refresh the lists
+  // of the particles that G2P found too far from their build positions.
+  const int refresh_blocks = sf.mode >= 0 && (g2p_flags & 2) && p->mover_capacity > 0 ? kRefreshBlocks : 0;
   defer = defer && s->update_pending && s->update_done && s->field_idx_prev;
   if (defer) {
     // The F update is deferred to the next P2G: one extra workgroup of the node kernel sets
     // that up, and no F-update launch follows.
-    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks);
+    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks,
+                           refresh_blocks);
     return;
   }
-  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks);
+  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks, refresh_blocks);
   const int ub = update_blocks(tb);
   launch_update_F(p, s, D, nb < ub ? nb : ub, tb, 0, nb, st);
 }

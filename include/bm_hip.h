@@ -81,6 +81,11 @@ typedef struct bm_params {
   double viscosity;       /* dynamic viscosity (Pa s); > 0 adds viscous_impulse (material.py:321-340) */
   double mu_maxwell;      /* maxwell_fraction * mu */
   double maxwell_relax;   /* 1 - exp(-dt / maxwell_tau) */
+  int64_t mover_capacity; /* > 0: a particle that moves too far from its pair-list build position
+                             (a mover) only refreshes its own list; at most this many movers
+                             per step and displaced particles per full build, else every list is
+                             rebuilt.  0: every list is rebuilt.  One solver in the list mode with
+                             at least 2^17 buckets and list_lanes <= 16 only */
 } bm_params;
 
 typedef struct bm_state {
@@ -192,6 +197,18 @@ typedef struct bm_state {
   int32_t *multi_count;    /* [2] entries, epoch (starts at 1) */
   uint8_t *field_idx_prev; /* [n][8] field slots before a registration that ran while the
                             * particle's F update waited (marked in update_done) */
+  /* Pair-list refresh (mover_capacity > 0).  movers: the particles that moved more than half
+   * the skin from their build positions in this step's G2P; the next pair kernel gives each a
+   * new build position and list.  refresh_mark: a particle's last refresh time (2 epoch + 3 of
+   * the step that marked it; zeros at the start).  A particle refreshed since the last full
+   * build is displaced: the cell lists (of that build) hold it at a stale position, and the
+   * searches take it from `displaced`.  A pair belongs to the list of the particle refreshed
+   * last (the smaller index on a tie).  refresh_state: [0] movers, [1] displaced particles,
+   * [2] refresh steps so far, [3] time of the last full build.  All zeros at the start. */
+  int32_t *movers;         /* [mover_capacity] */
+  int32_t *displaced;      /* [mover_capacity] */
+  int32_t *refresh_mark;   /* [n] */
+  int32_t *refresh_state;  /* [4] */
 } bm_state;
 
 /* Library identification; returns the number of exported stage entry points. */

@@ -73,3 +73,51 @@ def test_search_modes_agree_over_a_cut():
 def test_all_pairs_refuses_large_solvers():
     with pytest.raises(ValueError, match="pair_search='all'"):
         Solver(Config(size=(0.3, 0.15, 0.075), grid_spacing=0.003, pair_search="all"))
+
+
+@pytest.mark.parametrize("capacity", [256, 2])
+def test_refreshed_lists_find_the_contacts_of_full_rebuilds(capacity):
+    """A particle that moves more than half the skin from its build position refreshes only its
+    own partner list (one solver, list mode, at least 2^17 buckets).  The contacts and
+    the motion equal those of a run that rebuilds every list each time instead, also when the
+    movers or displaced particles overflow a small capacity (a full rebuild then)."""
+    cfg = Config(size=(0.096, 0.048, 0.048), grid_spacing=0.003, dt=0.0001, cache_topology=True, pair_search="list")
+
+    class Refresh(Solver):
+        MOVER_CAPACITY = capacity
+
+    class Full(Solver):
+        PAIR_REFRESH = False
+
+    runs = []
+    for cls in (Refresh, Full):
+        s = cls(cfg)
+        assert s.n >= 65536
+        ref = s.reference
+        codes = np.where(ref[:, 0] < 0, 1, 2).astype(np.int64)
+        x = s.x.numpy()
+        x[:, 0] += np.where(codes == 1, 0.0004, -0.0004)  # the halves pressed together across the cut
+        rng = np.random.default_rng(7)
+        v = rng.normal(0, 0.01, (s.n, 3)).astype(np.float32)
+        # A few particles at the cut flung at 2 m/s: they leave their build positions within a
+        # few steps, again and again.
+        fast = np.nonzero((np.abs(ref[:, 0]) < 0.001) & (np.abs(ref[:, 1]) < 0.003))[0][:6]
+        v[fast] = rng.normal(0, 1.2, (len(fast), 3)).astype(np.float32)
+        s.x.assign(x)
+        s.codes.assign(codes)
+        s.v.assign(v)
+        counts, refreshed = [], []
+        for _ in range(40):
+            s.step(cutting=False)
+            counts.append(int(s.pair_count[s.step_count % 2].numpy()[0]))
+            refreshed.append(int(s._t["refresh_state"][2].item()))
+        runs.append((s, counts, refreshed))
+    (a, count_a, refreshed_a), (b, count_b, _) = runs
+    assert a.mover_capacity > 0 and b.mover_capacity == 0
+    assert any(refreshed_a), "no refresh happened"
+    assert int(a.pair_list_builds.numpy()[0]) < int(b.pair_list_builds.numpy()[0])
+    assert max(count_a) > 0
+    assert count_a == count_b
+    for name in ("x", "v"):
+        xa, xb = getattr(a, name).numpy(), getattr(b, name).numpy()
+        assert np.abs(xa - xb).max() <= 1e-5 * np.abs(xb).max(), name
