@@ -307,8 +307,7 @@ __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_s
   return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, now + (build ? 1 : 0), now, build,
                     build};
 }
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
-                                     bool refreshing);
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
 // Pair-list refresh (one solver, list mode): a particle that moved more than half the skin
 // from its build position (a mover) takes its current position as its build position and gets
 // a new partner list in the next pair kernel, while every other list keeps its build.
@@ -370,13 +369,14 @@ __device__ inline f3 resist_cut(f3 j, f3 xp, f3 vp, f3 origin, f3 tvel, quat q, 
 
 __device__ void refresh_list_of(const bm_params& P, const bm_state& S, int32_t a, int time, int built, float reach,
                                 int* group_lds);
-constexpr int kRefreshLanes = 16;  // list_lanes at most this with refresh (LDS of a refresh wave)
+constexpr int kRefreshLanes = 16;  // list_lanes at most this with refresh (LDS of a refresh)
 constexpr int kRefreshBlocks = 16;
+__device__ void refresh_movers(const bm_params& P, const bm_state& S, int block, int blocks);
 // Particles relabelled this step (assign) add their new candidate pairs to the partner lists
 // that the next step's pair kernel uses (add_relabelled_pairs), unless those lists are rebuilt
-// anyway; one wave per relabelled particle: wave `wave` of `waves`.  Beside the refresh of the
-// movers' lists (refreshing), a mover's pairs are left to its refresh.
-__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves, bool refreshing) {
+// anyway; one wave per relabelled particle: wave `wave` of `waves`.  A mover's list, which
+// may take pairs here, is refreshed afterwards (next contact kernel) from scratch.
+__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves) {
   if (S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
   const int lane = int(threadIdx.x) & 63;
   const int relabels = min(S.changed_count[0], int(P.changed_capacity));
@@ -387,7 +387,7 @@ __device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave,
     int seen = 0;
     if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
     seen = __shfl(seen, 0, 64);
-    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L, refreshing);  // a list rebuilt in full needs nothing
+    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
   }
 }
 
@@ -397,8 +397,14 @@ __device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave,
 // the new table and resets its insertion count, bins particles for a pair-list rebuild and
 // clears the field table for a re-registration.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
-                                                  int parity_new) {
-  int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+                                                  int parity_new, int refresh) {
+  // The first `refresh` workgroups refresh the movers' pair lists (refresh_movers).
+  if (int(blockIdx.x) < refresh) {
+    refresh_movers(P, S, int(blockIdx.x), refresh);
+    return;
+  }
+  const int bid = int(blockIdx.x) - refresh;
+  int64_t p = int64_t(bid) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
   // The particle's own loads go first, ahead of the step's rare preparation loops below
   // (whose stores the compiler cannot move them past).
@@ -409,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   const int64_t mask_words = (P.n + 63) >> 6;
   uint64_t may0 = ~0ull, may1 = ~0ull;
   if (S.history_mask != nullptr) {
-    const int64_t word = ((int64_t(blockIdx.x) * blockDim.x) >> 6) + (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6);
+    const int64_t word = ((int64_t(bid) * blockDim.x) >> 6) + (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6);
     if (word < mask_words) {
       may0 = uniform_load(S.history_mask + word);
       may1 = uniform_load(S.history_mask + mask_words + word);
@@ -444,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   if (p == 0 && (mode & kPairPrep) != 0) S.counters[1 + parity_new] = 0u;
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
-  int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t stride = int64_t(int(gridDim.x) - refresh) * blockDim.x;
   if ((mode & kPairPrep) != 0) {
     // clear_pair_slots (contact.py:643-652): empty the slots of the table used two steps ago.
     const int count = uniform_load(S.pair_count + parity_new);
@@ -569,10 +575,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   // particle names the instance of the block reduction below; a wave with a live particle of
   // another instance (at most the waves where instances meet, since they are stored one after
   // the other) adds its lanes' sums directly instead.
-  const int block_inst = S.instance != nullptr ? S.instance[int64_t(blockIdx.x) * blockDim.x] : 0;
+  const int block_inst = S.instance != nullptr ? S.instance[int64_t(bid) * blockDim.x] : 0;
   if (S.instance != nullptr && __ballot(live && inst != block_inst) != 0) {
     float v[10] = {board_j.x, board_j.y, board_j.z, react.x, react.y, react.z, torq.x, torq.y, torq.z, work};
-    float* dst = S.inst_sums + (int64_t(inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10;
+    float* dst = S.inst_sums + (int64_t(inst) * kInstSlots + (bid & (kInstSlots - 1))) * 10;
 #pragma unroll
     for (int k = 0; k < 10; ++k)
       if (v[k] != 0.0f) atomicAdd(dst + k, v[k]);
@@ -609,10 +615,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
     if (S.instance != nullptr && k < 10) {
       // One add per sum and block into a slot of the block's instance (the step's finish reads
       // only the speed of rigid_partial for batched instances).
-      if (acc != 0.0f) atomicAdd(S.inst_sums + (int64_t(block_inst) * kInstSlots + (blockIdx.x & (kInstSlots - 1))) * 10 + k, acc);
+      if (acc != 0.0f) atomicAdd(S.inst_sums + (int64_t(block_inst) * kInstSlots + (bid & (kInstSlots - 1))) * 10 + k, acc);
       acc = 0.0f;
     }
-    S.rigid_partial[int64_t(blockIdx.x) * kRigidSums + k] = acc;
+    S.rigid_partial[int64_t(bid) * kRigidSums + k] = acc;
   }
 }
 
@@ -1492,6 +1498,20 @@ __device__ void refresh_list_of(const bm_params& P, const bm_state& S, int32_t a
   __syncthreads();  // the LDS is taken again for the workgroup's next mover
 }
 
+// The movers' list refreshes, one workgroup per mover (workgroup `block` of `blocks`), in the
+// first workgroups of the contact kernel after the G2P that found them: they run beside the
+// contacts, and the pair kernel that follows uses the new lists.
+__device__ void refresh_movers(const bm_params& P, const bm_state& S, int block, int blocks) {
+  if (S.instance != nullptr || P.mover_capacity <= 0 || S.pair_dirty[0] != 0 || S.refresh_state[0] == 0) return;
+  __shared__ int group_lds[1 + kRefreshLanes];
+  const int n = min(S.refresh_state[0], int(P.mover_capacity));
+  // A refreshed list keeps the reach of the last build, at least 2 r + skin, which suffices
+  // while every mover has dt |v| <= skin / 2 (a faster one rebuilds every list).
+  const float reach = S.list_reach[0];
+  const int time = refresh_time(S), built = S.refresh_state[3];
+  for (int i = block; i < n; i += blocks) refresh_list_of(P, S, S.movers[i], time, built, reach, group_lds);
+}
+
 // After particle d's code changed, adds the pairs (d, q) that pair_at no longer skips to the
 // partner lists, so the lists need no rebuild: q within the list reach of d at the build
 // positions, the pair now a candidate (codes incompatible or references apart).  A pair goes
@@ -1502,8 +1522,7 @@ __device__ void refresh_list_of(const bm_params& P, const bm_state& S, int32_t a
 // knives (a code gains a digit), and the lists keep the Verlet guarantee of the last build.
 // One wave per relabelled particle: the lanes take the cells of the search cube (those near
 // enough to matter) and walk their bucket chains side by side.
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
-                                     bool refreshing) {
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
   const int tag = L.now;  // the cell lists binned before this walk (none is binned meanwhile)
   const float reach = S.list_reach[0];
   const float reach2 = reach * reach, two_h = 2.0f * float(P.spacing);
@@ -1522,9 +1541,6 @@ __device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int3
   const int nd = displaced_lists ? min(S.refresh_state[1], int(P.mover_capacity)) : 0;
   const int built = displaced_lists ? S.refresh_state[3] : 0;
   const int rank_d = displaced_lists ? refresh_rank(S, d, built) : 0;
-  // A mover's list is refreshed beside this walk, with its new pairs.
-  const int mover = refreshing ? refresh_time(S) : -1;
-  if (rank_d == mover) return;
   const int cube = side * side * side;
   // Distinct buckets for the cells of a search cube (at least 2^17 buckets, as in the rebuild):
   // a particle met is in one searched chain only, and needs no cell check.
@@ -1544,7 +1560,7 @@ __device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int3
     for (int32_t q = first; q >= 0 && step < P.n; q = k < cube ? S.list_links[q] : -1, ++step) {
       if (q == d) continue;
       const int rq = displaced_lists ? refresh_rank(S, q, built) : 0;
-      if ((k < cube && rq > built) || rq == mover) continue;
+      if (k < cube && rq > built) continue;
       const f3 xq = ld3(S.built_x, q);
       if (k < cube && !distinct) {
         const Cell cq = cell_of(xq, inv);
@@ -2557,37 +2573,23 @@ __device__ void defer_update(const bm_state& S, int block);
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
 // update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
-                                                            int update_blocks, int mark_blocks, int refresh_blocks) {
+                                                            int update_blocks, int mark_blocks) {
   if (int(blockIdx.x) < kMultiBlocks) {
     multi_remapped(P, S, D);
-    return;
-  }
-  if (int(blockIdx.x) < kMultiBlocks + refresh_blocks) {
-    // The movers' list refreshes (one workgroup each), for the next step's pair kernel; these
-    // workgroups start early, beside the node work.
-    if (S.pair_dirty[0] != 0 || S.refresh_state[0] == 0) return;
-    __shared__ int group_lds[1 + kRefreshLanes];
-    const int n = min(S.refresh_state[0], int(P.mover_capacity));
-    // A refreshed list keeps the reach of the last build, at least 2 r + skin, which suffices
-    // while every mover has dt |v| <= skin / 2 (a faster one rebuilds every list).
-    const float reach = S.list_reach[0];
-    const int time = refresh_time(S), built = S.refresh_state[3];
-    for (int i = int(blockIdx.x) - kMultiBlocks; i < n; i += refresh_blocks)
-      refresh_list_of(P, S, S.movers[i], time, built, reach, group_lds);
     return;
   }
   const int node_end = int(gridDim.x) - update_blocks - mark_blocks;
   if (int(blockIdx.x) >= node_end + update_blocks) {
     // The last workgroups: the relabelled particles' new pairs into the partner lists.
     mark_relabelled(P, S, (int(blockIdx.x) - node_end - update_blocks) * (kBlock / 64) + int(threadIdx.x >> 6),
-                    mark_blocks * (kBlock / 64), refresh_blocks > 0);
+                    mark_blocks * (kBlock / 64));
     return;
   }
   if (int(blockIdx.x) >= node_end) {
     defer_update(S, int(blockIdx.x) - node_end);
     return;
   }
-  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks - refresh_blocks) * blockDim.x + threadIdx.x;
+  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
   int64_t N = P.node_count;
@@ -3258,7 +3260,7 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 
 // assign_sides (solver.py:866-890) for knives first .. tools-1: knife k labels digit * 3^k.
 __global__ void __launch_bounds__(kBlock) k_mark_relabelled(const bm_params P, bm_state S) {
-  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64), false);
+  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64));
 }
 __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting,
                                                    int first) {
@@ -3545,7 +3547,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   if (p->rigid_blocks < blocks_for(p->n)) return -2;
   int nb = blocks_for(p->n);
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
-                     int(kAccumulate), 0);
+                     int(kAccumulate), 0, 0);
   if (p->tools > 1)
     hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
@@ -3630,10 +3632,9 @@ static int brick_threads(const bm_params* p) {
 // workgroups among `blocks`) and the F update, in the instantiation of the material model.
 // `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
-                                   int update_blocks, hipStream_t st, int mark_blocks = 0, int refresh_blocks = 0) {
-  blocks += kMultiBlocks + mark_blocks + refresh_blocks;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks,
-                     refresh_blocks);
+                                   int update_blocks, hipStream_t st, int mark_blocks = 0) {
+  blocks += kMultiBlocks + mark_blocks;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
                             int defer, int nbricks, hipStream_t st) {
@@ -3682,18 +3683,14 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   // kernel's last workgroups: one wave per relabelled particle, 64 workgroups of four waves.
   const int mark_blocks = sf.mode < 0 || !(g2p_flags & 1) ? 0
                           : int(p->changed_capacity < 256 ? (p->changed_capacity + 3) / 4 : 64);
-  // In a substep with pair-list refresh, the node kernel's last workgroups refresh the lists
-  // of the particles that G2P found too far from their build positions.
-  const int refresh_blocks = sf.mode >= 0 && (g2p_flags & 2) && p->mover_capacity > 0 ? kRefreshBlocks : 0;
   defer = defer && s->update_pending && s->update_done && s->field_idx_prev;
   if (defer) {
     // The F update is deferred to the next P2G: one extra workgroup of the node kernel sets
     // that up, and no F-update launch follows.
-    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks,
-                           refresh_blocks);
+    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks);
     return;
   }
-  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks, refresh_blocks);
+  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks);
   const int ub = update_blocks(tb);
   launch_update_F(p, s, D, nb < ub ? nb : ub, tb, 0, nb, st);
 }
@@ -3832,7 +3829,11 @@ static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int3
   const int insert_blocks = int(p->n < 512 * kBlock ? blocks_for(p->n) : 512);
   // 1. contacts with the knife and the board, plus the step's preparation: empty the new
   //    pair table's slots, bin particles for a stale pair list, clear a stale field table.
-  hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0, mode, new_p);
+  // Its first workgroups refresh the pair lists of the last G2P's movers (exiting at once
+  // without movers).
+  const int refresh_blocks = p->mover_capacity > 0 ? kRefreshBlocks : 0;
+  hipLaunchKernelGGL(k_rigid, dim3(refresh_blocks + nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0,
+                     mode, new_p, refresh_blocks);
   // Knives 1 .. tools-1: their contacts after knife 0's, their sums and tool steps after
   // knife 0's step.
   if (p->tools > 1)
