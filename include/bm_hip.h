@@ -198,7 +198,7 @@ typedef struct bm_state {
   uint8_t *field_idx_prev; /* [n][8] field slots before a registration that ran while the
                             * particle's F update waited (marked in update_done) */
   /* Pair-list refresh (mover_capacity > 0).  movers: the particles that moved more than half
-   * the skin from their build positions in this step's G2P; the next pair kernel gives each a
+   * the skin from their build positions in this step's G2P; the next contact kernel gives each a
    * new build position and list.  refresh_mark: a particle's last refresh time (2 epoch + 3 of
    * the step that marked it; zeros at the start).  A particle refreshed since the last full
    * build is displaced: the cell lists (of that build) hold it at a stale position, and the
