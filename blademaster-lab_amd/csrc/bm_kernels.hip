@@ -307,7 +307,8 @@ __device__ __forceinline__ ListRegion list_region(const bm_params& P, const bm_s
   return ListRegion{S.inst_buckets[2 * inst], S.inst_buckets[2 * inst + 1] - 1, now + (build ? 1 : 0), now, build,
                     build};
 }
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L);
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
+                                     bool refreshing);
 // Pair-list refresh (one solver, list mode): a particle that moved more than half the skin
 // from its build position (a mover) takes its current position as its build position and gets
 // a new partner list in the next pair kernel, while every other list keeps its build.
@@ -374,9 +375,9 @@ constexpr int kRefreshBlocks = 16;
 __device__ void refresh_movers(const bm_params& P, const bm_state& S, int block, int blocks);
 // Particles relabelled this step (assign) add their new candidate pairs to the partner lists
 // that the next step's pair kernel uses (add_relabelled_pairs), unless those lists are rebuilt
-// anyway; one wave per relabelled particle: wave `wave` of `waves`.  A mover's list, which
-// may take pairs here, is refreshed afterwards (next contact kernel) from scratch.
-__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves) {
+// anyway; one wave per relabelled particle: wave `wave` of `waves`.  Beside the refresh of the
+// movers' lists (refreshing), a mover's pairs are left to its refresh.
+__device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave, int waves, bool refreshing) {
   if (S.pair_dirty[0] != 0 || P.pair_mode != kPairList) return;
   const int lane = int(threadIdx.x) & 63;
   const int relabels = min(S.changed_count[0], int(P.changed_capacity));
@@ -387,7 +388,7 @@ __device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave,
     int seen = 0;
     if (lane == 0) seen = atomicExch(S.pair_mark + d, S.pair_epoch[0] + 1) == S.pair_epoch[0] + 1;
     seen = __shfl(seen, 0, 64);
-    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L);  // a list rebuilt in full needs nothing
+    if (!L.build && !seen) add_relabelled_pairs(P, S, d, lane, L, refreshing);  // a list rebuilt in full needs nothing
   }
 }
 
@@ -397,13 +398,19 @@ __device__ void mark_relabelled(const bm_params& P, const bm_state& S, int wave,
 // the new table and resets its insertion count, bins particles for a pair-list rebuild and
 // clears the field table for a re-registration.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) k_rigid(const bm_params P, const bm_state S, int cut_bits, int mode,
-                                                  int parity_new, int refresh) {
-  // The first `refresh` workgroups refresh the movers' pair lists (refresh_movers).
+                                                  int parity_new, int refresh, int marks) {
+  // The first `refresh` workgroups refresh the movers' pair lists (refresh_movers), the next
+  // `marks` add the last G2P's relabelled particles' new pairs to the lists (mark_relabelled).
   if (int(blockIdx.x) < refresh) {
     refresh_movers(P, S, int(blockIdx.x), refresh);
     return;
   }
-  const int bid = int(blockIdx.x) - refresh;
+  if (int(blockIdx.x) < refresh + marks) {
+    mark_relabelled(P, S, (int(blockIdx.x) - refresh) * (kBlock / 64) + int(threadIdx.x >> 6), marks * (kBlock / 64),
+                    refresh > 0);
+    return;
+  }
+  const int bid = int(blockIdx.x) - refresh - marks;
   int64_t p = int64_t(bid) * blockDim.x + threadIdx.x;
   bool live = p < P.n;
   // The particle's own loads go first, ahead of the step's rare preparation loops below
@@ -450,7 +457,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 
   if (p == 0 && (mode & kPairPrep) != 0) S.counters[1 + parity_new] = 0u;
   f3 board_j = f3{0.f, 0.f, 0.f}, react = board_j, torq = board_j;
   float work = 0.0f, spd = 0.0f;
-  int64_t stride = int64_t(int(gridDim.x) - refresh) * blockDim.x;
+  int64_t stride = int64_t(int(gridDim.x) - refresh - marks) * blockDim.x;
   if ((mode & kPairPrep) != 0) {
     // clear_pair_slots (contact.py:643-652): empty the slots of the table used two steps ago.
     const int count = uniform_load(S.pair_count + parity_new);
@@ -1299,8 +1306,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs(const bm_params P, bm_state S,
   int count;
   bool dirty = L.build;
   // A rebuild uses the current positions.  Relabelled particles do not rebuild: the contact
-  // kernel added their new pairs to the lists (add_relabelled_pairs); their build codes are
-  // brought up to date at the start of this kernel.
+  // kernel's first workgroups added their new pairs to the lists (add_relabelled_pairs); their
+  // build codes are brought up to date at the start of this kernel.
   const bool rebuild = dirty;
   const float* bx = dirty ? S.x : S.built_x;
   if (rebuild) {
@@ -1522,7 +1529,8 @@ __device__ void refresh_movers(const bm_params& P, const bm_state& S, int block,
 // knives (a code gains a digit), and the lists keep the Verlet guarantee of the last build.
 // One wave per relabelled particle: the lanes take the cells of the search cube (those near
 // enough to matter) and walk their bucket chains side by side.
-__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L) {
+__device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int32_t d, int lane, const ListRegion& L,
+                                     bool refreshing) {
   const int tag = L.now;  // the cell lists binned before this walk (none is binned meanwhile)
   const float reach = S.list_reach[0];
   const float reach2 = reach * reach, two_h = 2.0f * float(P.spacing);
@@ -1541,6 +1549,9 @@ __device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int3
   const int nd = displaced_lists ? min(S.refresh_state[1], int(P.mover_capacity)) : 0;
   const int built = displaced_lists ? S.refresh_state[3] : 0;
   const int rank_d = displaced_lists ? refresh_rank(S, d, built) : 0;
+  // A mover's list is refreshed beside this walk, with its new pairs.
+  const int mover = refreshing && displaced_lists ? refresh_time(S) : -1;
+  if (rank_d == mover) return;
   const int cube = side * side * side;
   // Distinct buckets for the cells of a search cube (at least 2^17 buckets, as in the rebuild):
   // a particle met is in one searched chain only, and needs no cell check.
@@ -1560,7 +1571,7 @@ __device__ void add_relabelled_pairs(const bm_params& P, const bm_state& S, int3
     for (int32_t q = first; q >= 0 && step < P.n; q = k < cube ? S.list_links[q] : -1, ++step) {
       if (q == d) continue;
       const int rq = displaced_lists ? refresh_rank(S, q, built) : 0;
-      if (k < cube && rq > built) continue;
+      if ((k < cube && rq > built) || rq == mover) continue;
       const f3 xq = ld3(S.built_x, q);
       if (k < cube && !distinct) {
         const Cell cq = cell_of(xq, inv);
@@ -2573,18 +2584,12 @@ __device__ void defer_update(const bm_state& S, int block);
 // With update_blocks > 0 the last update_blocks workgroups instead defer this substep's F
 // update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
-                                                            int update_blocks, int mark_blocks) {
+                                                            int update_blocks) {
   if (int(blockIdx.x) < kMultiBlocks) {
     multi_remapped(P, S, D);
     return;
   }
-  const int node_end = int(gridDim.x) - update_blocks - mark_blocks;
-  if (int(blockIdx.x) >= node_end + update_blocks) {
-    // The last workgroups: the relabelled particles' new pairs into the partner lists.
-    mark_relabelled(P, S, (int(blockIdx.x) - node_end - update_blocks) * (kBlock / 64) + int(threadIdx.x >> 6),
-                    mark_blocks * (kBlock / 64));
-    return;
-  }
+  const int node_end = int(gridDim.x) - update_blocks;
   if (int(blockIdx.x) >= node_end) {
     defer_update(S, int(blockIdx.x) - node_end);
     return;
@@ -3260,7 +3265,7 @@ __device__ void assign_dev(const bm_params& P, const bm_state& S, int64_t p, int
 
 // assign_sides (solver.py:866-890) for knives first .. tools-1: knife k labels digit * 3^k.
 __global__ void __launch_bounds__(kBlock) k_mark_relabelled(const bm_params P, bm_state S) {
-  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64));
+  mark_relabelled(P, S, int(blockIdx.x) * (kBlock / 64) + int(threadIdx.x >> 6), int(gridDim.x) * (kBlock / 64), false);
 }
 __global__ void __launch_bounds__(kBlock) k_assign(const bm_params P, bm_state S, int64_t digit, int cutting,
                                                    int first) {
@@ -3547,7 +3552,7 @@ int bm_rigid_contacts(const bm_params* p, const bm_state* s, int32_t cut_bits, v
   if (p->rigid_blocks < blocks_for(p->n)) return -2;
   int nb = blocks_for(p->n);
   hipLaunchKernelGGL(k_rigid, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits),
-                     int(kAccumulate), 0, 0);
+                     int(kAccumulate), 0, 0, 0);
   if (p->tools > 1)
     hipLaunchKernelGGL(k_extra_blades, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, *p, *s, int(cut_bits));
   hipLaunchKernelGGL(k_prep, dim3(1), dim3(kPrepBlock), 0, (hipStream_t)stream, *p, *s, nb, int(kAccumulate),
@@ -3632,9 +3637,9 @@ static int brick_threads(const bm_params* p) {
 // workgroups among `blocks`) and the F update, in the instantiation of the material model.
 // `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
-                                   int update_blocks, hipStream_t st, int mark_blocks = 0) {
-  blocks += kMultiBlocks + mark_blocks;
-  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks, mark_blocks);
+                                   int update_blocks, hipStream_t st) {
+  blocks += kMultiBlocks;
+  hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
                             int defer, int nbricks, hipStream_t st) {
@@ -3679,18 +3684,14 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
   if (p->tools > 1 && (g2p_flags & 1))
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, *p, *s, digit, 1, 1);
   const int node_blocks = int(blocks_for(node_launch(p)));
-  // In a substep the relabelled particles' new pairs go to the partner lists in the node
-  // kernel's last workgroups: one wave per relabelled particle, 64 workgroups of four waves.
-  const int mark_blocks = sf.mode < 0 || !(g2p_flags & 1) ? 0
-                          : int(p->changed_capacity < 256 ? (p->changed_capacity + 3) / 4 : 64);
   defer = defer && s->update_pending && s->update_done && s->field_idx_prev;
   if (defer) {
     // The F update is deferred to the next P2G: one extra workgroup of the node kernel sets
     // that up, and no F-update launch follows.
-    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st, mark_blocks);
+    launch_couple_remapped(p, s, D, node_blocks + kDeferredUpdateBlocks, kDeferredUpdateBlocks, st);
     return;
   }
-  launch_couple_remapped(p, s, D, node_blocks, 0, st, mark_blocks);
+  launch_couple_remapped(p, s, D, node_blocks, 0, st);
   const int ub = update_blocks(tb);
   launch_update_F(p, s, D, nb < ub ? nb : ub, tb, 0, nb, st);
 }
@@ -3829,11 +3830,14 @@ static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int3
   const int insert_blocks = int(p->n < 512 * kBlock ? blocks_for(p->n) : 512);
   // 1. contacts with the knife and the board, plus the step's preparation: empty the new
   //    pair table's slots, bin particles for a stale pair list, clear a stale field table.
-  // Its first workgroups refresh the pair lists of the last G2P's movers (exiting at once
-  // without movers).
+  // Its first workgroups refresh the pair lists of the last G2P's movers, the next add the
+  // pairs of the particles it relabelled (one wave per particle, 64 workgroups of four
+  // waves); both exit at once when there is nothing to do.
   const int refresh_blocks = p->mover_capacity > 0 ? kRefreshBlocks : 0;
-  hipLaunchKernelGGL(k_rigid, dim3(refresh_blocks + nb), dim3(kBlock), 0, st, *p, *s, cutting ? int(cut_bits) : 0,
-                     mode, new_p, refresh_blocks);
+  const int mark_blocks =
+      p->pair_mode != 0 ? 0 : int(p->changed_capacity < 256 ? (p->changed_capacity + 3) / 4 : 64);
+  hipLaunchKernelGGL(k_rigid, dim3(refresh_blocks + mark_blocks + nb), dim3(kBlock), 0, st, *p, *s,
+                     cutting ? int(cut_bits) : 0, mode, new_p, refresh_blocks, mark_blocks);
   // Knives 1 .. tools-1: their contacts after knife 0's, their sums and tool steps after
   // knife 0's step.
   if (p->tools > 1)
@@ -3844,8 +3848,9 @@ static int substep(const bm_params* p, const bm_state* s, int32_t cut_bits, int3
                      PairExtras{nb, kReduceBlocks, insert_blocks, nb});
   // 3. transfers; the node kernel's last workgroup applies the contact sums and moves the
   //    knife (its update reads only this step's rigid reaction, solver.py:824-864), which
-  //    G2P's labelling reads; G2P labels the moved particles and checks the pair list's skin,
-  //    and the last node kernel adds the relabelled particles' new pairs to the lists.
+  //    G2P's labelling reads; G2P labels the moved particles and checks the pair list's skin
+  //    (the next contact kernel adds the relabelled particles' new pairs to the lists and
+  //    refreshes the movers' lists).
   launch_transfers(p, s, 3, digit, (cutting ? 1 : 0) | 2, st, defer,
                    StepFinish{mode, nb, ToolTarget{targets, rotations, int(index)}});
   if (external && p->blade_hold) hipLaunchKernelGGL(k_finish_external, dim3(1), dim3(1), 0, st, *p, *s);
