@@ -121,3 +121,52 @@ def test_refreshed_lists_find_the_contacts_of_full_rebuilds(capacity):
     for name in ("x", "v"):
         xa, xb = getattr(a, name).numpy(), getattr(b, name).numpy()
         assert np.abs(xa - xb).max() <= 1e-5 * np.abs(xb).max(), name
+
+
+def test_refreshes_beside_relabels_match_full_rebuilds():
+    """Relabels (recorded as assign records them) and list refreshes in the same substeps: the
+    relabel walks leave the movers' pairs to the movers' refreshes, which run beside them.  The
+    contacts and the motion equal those of a run that rebuilds every list instead of refreshing."""
+    cfg = Config(size=(0.096, 0.048, 0.048), grid_spacing=0.003, dt=0.0001, cache_topology=True, pair_search="list")
+
+    class Full(Solver):
+        PAIR_REFRESH = False
+
+    runs = []
+    for cls in (Solver, Full):
+        s = cls(cfg)
+        ref = s.reference
+        codes = np.where(ref[:, 0] < 0, 1, 2).astype(np.int64)
+        x = s.x.numpy()
+        x[:, 0] += np.where(codes == 1, 0.0004, -0.0004)
+        rng = np.random.default_rng(11)
+        v = rng.normal(0, 0.01, (s.n, 3)).astype(np.float32)
+        fast = np.nonzero((np.abs(ref[:, 0]) < 0.001) & (np.abs(ref[:, 1]) < 0.003))[0][:6]
+        v[fast] = rng.normal(0, 1.2, (len(fast), 3)).astype(np.float32)
+        s.x.assign(x)
+        s.codes.assign(codes)
+        s.v.assign(v)
+        T = s._t
+        dev = T["codes"].device
+        counts = []
+        for step in range(36):
+            if step in (6, 14, 22, 30):
+                # Relabel a strip of the right half next to the cut (and next to the fast
+                # particles) to the left half's code, as assign does in G2P.
+                lo = 0.0005 + 0.0015 * (step // 8)
+                ids = np.nonzero((ref[:, 0] > lo) & (ref[:, 0] < lo + 0.0016) & (np.abs(ref[:, 1]) < 0.006))[0]
+                ids = ids.astype(np.int32)
+                T["codes"][torch.from_numpy(ids).long().to(dev)] = 1
+                T["changed"][: len(ids)] = torch.from_numpy(ids).to(dev)
+                T["changed_count"].fill_(len(ids))
+                T["topology_dirty"].fill_(2)
+            s.step(cutting=False)
+            counts.append(int(s.pair_count[s.step_count % 2].numpy()[0]))
+        runs.append((s, counts))
+    (a, count_a), (b, count_b) = runs
+    assert a.mover_capacity > 0 and int(a._t["refresh_state"][2].item()) > 0, "no refresh happened"
+    assert max(count_a) > 0
+    assert count_a == count_b
+    for name in ("x", "v"):
+        xa, xb = getattr(a, name).numpy(), getattr(b, name).numpy()
+        assert np.abs(xa - xb).max() <= 1e-5 * np.abs(xb).max(), name
