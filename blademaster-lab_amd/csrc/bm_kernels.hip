@@ -2080,7 +2080,7 @@ __device__ __forceinline__ void node_coords(const bm_params& P, const Divisors& 
   nz = int(rem - int64_t(ny) * dz);
 }
 
-// Nodes of 2 .. kMultiFields fields are exchanged by the first kMultiBlocks workgroups of
+// Nodes of 2 .. kMultiFields fields are exchanged by the first multi_blocks(P) workgroups of
 // each node kernel: four lanes per listed node, one per field.  The lanes load the list entry
 // (with the count), then their field's window sums, code and mass in one round, and trade
 // them by cross-lane reads within the four.  These workgroups are dispatched first and run
@@ -2089,6 +2089,11 @@ __device__ __forceinline__ void node_coords(const bm_params& P, const Divisors& 
 // chain, not bandwidth, sets the kernel's duration.
 constexpr int kMultiBlocks = 8;
 constexpr int kMultiFields = 4;
+// Batched solvers, whose many cuts give far more multi-field nodes, take eight times as many
+// of these workgroups (eight had cost the batched sweep a quarter of its throughput).
+__host__ __device__ __forceinline__ int multi_blocks(const bm_params& P) {
+  return P.instances > 1 ? 8 * kMultiBlocks : kMultiBlocks;
+}
 
 __device__ __forceinline__ bool multi_listed(const bm_state& S, int nf) {
   return S.multi_nodes != nullptr && nf >= 2 && nf <= kMultiFields;
@@ -2104,8 +2109,8 @@ __device__ void multi_velocity(const bm_params& P, const bm_state& S, const Divi
   if (S.multi_nodes == nullptr) return;
   const int f = int(threadIdx.x) & 3, base = int(threadIdx.x) & 60;
   const int64_t N = P.node_count;
-  constexpr int kStride = kMultiBlocks * kBlock / 4;
-  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += kStride) {
+  const int stride = multi_blocks(P) * kBlock / 4;
+  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += stride) {
     const int count = S.multi_count[0];
     const int64_t node = q < N ? S.multi_nodes[q] : 0;  // in bounds whatever the count
     if (q >= count) return;
@@ -2162,8 +2167,8 @@ __device__ void multi_remapped(const bm_params& P, const bm_state& S, const Divi
   if (S.multi_nodes == nullptr) return;
   const int f = int(threadIdx.x) & 3, base = int(threadIdx.x) & 60;
   const int64_t N = P.node_count;
-  constexpr int kStride = kMultiBlocks * kBlock / 4;
-  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += kStride) {
+  const int stride = multi_blocks(P) * kBlock / 4;
+  for (int q = (int(blockIdx.x) * kBlock + int(threadIdx.x)) >> 2;; q += stride) {
     const int count = S.multi_count[0];
     const int64_t node = q < N ? S.multi_nodes[q] : 0;
     if (q >= count) return;
@@ -2233,11 +2238,11 @@ __global__ void __launch_bounds__(kBlock) k_couple_velocity(const bm_params P, b
     finish_rigid_step(P, S, part[0], SF.mode, SF.tt);
     return;
   }
-  if (int(blockIdx.x) < kMultiBlocks) {
+  if (int(blockIdx.x) < multi_blocks(P)) {
     multi_velocity(P, S, D);
     return;
   }
-  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
+  int64_t i = int64_t(int(blockIdx.x) - multi_blocks(P)) * blockDim.x + threadIdx.x;
   // Every P2G block has read update_pending by now.
   if (i == 0 && S.update_pending != nullptr) S.update_pending[0] = 0;
   if (i >= active_node_count(P)) return;
@@ -2585,7 +2590,7 @@ __device__ void defer_update(const bm_state& S, int block);
 // update to the next P2G (defer_update), so no separate F-update launch follows.
 __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, bm_state S, const Divisors D,
                                                             int update_blocks) {
-  if (int(blockIdx.x) < kMultiBlocks) {
+  if (int(blockIdx.x) < multi_blocks(P)) {
     multi_remapped(P, S, D);
     return;
   }
@@ -2594,7 +2599,7 @@ __global__ void __launch_bounds__(kBlock) k_couple_remapped(const bm_params P, b
     defer_update(S, int(blockIdx.x) - node_end);
     return;
   }
-  int64_t i = int64_t(int(blockIdx.x) - kMultiBlocks) * blockDim.x + threadIdx.x;
+  int64_t i = int64_t(int(blockIdx.x) - multi_blocks(P)) * blockDim.x + threadIdx.x;
   if (i >= active_node_count(P)) return;
   int64_t node = active_node(S, i);
   int64_t N = P.node_count;
@@ -3638,7 +3643,7 @@ static int brick_threads(const bm_params* p) {
 // `blocks` counts the node and update workgroups; the multi-field workgroups come first.
 static void launch_couple_remapped(const bm_params* p, const bm_state* s, const Divisors& D, int blocks,
                                    int update_blocks, hipStream_t st) {
-  blocks += kMultiBlocks;
+  blocks += multi_blocks(*p);
   hipLaunchKernelGGL(k_couple_remapped, dim3(blocks), dim3(kBlock), 0, st, *p, *s, D, update_blocks);
 }
 static void launch_update_F(const bm_params* p, const bm_state* s, const Divisors& D, int blocks, int threads,
@@ -3665,7 +3670,7 @@ static void launch_transfers(const bm_params* p, const bm_state* s, int p2g_flag
     hipLaunchKernelGGL(k_p2g<true>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
   else
     hipLaunchKernelGGL(k_p2g<false>, dim3(nb), dim3(tb), lds_bytes(p, 7, 7), st, *p, *s, p2g_flags, D);
-  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p)) + (sf.mode >= 0 ? 1 : 0)),
+  hipLaunchKernelGGL(k_couple_velocity, dim3(multi_blocks(*p) + blocks_for(node_launch(p)) + (sf.mode >= 0 ? 1 : 0)),
                      dim3(kBlock), 0, st, *p, *s, D, sf);
   // Knives 1 .. tools-1: their sums and tool steps after knife 0's.
   if (sf.mode >= 0 && p->tools > 1)
@@ -3736,7 +3741,7 @@ int bm_p2g(const bm_params* p, const bm_state* s, void* stream) {
 
 int bm_couple_and_velocity(const bm_params* p, const bm_state* s, void* stream) {
   if (!brick_args_ok(p, s)) return -1;
-  hipLaunchKernelGGL(k_couple_velocity, dim3(kMultiBlocks + blocks_for(node_launch(p))), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_couple_velocity, dim3(multi_blocks(*p) + blocks_for(node_launch(p))), dim3(kBlock), 0,
                      (hipStream_t)stream, *p, *s, divisors_of(p), StepFinish{-1, 0, ToolTarget{nullptr, nullptr, 0}});
   return launch_status();
 }
